@@ -1,0 +1,286 @@
+"""ctypes binding of libalens.so (include/alens_capi.h).
+
+This is the Python face of the C ABI: it loads the in-tree library built by alens_amd.build and
+raises if it is missing -- there is no CPU fallback.  Array conventions follow the header: rods are
+given in caller order (center/dir as (n, 3) float64), constraints come back in canonical order
+(ascending rod index I, then J).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libalens.so")
+
+ALENS_OK, ALENS_E_ARG, ALENS_E_HIP, ALENS_E_OOM, ALENS_E_NONFINITE, ALENS_E_NOCONV, ALENS_E_BOX, ALENS_E_STATE = range(8)
+STATUS_NAMES = {0: "OK", 1: "E_ARG", 2: "E_HIP", 3: "E_OOM", 4: "E_NONFINITE", 5: "E_NOCONV", 6: "E_BOX",
+                7: "E_STATE"}
+
+# every entry point declared in include/alens_capi.h
+EXPORTS = [
+    "alens_create", "alens_destroy", "alens_last_error", "alens_version", "alens_set_rods", "alens_get_rods",
+    "alens_set_step", "alens_get_step", "alens_step_constraints", "alens_step_euler",
+    "alens_get_num_constraints", "alens_get_constraints", "alens_get_constraint_arrays", "alens_get_gamma",
+    "alens_get_velocity", "alens_bcqp_solve",
+]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("box_lo", ctypes.c_double * 3),
+        ("box_hi", ctypes.c_double * 3),
+        ("pbc", ctypes.c_int32 * 3),
+        ("monolayer", ctypes.c_int32),
+        ("col_buf", ctypes.c_double),
+        ("viscosity", ctypes.c_double),
+        ("kT", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("tol", ctypes.c_double),
+        ("diam_col_ratio", ctypes.c_double),
+        ("len_col_ratio", ctypes.c_double),
+        ("cell_size", ctypes.c_double),
+        ("max_ite", ctypes.c_int32),
+        ("solver", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("warm_start", ctypes.c_int32),
+        ("seed", ctypes.c_uint64),
+        ("device", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class ConBlock(ctypes.Structure):
+    _fields_ = [
+        ("delta0", ctypes.c_double), ("gamma", ctypes.c_double), ("kappa", ctypes.c_double),
+        ("gamma_lb", ctypes.c_double),
+        ("gid_i", ctypes.c_int32), ("gid_j", ctypes.c_int32), ("gidx_i", ctypes.c_int32), ("gidx_j", ctypes.c_int32),
+        ("norm_i", ctypes.c_double * 3), ("norm_j", ctypes.c_double * 3), ("pos_i", ctypes.c_double * 3),
+        ("pos_j", ctypes.c_double * 3), ("lab_i", ctypes.c_double * 3), ("lab_j", ctypes.c_double * 3),
+        ("stress", ctypes.c_double * 9),
+        ("one_side", ctypes.c_int32), ("bilateral", ctypes.c_int32),
+    ]
+
+
+class Report(ctypes.Structure):
+    _fields_ = [
+        ("iters", ctypes.c_int32), ("matvecs", ctypes.c_int32), ("converged", ctypes.c_int32),
+        ("fallback_used", ctypes.c_int32), ("status", ctypes.c_int32), ("iters_fp32", ctypes.c_int32),
+        ("n_con", ctypes.c_int64), ("n_entries", ctypes.c_int64),
+        ("res0", ctypes.c_double), ("res", ctypes.c_double),
+        ("t_collect_ms", ctypes.c_double), ("t_solve_ms", ctypes.c_double), ("t_step_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class AlensError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+
+def lib():
+    """Load libalens.so; raises if it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m alens_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    L.alens_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(vp)]
+    L.alens_destroy.argtypes = [vp]
+    L.alens_destroy.restype = None
+    L.alens_last_error.argtypes = [vp]
+    L.alens_last_error.restype = ctypes.c_char_p
+    L.alens_version.restype = ctypes.c_char_p
+    L.alens_set_rods.argtypes = [vp, ctypes.c_int32, _ip, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_uint8)]
+    L.alens_get_rods.argtypes = [vp, _dp, _dp]
+    L.alens_set_step.argtypes = [vp, ctypes.c_int64]
+    L.alens_get_step.argtypes = [vp]
+    L.alens_get_step.restype = ctypes.c_int64
+    L.alens_step_constraints.argtypes = [vp, _dp, ctypes.POINTER(Report)]
+    L.alens_step_euler.argtypes = [vp]
+    L.alens_get_num_constraints.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+    L.alens_get_constraints.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(ConBlock), ctypes.POINTER(ctypes.c_int64)]
+    L.alens_get_constraint_arrays.argtypes = [vp, _ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.alens_get_gamma.argtypes = [vp, _dp]
+    L.alens_get_velocity.argtypes = [vp, _dp, _dp, _dp, _dp]
+    L.alens_bcqp_solve.argtypes = [vp, ctypes.c_int32, ctypes.c_int64, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                   ctypes.POINTER(Report)]
+    for name in EXPORTS:
+        fn = getattr(L, name)
+        if name not in ("alens_destroy", "alens_last_error", "alens_version", "alens_get_step"):
+            fn.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _p(a, ct):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+def make_config(box_lo=(0, 0, 0), box_hi=(1, 1, 1), pbc=(1, 1, 1), monolayer=0, col_buf=0.025, viscosity=0.01,
+                kT=0.00411, dt=1e-4, tol=1e-5, diam_col_ratio=1.0, len_col_ratio=1.0, cell_size=0.0,
+                max_ite=100000, solver=0, precision=0, warm_start=0, seed=1234, device=0, **_ignored):
+    c = Config()
+    c.box_lo[:] = [float(v) for v in box_lo]
+    c.box_hi[:] = [float(v) for v in box_hi]
+    c.pbc[:] = [int(v) for v in pbc]
+    c.monolayer = int(monolayer)
+    c.col_buf = float(col_buf)
+    c.viscosity = float(viscosity)
+    c.kT = float(kT)
+    c.dt = float(dt)
+    c.tol = float(tol)
+    c.diam_col_ratio = float(diam_col_ratio)
+    c.len_col_ratio = float(len_col_ratio)
+    c.cell_size = float(cell_size)
+    c.max_ite = int(max_ite)
+    c.solver = int(solver)
+    c.precision = int(precision)
+    c.warm_start = int(warm_start)
+    c.seed = int(seed)
+    c.device = int(device)
+    return c
+
+
+class Context:
+    """One rod system on one GPU (mirrors the lifetime of SimToolbox's SylinderSystem)."""
+
+    def __init__(self, **cfg):
+        self.cfg = make_config(**cfg)
+        self._h = ctypes.c_void_p()
+        st = lib().alens_create(ctypes.byref(self.cfg), ctypes.byref(self._h))
+        if st != ALENS_OK:
+            raise AlensError(st, "alens_create failed")
+        self.n = 0
+
+    def close(self):
+        if self._h:
+            lib().alens_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, st, allow=()):
+        if st != ALENS_OK and st not in allow:
+            raise AlensError(st, lib().alens_last_error(self._h).decode())
+        return st
+
+    def set_rods(self, rods):
+        gid = np.ascontiguousarray(rods["gid"], np.int32)
+        c = np.ascontiguousarray(rods["center"], np.float64).reshape(-1, 3)
+        t = np.ascontiguousarray(rods["dir"], np.float64).reshape(-1, 3)
+        L = np.ascontiguousarray(rods["length"], np.float64)
+        r = np.ascontiguousarray(rods["radius"], np.float64)
+        imm = rods.get("immovable")
+        imm = None if imm is None else np.ascontiguousarray(imm, np.uint8)
+        n = gid.size
+        self._check(lib().alens_set_rods(self._h, n, _p(gid, ctypes.c_int32), _p(c, ctypes.c_double),
+                                         _p(t, ctypes.c_double), _p(L, ctypes.c_double), _p(r, ctypes.c_double),
+                                         _p(imm, ctypes.c_uint8)))
+        self.n = n
+
+    def get_rods(self):
+        c = np.zeros((self.n, 3))
+        t = np.zeros((self.n, 3))
+        self._check(lib().alens_get_rods(self._h, _p(c, ctypes.c_double), _p(t, ctypes.c_double)))
+        return c, t
+
+    def set_step(self, step):
+        self._check(lib().alens_set_step(self._h, int(step)))
+
+    @property
+    def step_count(self):
+        return lib().alens_get_step(self._h)
+
+    def step(self, f_ext=None, allow=(ALENS_E_NOCONV,)):
+        rep = Report()
+        fe = None if f_ext is None else np.ascontiguousarray(f_ext, np.float64).reshape(-1)
+        st = lib().alens_step_constraints(self._h, _p(fe, ctypes.c_double), ctypes.byref(rep))
+        self._check(st, allow)
+        d = rep.as_dict()
+        d["status_code"] = st
+        return d
+
+    def euler(self):
+        self._check(lib().alens_step_euler(self._h))
+
+    def num_constraints(self):
+        m = ctypes.c_int64()
+        self._check(lib().alens_get_num_constraints(self._h, ctypes.byref(m)))
+        return m.value
+
+    def constraint_arrays(self):
+        m = self.num_constraints()
+        out = dict(I=np.zeros(m, np.int32), J=np.zeros(m, np.int32), img=np.zeros((m, 3), np.int32),
+                   delta0=np.zeros(m), nrm=np.zeros((m, 3)), posI=np.zeros((m, 3)), posJ=np.zeros((m, 3)),
+                   rI=np.zeros((m, 3)), rJ=np.zeros((m, 3)), b=np.zeros(m))
+        if m:
+            self._check(lib().alens_get_constraint_arrays(
+                self._h, _p(out["I"], ctypes.c_int32), _p(out["J"], ctypes.c_int32), _p(out["img"], ctypes.c_int32),
+                _p(out["delta0"], ctypes.c_double), _p(out["nrm"], ctypes.c_double), _p(out["posI"], ctypes.c_double),
+                _p(out["posJ"], ctypes.c_double), _p(out["rI"], ctypes.c_double), _p(out["rJ"], ctypes.c_double),
+                _p(out["b"], ctypes.c_double)))
+        return out
+
+    def constraint_blocks(self):
+        m = self.num_constraints()
+        arr = (ConBlock * max(m, 1))()
+        mm = ctypes.c_int64()
+        self._check(lib().alens_get_constraints(self._h, m, arr, ctypes.byref(mm)))
+        return arr[:m]
+
+    def gamma(self):
+        m = self.num_constraints()
+        g = np.zeros(m)
+        self._check(lib().alens_get_gamma(self._h, _p(g, ctypes.c_double)))
+        return g
+
+    def velocity(self):
+        n6 = 6 * self.n
+        u, unc, f, mob = np.zeros(n6), np.zeros(n6), np.zeros(n6), np.zeros(n6)
+        self._check(lib().alens_get_velocity(self._h, _p(u, ctypes.c_double), _p(unc, ctypes.c_double),
+                                             _p(f, ctypes.c_double), _p(mob, ctypes.c_double)))
+        return dict(u=u, unc=unc, fcon=f, mob=mob)
+
+    def bcqp_solve(self, n, I, J, nrm, rI, rJ, b, mob, x0=None, allow=(ALENS_E_NOCONV,)):
+        I = np.ascontiguousarray(I, np.int32)
+        J = np.ascontiguousarray(J, np.int32)
+        nrm = np.ascontiguousarray(nrm, np.float64).reshape(-1)
+        rI = np.ascontiguousarray(rI, np.float64).reshape(-1)
+        rJ = np.ascontiguousarray(rJ, np.float64).reshape(-1)
+        b = np.ascontiguousarray(b, np.float64)
+        mob = np.ascontiguousarray(mob, np.float64).reshape(-1)
+        x0 = None if x0 is None else np.ascontiguousarray(x0, np.float64)
+        m = I.size
+        g = np.zeros(max(m, 1))
+        rep = Report()
+        st = lib().alens_bcqp_solve(self._h, int(n), m, _p(I, ctypes.c_int32), _p(J, ctypes.c_int32),
+                                    _p(nrm, ctypes.c_double), _p(rI, ctypes.c_double), _p(rJ, ctypes.c_double),
+                                    _p(b, ctypes.c_double), _p(mob, ctypes.c_double), _p(x0, ctypes.c_double),
+                                    _p(g, ctypes.c_double), ctypes.byref(rep))
+        self._check(st, allow)
+        d = rep.as_dict()
+        d["status_code"] = st
+        return g[:m], d

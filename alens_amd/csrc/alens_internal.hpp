@@ -1,0 +1,160 @@
+// alens_internal.hpp -- host-side types shared by the HIP translation units of libalens.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/alens_capi.h"
+
+namespace alens {
+
+#define ALENS_HIP_TRY(call)                                                                     \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) return ::alens::HipErr{e_, #call, __FILE__, __LINE__};             \
+    } while (0)
+
+struct HipErr {
+    hipError_t code = hipSuccess;
+    const char *what = nullptr;
+    const char *file = nullptr;
+    int line = 0;
+    bool ok() const { return code == hipSuccess; }
+};
+inline HipErr hip_ok() { return HipErr{}; }
+
+// growable device buffer (contents not preserved on growth)
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t n) {
+        if (n <= cap && p) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            if (e != hipSuccess) return e;
+            p = nullptr;
+            cap = 0;
+        }
+        size_t want = n < 16 ? 16 : n + n / 8;
+        hipError_t e = hipMalloc((void **)&p, want * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        cap = want;
+        return hipSuccess;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+struct Grid {
+    int nc[3];
+    double h[3];
+    double len[3];
+    int64_t ncell;
+    int keybits;
+};
+
+struct RodsDev {
+    int32_t n;
+    const int32_t *gid;
+    const double *center;  // 3n
+    const double *dir;     // 3n
+    const double *hcol;    // collision half-length (0 for spheres)
+    const double *rcol;    // collision radius
+};
+
+struct CfgDev {
+    double box_lo[3], len[3];
+    int pbc[3];
+    int mono;
+    double col_buf, dt, kT, mu;
+    uint64_t seed;
+    int64_t step;
+};
+
+// ---- scratch for the generic device primitives ----
+struct ScanScratch {
+    DBuf<uint32_t> lvl[4];
+};
+
+// collide kernels (k_collide.hip)
+HipErr launch_mob_unc(hipStream_t st, int32_t n, const int32_t *gid, const double *dir,
+                      const double *length, const double *radius, const uint8_t *immovable,
+                      const double *fext, const CfgDev &c, double *mob, double *unc);
+HipErr launch_aabb(hipStream_t st, const RodsDev &r, const CfgDev &c, const Grid &g, int32_t *clo,
+                   int32_t *chi, double *blo, double *bhi, uint32_t *ecnt);
+HipErr exclusive_scan_u32(hipStream_t st, const uint32_t *in, uint32_t *out, int64_t n,
+                          ScanScratch &s);
+HipErr launch_fill_entries(hipStream_t st, int32_t n, const int32_t *clo, const int32_t *chi,
+                           const uint32_t *eoff, const Grid &g, uint32_t *keys, uint32_t *vals);
+HipErr radix_sort_pairs(hipStream_t st, uint32_t *keys, uint32_t *vals, uint32_t *keys_alt,
+                        uint32_t *vals_alt, int64_t n, int keybits, DBuf<uint32_t> &hist,
+                        ScanScratch &s, bool &result_in_alt);
+HipErr launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, int64_t n);
+HipErr launch_cell_ranges(hipStream_t st, const uint32_t *keys, int64_t E, uint32_t *cstart,
+                          uint32_t *cend);
+HipErr launch_sweep(hipStream_t st, const RodsDev &r, const CfgDev &c, const Grid &g,
+                    const uint32_t *cstart, const uint32_t *cend, const uint32_t *vals,
+                    const int32_t *clo, const double *blo, const double *bhi, uint32_t *pair_count,
+                    uint32_t cap, uint32_t *pI, uint32_t *pJ, uint32_t *pImg, uint32_t *cntI,
+                    uint32_t *cntJ, int n_cu);
+HipErr launch_order_I(hipStream_t st, int32_t n, uint32_t m, const uint32_t *pI,
+                      const uint32_t *pJ, const uint32_t *pImg, const uint32_t *rowStart,
+                      uint32_t *cur, int32_t *cI, int32_t *cJ, uint32_t *cImg);
+HipErr launch_order_J(hipStream_t st, int32_t n, uint32_t m, const int32_t *cJ,
+                      const uint32_t *jStart, uint32_t *cur, uint32_t *jlist);
+HipErr launch_rows(hipStream_t st, const RodsDev &r, const CfgDev &c, uint32_t m, const int32_t *cI,
+                   const int32_t *cJ, const uint32_t *cImg, const double *unc, double *delta0,
+                   double *geo, double *posI, double *posJ, double *b);
+HipErr launch_adjacency(hipStream_t st, int32_t n, const uint32_t *rowStart,
+                        const uint32_t *jStart, const uint32_t *jlist, const double *geo,
+                        uint32_t *adjStart, uint32_t *adjK, double *adjW);
+
+// solve kernels (k_solve.hip)
+struct SolveArgs {
+    int32_t n;
+    uint32_t m;
+    const uint32_t *adjStart;
+    const uint32_t *adjK;
+    const double *adjW;
+    const double *mob;
+    const int32_t *cI;
+    const int32_t *cJ;
+    const double *geo;
+    const double *b;
+    const double *lb;  // NULL = all zero
+    double2 *xg;       // (x, g) per constraint; x holds the start point on entry
+    double *U;         // 6n scratch
+    double *part;      // kRedQ * nchunks partials
+    uint32_t *bar;     // barrier words (zeroed before launch)
+    double *out;       // report words
+    double dt, tol;
+    int32_t max_ite;
+    int32_t mono;
+    int32_t warm;
+    int32_t nchunks;
+};
+HipErr launch_bbpgd64(hipStream_t st, SolveArgs a, int max_blocks, int *grid_used);
+HipErr launch_writeback(hipStream_t st, int32_t n, const uint32_t *adjStart, const uint32_t *adjK,
+                        const double *adjW, const double2 *xg, const double *mob,
+                        const double *unc, int mono, double *F, double *U, double *gamma,
+                        uint32_t m);
+HipErr launch_euler(hipStream_t st, int32_t n, double *center, double *dir, const double *u,
+                    const CfgDev &c);
+HipErr launch_init_xg(hipStream_t st, double2 *xg, const double *x0, uint32_t m);
+HipErr launch_warm_match(hipStream_t st, uint32_t m, const int32_t *cI, const int32_t *cJ,
+                         const int32_t *gid, uint32_t mprev, const uint64_t *prev_key,
+                         const double *prev_gamma, double2 *xg);
+HipErr launch_make_keys(hipStream_t st, uint32_t m, const int32_t *cI, const int32_t *cJ,
+                        const int32_t *gid, const double2 *xg, uint64_t *key, double *gam);
+
+constexpr int kRedQ = 8;  // partial-sum slots per chunk
+
+}  // namespace alens

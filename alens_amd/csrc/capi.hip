@@ -1,0 +1,713 @@
+// capi.hip -- the C ABI of libalens.so (include/alens_capi.h): context lifecycle and the per-step
+// orchestration of the collision-constraint path on one GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "alens_dev.hpp"
+#include "alens_internal.hpp"
+
+using namespace alens;
+
+struct alens_ctx {
+    alens_config cfg{};
+    std::string err;
+    hipStream_t st = nullptr;
+    int dev = 0;
+    int ncu = 0;
+    int64_t step = 0;
+    int32_t n = 0;
+    bool have_rods = false;
+    bool have_step = false;
+    Grid grid{};
+    int grid_used = 0;
+
+    std::vector<int32_t> h_gid;
+    DBuf<int32_t> gid;
+    DBuf<double> center, dir, length, radius, hcol, rcol;
+    DBuf<uint8_t> immov;
+    bool has_immov = false;
+    DBuf<double> mob, unc, U, F, Us, fext;
+
+    DBuf<int32_t> clo, chi;
+    DBuf<double> blo, bhi;
+    DBuf<uint32_t> ecnt, eoff, keys, vals, keys2, vals2, cstart, cend, hist;
+    ScanScratch scan;
+
+    DBuf<uint32_t> counter, pI, pJ, pImg, cntI, cntJ, rowStart, jStart, cur, jlist;
+    uint32_t pcap = 0;
+    DBuf<int32_t> cI, cJ;
+    DBuf<uint32_t> cImg;
+    DBuf<double> delta0, geo, posI, posJ, b;
+    DBuf<uint32_t> adjStart, adjK;
+    DBuf<double> adjW;
+
+    DBuf<double2> xg;
+    DBuf<double> gamma, part, out;
+    DBuf<uint32_t> bar;
+
+    DBuf<uint64_t> pkey;
+    DBuf<double> pgam;
+    uint32_t mprev = 0;
+    bool have_prev = false;
+
+    int64_t m = 0, E = 0;
+    hipEvent_t ev[6] = {};
+};
+
+namespace {
+
+const char *kVersion = "alens-mi355x 0.1 (gfx950, FP64 bit-exact path)";
+
+alens_status fail(alens_ctx *c, alens_status s, const std::string &msg) {
+    if (c) c->err = msg;
+    return s;
+}
+
+alens_status hip_fail(alens_ctx *c, const HipErr &e) {
+    char buf[512];
+    snprintf(buf, sizeof buf, "HIP error %d (%s) in %s at %s:%d", (int)e.code, hipGetErrorString(e.code),
+             e.what ? e.what : "?", e.file ? e.file : "?", e.line);
+    c->err = buf;
+    return e.code == hipErrorOutOfMemory ? ALENS_E_OOM : ALENS_E_HIP;
+}
+
+#define TRY_HIP(call)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) return hip_fail(ctx, HipErr{e_, #call, __FILE__, __LINE__}); \
+    } while (0)
+#define TRY_K(call)                                   \
+    do {                                              \
+        HipErr e_ = (call);                           \
+        if (!e_.ok()) return hip_fail(ctx, e_);       \
+    } while (0)
+
+CfgDev make_cfgdev(const alens_ctx *ctx) {
+    CfgDev c{};
+    for (int k = 0; k < 3; ++k) {
+        c.box_lo[k] = ctx->cfg.box_lo[k];
+        c.len[k] = ctx->cfg.box_hi[k] - ctx->cfg.box_lo[k];
+        c.pbc[k] = ctx->cfg.pbc[k] ? 1 : 0;
+    }
+    c.mono = ctx->cfg.monolayer ? 1 : 0;
+    c.col_buf = ctx->cfg.col_buf;
+    c.dt = ctx->cfg.dt;
+    c.kT = ctx->cfg.kT;
+    c.mu = ctx->cfg.viscosity;
+    c.seed = ctx->cfg.seed;
+    c.step = ctx->step;
+    return c;
+}
+
+RodsDev make_rods(const alens_ctx *ctx) {
+    RodsDev r{};
+    r.n = ctx->n;
+    r.gid = ctx->gid.p;
+    r.center = ctx->center.p;
+    r.dir = ctx->dir.p;
+    r.hcol = ctx->hcol.p;
+    r.rcol = ctx->rcol.p;
+    return r;
+}
+
+bool finite_all(const double *p, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(p[i])) return false;
+    return true;
+}
+
+template <typename T>
+hipError_t d2h(alens_ctx *ctx, T *dst, const T *src, size_t count) {
+    if (!dst || count == 0) return hipSuccess;
+    return hipMemcpyAsync(dst, src, sizeof(T) * count, hipMemcpyDeviceToHost, ctx->st);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *alens_version(void) { return kVersion; }
+
+const char *alens_last_error(const alens_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+alens_status alens_create(const alens_config *cfg, alens_ctx **out) {
+    if (!cfg || !out) return ALENS_E_ARG;
+    *out = nullptr;
+    alens_ctx *ctx = new alens_ctx();
+    ctx->cfg = *cfg;
+    auto bad = [&](const char *m) {
+        delete ctx;
+        (void)m;
+        return ALENS_E_ARG;
+    };
+    for (int k = 0; k < 3; ++k)
+        if (!(cfg->box_hi[k] > cfg->box_lo[k])) return bad("box");
+    if (!(cfg->dt > 0.0) || !(cfg->viscosity > 0.0) || !(cfg->tol > 0.0) || cfg->max_ite < 0 ||
+        !(cfg->col_buf >= 0.0) || !(cfg->diam_col_ratio > 0.0) || !(cfg->len_col_ratio > 0.0))
+        return bad("params");
+    if (cfg->solver != 0) return bad("solver");
+    if (cfg->precision != 0 && cfg->precision != 1) return bad("precision");
+    if (cfg->warm_start < 0 || cfg->warm_start > 1) return bad("warm_start");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        delete ctx;
+        return ALENS_E_HIP;
+    }
+    if (cfg->device < 0 || cfg->device >= ndev) return bad("device");
+    ctx->dev = cfg->device;
+    if (hipSetDevice(ctx->dev) != hipSuccess || hipStreamCreate(&ctx->st) != hipSuccess) {
+        delete ctx;
+        return ALENS_E_HIP;
+    }
+    if (hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, ctx->dev) != hipSuccess) ctx->ncu = 256;
+    for (auto &e : ctx->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete ctx;
+            return ALENS_E_HIP;
+        }
+    *out = ctx;
+    return ALENS_OK;
+}
+
+void alens_destroy(alens_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->dev);
+    if (ctx->st) (void)hipStreamSynchronize(ctx->st);
+    auto rel = [](auto &b) { b.release(); };
+    rel(ctx->gid); rel(ctx->center); rel(ctx->dir); rel(ctx->length); rel(ctx->radius); rel(ctx->hcol);
+    rel(ctx->rcol); rel(ctx->immov); rel(ctx->mob); rel(ctx->unc); rel(ctx->U); rel(ctx->F); rel(ctx->Us);
+    rel(ctx->fext); rel(ctx->clo); rel(ctx->chi); rel(ctx->blo); rel(ctx->bhi); rel(ctx->ecnt);
+    rel(ctx->eoff); rel(ctx->keys); rel(ctx->vals); rel(ctx->keys2); rel(ctx->vals2); rel(ctx->cstart);
+    rel(ctx->cend); rel(ctx->hist);
+    for (auto &l : ctx->scan.lvl) rel(l);
+    rel(ctx->counter); rel(ctx->pI); rel(ctx->pJ); rel(ctx->pImg); rel(ctx->cntI); rel(ctx->cntJ);
+    rel(ctx->rowStart); rel(ctx->jStart); rel(ctx->cur); rel(ctx->jlist); rel(ctx->cI); rel(ctx->cJ);
+    rel(ctx->cImg); rel(ctx->delta0); rel(ctx->geo); rel(ctx->posI); rel(ctx->posJ); rel(ctx->b);
+    rel(ctx->adjStart); rel(ctx->adjK); rel(ctx->adjW); rel(ctx->xg); rel(ctx->gamma); rel(ctx->part);
+    rel(ctx->out); rel(ctx->bar); rel(ctx->pkey); rel(ctx->pgam);
+    for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->st) (void)hipStreamDestroy(ctx->st);
+    delete ctx;
+}
+
+alens_status alens_set_rods(alens_ctx *ctx, int32_t n, const int32_t *gid, const double *center,
+                            const double *dir, const double *length, const double *radius,
+                            const uint8_t *immovable) {
+    if (!ctx) return ALENS_E_ARG;
+    if (n < 0 || (n > 0 && (!gid || !center || !dir || !length || !radius)))
+        return fail(ctx, ALENS_E_ARG, "set_rods: null array or negative count");
+    if (n > (1 << 30)) return fail(ctx, ALENS_E_ARG, "set_rods: too many rods");
+    if (!finite_all(center, 3 * (size_t)n) || !finite_all(dir, 3 * (size_t)n) ||
+        !finite_all(length, (size_t)n) || !finite_all(radius, (size_t)n))
+        return fail(ctx, ALENS_E_NONFINITE, "set_rods: non-finite rod data");
+    const alens_config &cf = ctx->cfg;
+    std::vector<double> hc(n), rc(n);
+    double lmax = 0.0, rmax = 0.0, lsum = 0.0, imax = 0.0;
+    for (int32_t i = 0; i < n; ++i) {
+        if (!(length[i] >= 0.0) || !(radius[i] > 0.0))
+            return fail(ctx, ALENS_E_ARG, "set_rods: length must be >= 0 and radius > 0");
+        hc[i] = (length[i] < radius[i] * 2.0) ? 0.0 : 0.5 * (length[i] * cf.len_col_ratio);
+        rc[i] = radius[i] * cf.diam_col_ratio;
+        lmax = std::max(lmax, 2.0 * hc[i]);
+        rmax = std::max(rmax, rc[i]);
+        lsum += 2.0 * hc[i];
+        imax = std::max(imax, rc[i] + 0.5 * cf.col_buf + kAabbMargin);
+    }
+    for (int k = 0; k < 3; ++k) {
+        if (!cf.pbc[k]) continue;
+        const double len = cf.box_hi[k] - cf.box_lo[k];
+        if (!(len >= 2.0 * (lmax + 2.0 * rmax + cf.col_buf)))
+            return fail(ctx, ALENS_E_BOX, "periodic box shorter than twice the largest interaction range");
+    }
+    // cell grid: edge = configured size, or half the mean collision length (at least twice the
+    // largest AABB inflation); capped at 8 cells per rod
+    double h = cf.cell_size;
+    if (!(h > 0.0)) {
+        h = n ? 0.5 * lsum / n : 1.0;
+        if (h < 2.0 * imax) h = 2.0 * imax;
+        if (!(h > 0.0)) h = 1.0;
+    }
+    const double maxcells = std::max<double>(8.0 * n, 4096.0);
+    Grid g{};
+    for (;;) {
+        double tot = 1.0;
+        for (int k = 0; k < 3; ++k) {
+            g.len[k] = cf.box_hi[k] - cf.box_lo[k];
+            int c = (int)std::floor(g.len[k] / h);
+            if (c < 1) c = 1;
+            g.nc[k] = c;
+            g.h[k] = g.len[k] / c;
+            tot *= c;
+        }
+        if (tot <= maxcells) break;
+        h *= 1.25;
+    }
+    g.ncell = (int64_t)g.nc[0] * g.nc[1] * g.nc[2];
+    int kb = 1;
+    while (((int64_t)1 << kb) < g.ncell) ++kb;
+    g.keybits = kb;
+    ctx->grid = g;
+
+    TRY_HIP(hipSetDevice(ctx->dev));
+    const size_t N = (size_t)std::max(n, 1);
+    TRY_HIP(ctx->gid.ensure(N));
+    TRY_HIP(ctx->center.ensure(3 * N));
+    TRY_HIP(ctx->dir.ensure(3 * N));
+    TRY_HIP(ctx->length.ensure(N));
+    TRY_HIP(ctx->radius.ensure(N));
+    TRY_HIP(ctx->hcol.ensure(N));
+    TRY_HIP(ctx->rcol.ensure(N));
+    for (auto *bf : {&ctx->mob, &ctx->unc, &ctx->U, &ctx->F, &ctx->Us, &ctx->fext}) TRY_HIP(bf->ensure(6 * N));
+    TRY_HIP(ctx->clo.ensure(3 * N));
+    TRY_HIP(ctx->chi.ensure(3 * N));
+    TRY_HIP(ctx->blo.ensure(3 * N));
+    TRY_HIP(ctx->bhi.ensure(3 * N));
+    TRY_HIP(ctx->ecnt.ensure(N + 1));
+    TRY_HIP(ctx->eoff.ensure(N + 1));
+    for (auto *bf : {&ctx->cntI, &ctx->cntJ, &ctx->rowStart, &ctx->jStart, &ctx->cur, &ctx->adjStart})
+        TRY_HIP(bf->ensure(N + 1));
+    TRY_HIP(ctx->cstart.ensure((size_t)g.ncell));
+    TRY_HIP(ctx->cend.ensure((size_t)g.ncell));
+    TRY_HIP(ctx->counter.ensure(4));
+    TRY_HIP(ctx->bar.ensure(4));
+    TRY_HIP(ctx->out.ensure(16));
+    if (n > 0) {
+        TRY_HIP(hipMemcpyAsync(ctx->gid.p, gid, sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->st));
+        TRY_HIP(hipMemcpyAsync(ctx->center.p, center, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->st));
+        TRY_HIP(hipMemcpyAsync(ctx->dir.p, dir, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->st));
+        TRY_HIP(hipMemcpyAsync(ctx->length.p, length, sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
+        TRY_HIP(hipMemcpyAsync(ctx->radius.p, radius, sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
+        TRY_HIP(hipMemcpyAsync(ctx->hcol.p, hc.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
+        TRY_HIP(hipMemcpyAsync(ctx->rcol.p, rc.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
+    }
+    ctx->has_immov = immovable != nullptr;
+    if (immovable && n > 0) {
+        TRY_HIP(ctx->immov.ensure(N));
+        TRY_HIP(hipMemcpyAsync(ctx->immov.p, immovable, n, hipMemcpyHostToDevice, ctx->st));
+    }
+    // the previous-step impulses stay valid only for the same rods in the same order
+    std::vector<int32_t> ng(gid, gid + n);
+    if (ng != ctx->h_gid) ctx->have_prev = false;
+    ctx->h_gid.swap(ng);
+    TRY_HIP(hipStreamSynchronize(ctx->st));
+    ctx->n = n;
+    ctx->have_rods = true;
+    ctx->have_step = false;
+    return ALENS_OK;
+}
+
+alens_status alens_get_rods(alens_ctx *ctx, double *center, double *dir) {
+    if (!ctx) return ALENS_E_ARG;
+    if (!ctx->have_rods) return fail(ctx, ALENS_E_STATE, "get_rods before set_rods");
+    TRY_HIP(hipSetDevice(ctx->dev));
+    if (ctx->n > 0) {
+        if (center) TRY_HIP(hipMemcpyAsync(center, ctx->center.p, sizeof(double) * 3 * ctx->n, hipMemcpyDeviceToHost, ctx->st));
+        if (dir) TRY_HIP(hipMemcpyAsync(dir, ctx->dir.p, sizeof(double) * 3 * ctx->n, hipMemcpyDeviceToHost, ctx->st));
+    }
+    TRY_HIP(hipStreamSynchronize(ctx->st));
+    return ALENS_OK;
+}
+
+alens_status alens_set_step(alens_ctx *ctx, int64_t step) {
+    if (!ctx) return ALENS_E_ARG;
+    ctx->step = step;
+    return ALENS_OK;
+}
+int64_t alens_get_step(const alens_ctx *ctx) { return ctx ? ctx->step : -1; }
+
+static alens_status run_solver(alens_ctx *ctx, int32_t n, uint32_t m, const double *mob, bool warm,
+                               alens_solve_report *rep, float *ms_solve) {
+    const uint32_t nchunks = (m + kRedChunk - 1) / kRedChunk;
+    TRY_HIP(ctx->part.ensure((size_t)kRedQ * std::max<uint32_t>(nchunks, 1)));
+    TRY_HIP(hipMemsetAsync(ctx->bar.p, 0, 4 * sizeof(uint32_t), ctx->st));
+    SolveArgs a{};
+    a.n = n;
+    a.m = m;
+    a.adjStart = ctx->adjStart.p;
+    a.adjK = ctx->adjK.p;
+    a.adjW = ctx->adjW.p;
+    a.mob = mob;
+    a.cI = ctx->cI.p;
+    a.cJ = ctx->cJ.p;
+    a.geo = ctx->geo.p;
+    a.b = ctx->b.p;
+    a.lb = nullptr;
+    a.xg = ctx->xg.p;
+    a.U = ctx->Us.p;
+    a.part = ctx->part.p;
+    a.bar = ctx->bar.p;
+    a.out = ctx->out.p;
+    a.dt = ctx->cfg.dt;
+    a.tol = ctx->cfg.tol;
+    a.max_ite = ctx->cfg.max_ite;
+    a.mono = ctx->cfg.monolayer ? 1 : 0;
+    a.warm = warm ? 1 : 0;
+    a.nchunks = (int32_t)nchunks;
+    TRY_HIP(hipEventRecord(ctx->ev[2], ctx->st));
+    TRY_K(launch_bbpgd64(ctx->st, a, 0, &ctx->grid_used));
+    TRY_HIP(hipEventRecord(ctx->ev[3], ctx->st));
+    double o[8];
+    uint32_t barw[4];
+    TRY_HIP(hipMemcpyAsync(o, ctx->out.p, sizeof o, hipMemcpyDeviceToHost, ctx->st));
+    TRY_HIP(hipMemcpyAsync(barw, ctx->bar.p, sizeof barw, hipMemcpyDeviceToHost, ctx->st));
+    TRY_HIP(hipStreamSynchronize(ctx->st));
+    if (barw[2] != 0) return fail(ctx, ALENS_E_HIP, "BBPGD grid barrier timed out (blocks not co-resident)");
+    TRY_HIP(hipEventElapsedTime(ms_solve, ctx->ev[2], ctx->ev[3]));
+    rep->iters = (int32_t)o[0];
+    rep->matvecs = (int32_t)o[1];
+    rep->converged = (int32_t)o[2];
+    rep->fallback_used = (int32_t)o[3];
+    rep->status = (int32_t)o[4];
+    rep->res0 = o[5];
+    rep->res = o[6];
+    return ALENS_OK;
+}
+
+alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_solve_report *rep_out) {
+    if (!ctx) return ALENS_E_ARG;
+    if (!ctx->have_rods) return fail(ctx, ALENS_E_STATE, "step_constraints before set_rods");
+    alens_solve_report rep{};
+    TRY_HIP(hipSetDevice(ctx->dev));
+    hipStream_t st = ctx->st;
+    const int32_t n = ctx->n;
+    const Grid &g = ctx->grid;
+    const CfgDev cd = make_cfgdev(ctx);
+    const RodsDev rd = make_rods(ctx);
+    if (f_ext) {
+        if (!finite_all(f_ext, 6 * (size_t)n)) return fail(ctx, ALENS_E_NONFINITE, "non-finite f_ext");
+        if (n > 0) TRY_HIP(hipMemcpyAsync(ctx->fext.p, f_ext, sizeof(double) * 6 * n, hipMemcpyHostToDevice, st));
+    }
+    TRY_HIP(hipEventRecord(ctx->ev[0], st));
+    TRY_K(launch_mob_unc(st, n, ctx->gid.p, ctx->dir.p, ctx->length.p, ctx->radius.p,
+                         ctx->has_immov ? ctx->immov.p : nullptr, f_ext ? ctx->fext.p : nullptr, cd,
+                         ctx->mob.p, ctx->unc.p));
+    // cell list
+    TRY_K(launch_aabb(st, rd, cd, g, ctx->clo.p, ctx->chi.p, ctx->blo.p, ctx->bhi.p, ctx->ecnt.p));
+    TRY_K(exclusive_scan_u32(st, ctx->ecnt.p, ctx->eoff.p, n, ctx->scan));
+    uint32_t E = 0;
+    TRY_HIP(hipMemcpyAsync(&E, ctx->eoff.p + n, sizeof E, hipMemcpyDeviceToHost, st));
+    TRY_HIP(hipStreamSynchronize(st));
+    ctx->E = E;
+    const size_t Ecap = std::max<size_t>(E, 1);
+    TRY_HIP(ctx->keys.ensure(Ecap));
+    TRY_HIP(ctx->vals.ensure(Ecap));
+    TRY_HIP(ctx->keys2.ensure(Ecap));
+    TRY_HIP(ctx->vals2.ensure(Ecap));
+    TRY_K(launch_fill_entries(st, n, ctx->clo.p, ctx->chi.p, ctx->eoff.p, g, ctx->keys.p, ctx->vals.p));
+    bool alt = false;
+    TRY_K(radix_sort_pairs(st, ctx->keys.p, ctx->vals.p, ctx->keys2.p, ctx->vals2.p, E, g.keybits,
+                           ctx->hist, ctx->scan, alt));
+    const uint32_t *skeys = alt ? ctx->keys2.p : ctx->keys.p;
+    const uint32_t *svals = alt ? ctx->vals2.p : ctx->vals.p;
+    TRY_HIP(hipMemsetAsync(ctx->cstart.p, 0, sizeof(uint32_t) * g.ncell, st));
+    TRY_HIP(hipMemsetAsync(ctx->cend.p, 0, sizeof(uint32_t) * g.ncell, st));
+    TRY_K(launch_cell_ranges(st, skeys, E, ctx->cstart.p, ctx->cend.p));
+    // pair sweep (re-run once with a larger buffer if the pair buffer overflowed)
+    if (ctx->pcap == 0) {
+        const size_t c0 = std::max<size_t>(4096, 8 * (size_t)n);
+        TRY_HIP(ctx->pI.ensure(c0));
+        TRY_HIP(ctx->pJ.ensure(c0));
+        TRY_HIP(ctx->pImg.ensure(c0));
+        ctx->pcap = (uint32_t)ctx->pI.cap;
+    }
+    uint32_t M = 0;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        TRY_HIP(hipMemsetAsync(ctx->counter.p, 0, sizeof(uint32_t), st));
+        TRY_HIP(hipMemsetAsync(ctx->cntI.p, 0, sizeof(uint32_t) * (n + 1), st));
+        TRY_HIP(hipMemsetAsync(ctx->cntJ.p, 0, sizeof(uint32_t) * (n + 1), st));
+        TRY_K(launch_sweep(st, rd, cd, g, ctx->cstart.p, ctx->cend.p, svals, ctx->clo.p, ctx->blo.p, ctx->bhi.p,
+                           ctx->counter.p, ctx->pcap, ctx->pI.p, ctx->pJ.p, ctx->pImg.p, ctx->cntI.p,
+                           ctx->cntJ.p, ctx->ncu));
+        TRY_HIP(hipMemcpyAsync(&M, ctx->counter.p, sizeof M, hipMemcpyDeviceToHost, st));
+        TRY_HIP(hipStreamSynchronize(st));
+        if (M <= ctx->pcap) break;
+        const size_t nc = (size_t)M + M / 4 + 1024;
+        TRY_HIP(ctx->pI.ensure(nc));
+        TRY_HIP(ctx->pJ.ensure(nc));
+        TRY_HIP(ctx->pImg.ensure(nc));
+        ctx->pcap = (uint32_t)std::min(ctx->pI.cap, std::min(ctx->pJ.cap, ctx->pImg.cap));
+    }
+    if (M > ctx->pcap) return fail(ctx, ALENS_E_HIP, "pair buffer overflow after resize");
+    ctx->m = M;
+    const size_t Mc = std::max<size_t>(M, 1);
+    TRY_HIP(ctx->cI.ensure(Mc));
+    TRY_HIP(ctx->cJ.ensure(Mc));
+    TRY_HIP(ctx->cImg.ensure(Mc));
+    TRY_HIP(ctx->jlist.ensure(Mc));
+    TRY_HIP(ctx->delta0.ensure(Mc));
+    TRY_HIP(ctx->geo.ensure(9 * Mc));
+    TRY_HIP(ctx->posI.ensure(3 * Mc));
+    TRY_HIP(ctx->posJ.ensure(3 * Mc));
+    TRY_HIP(ctx->b.ensure(Mc));
+    TRY_HIP(ctx->adjK.ensure(2 * Mc));
+    TRY_HIP(ctx->adjW.ensure(12 * Mc));
+    TRY_HIP(ctx->xg.ensure(Mc));
+    TRY_HIP(ctx->gamma.ensure(Mc));
+    // canonical order + rows + adjacency
+    TRY_K(exclusive_scan_u32(st, ctx->cntI.p, ctx->rowStart.p, n, ctx->scan));
+    TRY_K(exclusive_scan_u32(st, ctx->cntJ.p, ctx->jStart.p, n, ctx->scan));
+    TRY_HIP(hipMemsetAsync(ctx->cur.p, 0, sizeof(uint32_t) * (n + 1), st));
+    TRY_K(launch_order_I(st, n, M, ctx->pI.p, ctx->pJ.p, ctx->pImg.p, ctx->rowStart.p, ctx->cur.p, ctx->cI.p,
+                         ctx->cJ.p, ctx->cImg.p));
+    TRY_HIP(hipMemsetAsync(ctx->cur.p, 0, sizeof(uint32_t) * (n + 1), st));
+    TRY_K(launch_order_J(st, n, M, ctx->cJ.p, ctx->jStart.p, ctx->cur.p, ctx->jlist.p));
+    TRY_K(launch_rows(st, rd, cd, M, ctx->cI.p, ctx->cJ.p, ctx->cImg.p, ctx->unc.p, ctx->delta0.p, ctx->geo.p,
+                      ctx->posI.p, ctx->posJ.p, ctx->b.p));
+    TRY_K(launch_adjacency(st, n, ctx->rowStart.p, ctx->jStart.p, ctx->jlist.p, ctx->geo.p, ctx->adjStart.p,
+                           ctx->adjK.p, ctx->adjW.p));
+    TRY_HIP(hipEventRecord(ctx->ev[1], st));
+    // solve
+    float ms_solve = 0.0f;
+    const bool warm = ctx->cfg.warm_start == 1 && ctx->have_prev;
+    if (M > 0) {
+        if (warm)
+            TRY_K(launch_warm_match(st, M, ctx->cI.p, ctx->cJ.p, ctx->gid.p, ctx->mprev, ctx->pkey.p, ctx->pgam.p,
+                                    ctx->xg.p));
+        else
+            TRY_K(launch_init_xg(st, ctx->xg.p, nullptr, M));
+        alens_status s = run_solver(ctx, n, M, ctx->mob.p, warm, &rep, &ms_solve);
+        if (s != ALENS_OK) return s;
+    } else {
+        rep.converged = 1;
+    }
+    TRY_K(launch_writeback(st, n, ctx->adjStart.p, ctx->adjK.p, ctx->adjW.p, ctx->xg.p, ctx->mob.p, ctx->unc.p,
+                           ctx->cfg.monolayer ? 1 : 0, ctx->F.p, ctx->U.p, ctx->gamma.p, M));
+    if (ctx->cfg.warm_start == 1) {
+        TRY_HIP(ctx->pkey.ensure(Mc));
+        TRY_HIP(ctx->pgam.ensure(Mc));
+        TRY_K(launch_make_keys(st, M, ctx->cI.p, ctx->cJ.p, ctx->gid.p, ctx->xg.p, ctx->pkey.p, ctx->pgam.p));
+        ctx->mprev = M;
+        ctx->have_prev = true;
+    }
+    TRY_HIP(hipEventRecord(ctx->ev[4], st));
+    TRY_HIP(hipStreamSynchronize(st));
+    float ms_collect = 0.f, ms_step = 0.f;
+    TRY_HIP(hipEventElapsedTime(&ms_collect, ctx->ev[0], ctx->ev[1]));
+    TRY_HIP(hipEventElapsedTime(&ms_step, ctx->ev[0], ctx->ev[4]));
+    rep.n_con = M;
+    rep.n_entries = E;
+    rep.t_collect_ms = ms_collect;
+    rep.t_solve_ms = ms_solve;
+    rep.t_step_ms = ms_step;
+    ctx->have_step = true;
+    if (rep_out) *rep_out = rep;
+    if (rep.status == ALENS_E_NONFINITE) return fail(ctx, ALENS_E_NONFINITE, "non-finite BBPGD residual");
+    if (rep.status == ALENS_E_NOCONV) return fail(ctx, ALENS_E_NOCONV, "BBPGD did not reach tol within max_ite");
+    return ALENS_OK;
+}
+
+alens_status alens_step_euler(alens_ctx *ctx) {
+    if (!ctx) return ALENS_E_ARG;
+    if (!ctx->have_step) return fail(ctx, ALENS_E_STATE, "step_euler before step_constraints");
+    TRY_HIP(hipSetDevice(ctx->dev));
+    const CfgDev cd = make_cfgdev(ctx);
+    TRY_K(launch_euler(ctx->st, ctx->n, ctx->center.p, ctx->dir.p, ctx->U.p, cd));
+    TRY_HIP(hipStreamSynchronize(ctx->st));
+    ctx->step += 1;
+    ctx->have_step = false;
+    return ALENS_OK;
+}
+
+alens_status alens_get_num_constraints(alens_ctx *ctx, int64_t *m) {
+    if (!ctx || !m) return ALENS_E_ARG;
+    if (!ctx->have_step) return fail(ctx, ALENS_E_STATE, "no step has been run");
+    *m = ctx->m;
+    return ALENS_OK;
+}
+
+alens_status alens_get_constraint_arrays(alens_ctx *ctx, int32_t *I, int32_t *J, int32_t *img,
+                                         double *delta0, double *nrm, double *pos_i, double *pos_j,
+                                         double *r_i, double *r_j, double *b) {
+    if (!ctx) return ALENS_E_ARG;
+    if (!ctx->have_step) return fail(ctx, ALENS_E_STATE, "no step has been run");
+    TRY_HIP(hipSetDevice(ctx->dev));
+    const size_t m = (size_t)ctx->m;
+    TRY_HIP(d2h(ctx, I, ctx->cI.p, m));
+    TRY_HIP(d2h(ctx, J, ctx->cJ.p, m));
+    TRY_HIP(d2h(ctx, delta0, ctx->delta0.p, m));
+    TRY_HIP(d2h(ctx, pos_i, ctx->posI.p, 3 * m));
+    TRY_HIP(d2h(ctx, pos_j, ctx->posJ.p, 3 * m));
+    TRY_HIP(d2h(ctx, b, ctx->b.p, m));
+    std::vector<uint32_t> im;
+    std::vector<double> geo;
+    if (img && m) {
+        im.resize(m);
+        TRY_HIP(d2h(ctx, im.data(), ctx->cImg.p, m));
+    }
+    if ((nrm || r_i || r_j) && m) {
+        geo.resize(9 * m);
+        TRY_HIP(d2h(ctx, geo.data(), ctx->geo.p, 9 * m));
+    }
+    TRY_HIP(hipStreamSynchronize(ctx->st));
+    for (size_t k = 0; k < m; ++k) {
+        if (img)
+            for (int a = 0; a < 3; ++a) img[3 * k + a] = img_get(im[k], a);
+        for (int a = 0; a < 3; ++a) {
+            if (nrm) nrm[3 * k + a] = geo[9 * k + a];
+            if (r_i) r_i[3 * k + a] = geo[9 * k + 3 + a];
+            if (r_j) r_j[3 * k + a] = geo[9 * k + 6 + a];
+        }
+    }
+    return ALENS_OK;
+}
+
+alens_status alens_get_gamma(alens_ctx *ctx, double *gamma) {
+    if (!ctx || (!gamma && ctx->m)) return ALENS_E_ARG;
+    if (!ctx->have_step) return fail(ctx, ALENS_E_STATE, "no step has been run");
+    TRY_HIP(hipSetDevice(ctx->dev));
+    TRY_HIP(d2h(ctx, gamma, ctx->gamma.p, (size_t)ctx->m));
+    TRY_HIP(hipStreamSynchronize(ctx->st));
+    return ALENS_OK;
+}
+
+alens_status alens_get_velocity(alens_ctx *ctx, double *u, double *u_nc, double *f_con, double *mob) {
+    if (!ctx) return ALENS_E_ARG;
+    if (!ctx->have_step) return fail(ctx, ALENS_E_STATE, "no step has been run");
+    TRY_HIP(hipSetDevice(ctx->dev));
+    const size_t n6 = 6 * (size_t)ctx->n;
+    TRY_HIP(d2h(ctx, u, ctx->U.p, n6));
+    TRY_HIP(d2h(ctx, u_nc, ctx->unc.p, n6));
+    TRY_HIP(d2h(ctx, f_con, ctx->F.p, n6));
+    TRY_HIP(d2h(ctx, mob, ctx->mob.p, n6));
+    TRY_HIP(hipStreamSynchronize(ctx->st));
+    return ALENS_OK;
+}
+
+alens_status alens_get_constraints(alens_ctx *ctx, int64_t cap, alens_con_block *out, int64_t *m_out) {
+    if (!ctx || !m_out) return ALENS_E_ARG;
+    if (!ctx->have_step) return fail(ctx, ALENS_E_STATE, "no step has been run");
+    const size_t m = (size_t)ctx->m;
+    *m_out = (int64_t)m;
+    if (!out || cap < (int64_t)m) return m == 0 ? ALENS_OK : fail(ctx, ALENS_E_ARG, "get_constraints: capacity");
+    std::vector<int32_t> I(m), J(m), img(3 * m);
+    std::vector<double> d0(m), nrm(3 * m), pI(3 * m), pJ(3 * m), gam(m), cen(3 * (size_t)ctx->n);
+    alens_status s = alens_get_constraint_arrays(ctx, I.data(), J.data(), img.data(), d0.data(), nrm.data(),
+                                                 pI.data(), pJ.data(), nullptr, nullptr, nullptr);
+    if (s != ALENS_OK) return s;
+    s = alens_get_gamma(ctx, gam.data());
+    if (s != ALENS_OK) return s;
+    s = alens_get_rods(ctx, cen.data(), nullptr);
+    if (s != ALENS_OK) return s;
+    for (size_t k = 0; k < m; ++k) {
+        alens_con_block &cb = out[k];
+        memset(&cb, 0, sizeof cb);
+        cb.delta0 = d0[k];
+        cb.gamma = gam[k];
+        cb.kappa = 0.0;
+        cb.gamma_lb = 0.0;
+        cb.gid_i = ctx->h_gid[I[k]];
+        cb.gid_j = ctx->h_gid[J[k]];
+        cb.gidx_i = I[k];
+        cb.gidx_j = J[k];
+        for (int a = 0; a < 3; ++a) {
+            const double len = ctx->cfg.box_hi[a] - ctx->cfg.box_lo[a];
+            const double cj = ctx->cfg.pbc[a] ? cen[3 * J[k] + a] + (double)img[3 * k + a] * len : cen[3 * J[k] + a];
+            cb.norm_i[a] = nrm[3 * k + a];
+            cb.norm_j[a] = -nrm[3 * k + a];
+            cb.pos_i[a] = pI[3 * k + a];
+            cb.pos_j[a] = pJ[3 * k + a];
+            cb.lab_i[a] = cen[3 * I[k] + a] + pI[3 * k + a];
+            cb.lab_j[a] = cj + pJ[3 * k + a];
+        }
+        cb.one_side = 0;
+        cb.bilateral = 0;
+    }
+    return ALENS_OK;
+}
+
+alens_status alens_bcqp_solve(alens_ctx *ctx, int32_t n, int64_t m, const int32_t *I, const int32_t *J,
+                              const double *nrm, const double *r_i, const double *r_j, const double *b,
+                              const double *mob, const double *x0, double *gamma_out,
+                              alens_solve_report *rep_out) {
+    if (!ctx) return ALENS_E_ARG;
+    if (n < 0 || m < 0 || m > 0x7fffffff || (m > 0 && (!I || !J || !nrm || !r_i || !r_j || !b || !mob || !gamma_out)))
+        return fail(ctx, ALENS_E_ARG, "bcqp_solve: bad arguments");
+    alens_solve_report rep{};
+    if (m == 0) {
+        rep.converged = 1;
+        if (rep_out) *rep_out = rep;
+        return ALENS_OK;
+    }
+    for (int64_t k = 0; k < m; ++k)
+        if (I[k] < 0 || I[k] >= n || J[k] < 0 || J[k] >= n || I[k] == J[k])
+            return fail(ctx, ALENS_E_ARG, "bcqp_solve: rod index out of range");
+    TRY_HIP(hipSetDevice(ctx->dev));
+    // rod-major adjacency in ascending constraint index, built on the host for arbitrary rows
+    std::vector<uint32_t> start((size_t)n + 1, 0), adjK(2 * (size_t)m);
+    std::vector<double> adjW(12 * (size_t)m), geo(9 * (size_t)m);
+    for (int64_t k = 0; k < m; ++k) {
+        start[I[k] + 1]++;
+        start[J[k] + 1]++;
+        for (int a = 0; a < 3; ++a) {
+            geo[9 * k + a] = nrm[3 * k + a];
+            geo[9 * k + 3 + a] = r_i[3 * k + a];
+            geo[9 * k + 6 + a] = r_j[3 * k + a];
+        }
+    }
+    for (int32_t i = 0; i < n; ++i) start[i + 1] += start[i];
+    std::vector<uint32_t> cur(start.begin(), start.end() - 1);
+    for (int64_t k = 0; k < m; ++k) {
+        uint32_t e = cur[I[k]]++;
+        adjK[e] = (uint32_t)k;
+        for (int a = 0; a < 3; ++a) {
+            adjW[6 * e + a] = nrm[3 * k + a];
+            adjW[6 * e + 3 + a] = r_i[3 * k + a];
+        }
+        e = cur[J[k]]++;
+        adjK[e] = (uint32_t)k | 0x80000000u;
+        for (int a = 0; a < 3; ++a) {
+            adjW[6 * e + a] = -nrm[3 * k + a];
+            adjW[6 * e + 3 + a] = -r_j[3 * k + a];
+        }
+    }
+    const size_t M = (size_t)m, N = (size_t)std::max(n, 1);
+    TRY_HIP(ctx->adjStart.ensure(N + 1));
+    TRY_HIP(ctx->adjK.ensure(2 * M));
+    TRY_HIP(ctx->adjW.ensure(12 * M));
+    TRY_HIP(ctx->geo.ensure(9 * M));
+    TRY_HIP(ctx->cI.ensure(M));
+    TRY_HIP(ctx->cJ.ensure(M));
+    TRY_HIP(ctx->b.ensure(M));
+    TRY_HIP(ctx->xg.ensure(M));
+    TRY_HIP(ctx->gamma.ensure(M));
+    TRY_HIP(ctx->Us.ensure(6 * N));
+    TRY_HIP(ctx->fext.ensure(6 * N));
+    TRY_HIP(ctx->bar.ensure(4));
+    TRY_HIP(ctx->out.ensure(16));
+    TRY_HIP(ctx->delta0.ensure(M));
+    hipStream_t st = ctx->st;
+    TRY_HIP(hipMemcpyAsync(ctx->adjStart.p, start.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice, st));
+    TRY_HIP(hipMemcpyAsync(ctx->adjK.p, adjK.data(), sizeof(uint32_t) * 2 * M, hipMemcpyHostToDevice, st));
+    TRY_HIP(hipMemcpyAsync(ctx->adjW.p, adjW.data(), sizeof(double) * 12 * M, hipMemcpyHostToDevice, st));
+    TRY_HIP(hipMemcpyAsync(ctx->geo.p, geo.data(), sizeof(double) * 9 * M, hipMemcpyHostToDevice, st));
+    TRY_HIP(hipMemcpyAsync(ctx->cI.p, I, sizeof(int32_t) * M, hipMemcpyHostToDevice, st));
+    TRY_HIP(hipMemcpyAsync(ctx->cJ.p, J, sizeof(int32_t) * M, hipMemcpyHostToDevice, st));
+    TRY_HIP(hipMemcpyAsync(ctx->b.p, b, sizeof(double) * M, hipMemcpyHostToDevice, st));
+    TRY_HIP(hipMemcpyAsync(ctx->fext.p, mob, sizeof(double) * 6 * n, hipMemcpyHostToDevice, st));
+    if (x0) TRY_HIP(hipMemcpyAsync(ctx->delta0.p, x0, sizeof(double) * M, hipMemcpyHostToDevice, st));
+    TRY_K(launch_init_xg(st, ctx->xg.p, x0 ? ctx->delta0.p : nullptr, (uint32_t)M));
+    float ms = 0.f;
+    alens_status s = run_solver(ctx, n, (uint32_t)M, ctx->fext.p, x0 != nullptr, &rep, &ms);
+    if (s != ALENS_OK) return s;
+    TRY_K(launch_writeback(st, 0, nullptr, nullptr, nullptr, ctx->xg.p, nullptr, nullptr, 0, nullptr, nullptr,
+                           ctx->gamma.p, (uint32_t)M));
+    TRY_HIP(hipMemcpyAsync(gamma_out, ctx->gamma.p, sizeof(double) * M, hipMemcpyDeviceToHost, st));
+    TRY_HIP(hipStreamSynchronize(st));
+    rep.n_con = m;
+    rep.t_solve_ms = ms;
+    rep.t_step_ms = ms;
+    ctx->have_step = false;  // the step buffers now hold this solve's rows
+    if (rep_out) *rep_out = rep;
+    if (rep.status == ALENS_E_NONFINITE) return fail(ctx, ALENS_E_NONFINITE, "non-finite BBPGD residual");
+    if (rep.status == ALENS_E_NOCONV) return fail(ctx, ALENS_E_NOCONV, "BBPGD did not reach tol within max_ite");
+    return ALENS_OK;
+}
+
+}  // extern "C"

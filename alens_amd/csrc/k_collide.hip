@@ -1,0 +1,715 @@
+// k_collide.hip -- per-step collision collection on gfx950:
+//   per-rod mobility + U_nc (Brownian), AABB cell ranges, LSD radix sort of (cell, rod) entries,
+//   cell ranges, pair sweep (FP64 segment distance + cutoff), canonical ordering of the pairs,
+//   constraint rows and the rod-major adjacency used by the solver.
+// Reference roles: SylinderSystem::calcVelocityBrown / calcVelocityNonCon / collectPairCollision and
+// ConstraintCollector::buildConstraintMatrixVector, reached from rodSystem.runStep()
+// (SRC/TubuleSystem.cpp:223); block layout from SRC/TubuleSystem.cpp:719-736.
+#include "alens_dev.hpp"
+#include "alens_internal.hpp"
+
+namespace alens {
+
+#define LAUNCH_CHECK()                                                                           \
+    do {                                                                                         \
+        hipError_t e_ = hipGetLastError();                                                       \
+        if (e_ != hipSuccess) return HipErr{e_, "kernel launch", __FILE__, __LINE__};             \
+    } while (0)
+
+static inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+// ------------------------------------------------------------------------------------------------
+// mobility + U_nc = M F_ext + U_B.  U_B = sqrt(2 kT/dt) M^{1/2} xi, xi from Philox4x32-10 keyed by
+// the seed, counter (gid, step_lo, step_hi, c), Box-Muller on 53-bit uniforms.
+// ------------------------------------------------------------------------------------------------
+__global__ void k_mob_unc(int32_t n, const int32_t *__restrict__ gid, const double *__restrict__ dir,
+                          const double *__restrict__ length, const double *__restrict__ radius,
+                          const uint8_t *__restrict__ immovable, const double *__restrict__ fext,
+                          CfgDev c, double *__restrict__ mob, double *__restrict__ unc) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double ipar, iperp, irot;
+    drag_inv(length[i], radius[i], c.mu, ipar, iperp, irot);
+    double mb[6];
+    mb[0] = dir[3 * i];
+    mb[1] = dir[3 * i + 1];
+    mb[2] = dir[3 * i + 2];
+    if (immovable && immovable[i]) {
+        mb[3] = 0.0;
+        mb[4] = 0.0;
+        mb[5] = 0.0;
+    } else {
+        mb[3] = iperp;
+        mb[4] = ipar - iperp;
+        mb[5] = irot;
+    }
+    double ub[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (c.kT > 0.0) {
+        const double pre = sqrt(2.0 * c.kT / c.dt);
+        double z[6];
+        const uint32_t k0 = (uint32_t)c.seed, k1 = (uint32_t)(c.seed >> 32);
+#pragma unroll
+        for (uint32_t q = 0; q < 3; ++q) {
+            uint32_t ctr[4] = {(uint32_t)gid[i], (uint32_t)c.step, (uint32_t)((uint64_t)c.step >> 32), q};
+            philox4x32_10(ctr, k0, k1);
+            const double u1 =
+                ((double)((((uint64_t)ctr[0]) << 21) ^ (uint64_t)(ctr[1] >> 11)) + 1.0) * 0x1p-53;
+            const double u2 = (double)((((uint64_t)ctr[2]) << 21) ^ (uint64_t)(ctr[3] >> 11)) * 0x1p-53;
+            const double R = sqrt(-2.0 * alog(u1));
+            double sn, cs;
+            asincos2pi(u2, sn, cs);
+            z[2 * q] = R * cs;
+            z[2 * q + 1] = R * sn;
+        }
+        const double sq = sqrt(mb[3]);
+        const double sp = sqrt(mb[3] + mb[4]);
+        const double sr = sqrt(mb[5]);
+        const double tx = mb[0] * z[0] + mb[1] * z[1] + mb[2] * z[2];
+        const double ct = (sp - sq) * tx;
+        ub[0] = pre * (sq * z[0] + ct * mb[0]);
+        ub[1] = pre * (sq * z[1] + ct * mb[1]);
+        ub[2] = pre * (sq * z[2] + ct * mb[2]);
+        ub[3] = pre * (sr * z[3]);
+        ub[4] = pre * (sr * z[4]);
+        ub[5] = pre * (sr * z[5]);
+    }
+    double u[6];
+    if (fext) {
+        double fe[6], ue[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) fe[k] = fext[6 * i + k];
+        apply_M(mb, fe, c.mono, ue);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) u[k] = ue[k] + ub[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) u[k] = ub[k];
+    }
+    if (c.mono) {
+        u[2] = 0.0;
+        u[3] = 0.0;
+        u[4] = 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        mob[6 * i + k] = mb[k];
+        unc[6 * i + k] = u[k];
+    }
+}
+
+HipErr launch_mob_unc(hipStream_t st, int32_t n, const int32_t *gid, const double *dir,
+                      const double *length, const double *radius, const uint8_t *immovable,
+                      const double *fext, const CfgDev &c, double *mob, double *unc) {
+    if (n == 0) return hip_ok();
+    k_mob_unc<<<nblk(n, 256), 256, 0, st>>>(n, gid, dir, length, radius, immovable, fext, c, mob, unc);
+    LAUNCH_CHECK();
+    return hip_ok();
+}
+
+// ------------------------------------------------------------------------------------------------
+// AABB (inflated by collision radius + colBuf/2 + margin) and its cell range per axis.  Periodic
+// ranges are unwrapped integers; non-periodic ranges are clamped to the grid.
+// ------------------------------------------------------------------------------------------------
+__global__ void k_aabb(RodsDev r, CfgDev c, Grid g, int32_t *__restrict__ clo,
+                       int32_t *__restrict__ chi, double *__restrict__ blo,
+                       double *__restrict__ bhi, uint32_t *__restrict__ ecnt) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= r.n) return;
+    const double hh = r.hcol[i];
+    const double infl = r.rcol[i] + 0.5 * c.col_buf + kAabbMargin;
+    uint32_t cnt = 1;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double hw = hh * fabs(r.dir[3 * i + k]);
+        const double lo = (r.center[3 * i + k] - hw) - infl;
+        const double hi = (r.center[3 * i + k] + hw) + infl;
+        int a = (int)floor((lo - c.box_lo[k]) / g.h[k]);
+        int b = (int)floor((hi - c.box_lo[k]) / g.h[k]);
+        if (!c.pbc[k]) {
+            a = a < 0 ? 0 : (a >= g.nc[k] ? g.nc[k] - 1 : a);
+            b = b < 0 ? 0 : (b >= g.nc[k] ? g.nc[k] - 1 : b);
+        }
+        clo[3 * i + k] = a;
+        chi[3 * i + k] = b;
+        blo[3 * i + k] = lo;
+        bhi[3 * i + k] = hi;
+        cnt *= (uint32_t)(b - a + 1);
+    }
+    ecnt[i] = cnt;
+}
+
+HipErr launch_aabb(hipStream_t st, const RodsDev &r, const CfgDev &c, const Grid &g, int32_t *clo,
+                   int32_t *chi, double *blo, double *bhi, uint32_t *ecnt) {
+    if (r.n == 0) return hip_ok();
+    k_aabb<<<nblk(r.n, 256), 256, 0, st>>>(r, c, g, clo, chi, blo, bhi, ecnt);
+    LAUNCH_CHECK();
+    return hip_ok();
+}
+
+// ------------------------------------------------------------------------------------------------
+// Exclusive scan of u32 (out has n + 1 entries, out[n] = total).  Tiles of 2048 (256 x 8):
+// tile sums -> recursive scan of the sums -> per-tile scan with the tile offset.
+// ------------------------------------------------------------------------------------------------
+constexpr int kScanT = 256, kScanI = 8, kScanTile = kScanT * kScanI;
+
+__device__ inline uint32_t wave_incl_scan(uint32_t v, int lane) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// block-wide exclusive scan of one value per thread; returns exclusive prefix, total via *tot
+__device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t *sh, uint32_t *tot) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t inc = wave_incl_scan(v, lane);
+    if (lane == 63) sh[w] = inc;
+    __syncthreads();
+    uint32_t off = 0, all = 0;
+#pragma unroll
+    for (int q = 0; q < kScanT / 64; ++q) {
+        if (q < w) off += sh[q];
+        all += sh[q];
+    }
+    __syncthreads();
+    *tot = all;
+    return off + inc - v;
+}
+
+__global__ void k_scan_tilesum(const uint32_t *__restrict__ in, int64_t n, uint32_t *__restrict__ sums) {
+    __shared__ uint32_t sh[kScanT / 64];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanI; ++j) {
+        const int64_t idx = base + (int64_t)threadIdx.x * kScanI + j;
+        if (idx < n) s += in[idx];
+    }
+    uint32_t tot;
+    block_excl_scan(s, sh, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ void k_scan_tile(const uint32_t *__restrict__ in, int64_t n,
+                            const uint32_t *__restrict__ tileoff, uint32_t *__restrict__ out) {
+    __shared__ uint32_t sh[kScanT / 64];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    uint32_t v[kScanI];
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanI; ++j) {
+        const int64_t idx = base + (int64_t)threadIdx.x * kScanI + j;
+        v[j] = idx < n ? in[idx] : 0u;
+        s += v[j];
+    }
+    uint32_t tot;
+    uint32_t pre = block_excl_scan(s, sh, &tot) + (tileoff ? tileoff[blockIdx.x] : 0u);
+#pragma unroll
+    for (int j = 0; j < kScanI; ++j) {
+        const int64_t idx = base + (int64_t)threadIdx.x * kScanI + j;
+        if (idx < n) out[idx] = pre;
+        pre += v[j];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanT - 1) out[n] = pre;
+}
+
+static HipErr scan_level(hipStream_t st, const uint32_t *in, uint32_t *out, int64_t n,
+                         ScanScratch &s, int lvl) {
+    const int64_t nt = (n + kScanTile - 1) / kScanTile;
+    if (nt <= 1) {
+        k_scan_tile<<<1, kScanT, 0, st>>>(in, n, nullptr, out);
+        LAUNCH_CHECK();
+        return hip_ok();
+    }
+    if (lvl >= 3) return HipErr{hipErrorInvalidValue, "scan depth", __FILE__, __LINE__};
+    ALENS_HIP_TRY(s.lvl[lvl].ensure((size_t)(2 * nt + 2)));
+    uint32_t *sums = s.lvl[lvl].p;
+    uint32_t *offs = s.lvl[lvl].p + nt + 1;
+    k_scan_tilesum<<<(unsigned)nt, kScanT, 0, st>>>(in, n, sums);
+    LAUNCH_CHECK();
+    HipErr e = scan_level(st, sums, offs, nt, s, lvl + 1);
+    if (!e.ok()) return e;
+    k_scan_tile<<<(unsigned)nt, kScanT, 0, st>>>(in, n, offs, out);
+    LAUNCH_CHECK();
+    return hip_ok();
+}
+
+HipErr exclusive_scan_u32(hipStream_t st, const uint32_t *in, uint32_t *out, int64_t n,
+                          ScanScratch &s) {
+    if (n == 0) {
+        ALENS_HIP_TRY(hipMemsetAsync(out, 0, sizeof(uint32_t), st));
+        return hip_ok();
+    }
+    return scan_level(st, in, out, n, s, 0);
+}
+
+
+// ------------------------------------------------------------------------------------------------
+// cell entries: rod i writes (wrapped cell id, i) for every cell of its range, rod-major, so the
+// stable sort below leaves each cell's rods in ascending index order.
+// ------------------------------------------------------------------------------------------------
+__device__ inline int pmod_i(int a, int n) {
+    int r = a % n;
+    return r < 0 ? r + n : r;
+}
+
+__global__ void k_fill_entries(int32_t n, const int32_t *__restrict__ clo,
+                               const int32_t *__restrict__ chi, const uint32_t *__restrict__ eoff,
+                               Grid g, uint32_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t w = eoff[i];
+    const int a0 = clo[3 * i], a1 = clo[3 * i + 1], a2 = clo[3 * i + 2];
+    const int b0 = chi[3 * i], b1 = chi[3 * i + 1], b2 = chi[3 * i + 2];
+    for (int z = a2; z <= b2; ++z) {
+        const uint32_t cz = (uint32_t)pmod_i(z, g.nc[2]);
+        for (int y = a1; y <= b1; ++y) {
+            const uint32_t cy = (uint32_t)pmod_i(y, g.nc[1]);
+            for (int x = a0; x <= b0; ++x) {
+                const uint32_t cx = (uint32_t)pmod_i(x, g.nc[0]);
+                keys[w] = (cz * (uint32_t)g.nc[1] + cy) * (uint32_t)g.nc[0] + cx;
+                vals[w] = (uint32_t)i;
+                ++w;
+            }
+        }
+    }
+}
+
+HipErr launch_fill_entries(hipStream_t st, int32_t n, const int32_t *clo, const int32_t *chi,
+                           const uint32_t *eoff, const Grid &g, uint32_t *keys, uint32_t *vals) {
+    if (n == 0) return hip_ok();
+    k_fill_entries<<<nblk(n, 256), 256, 0, st>>>(n, clo, chi, eoff, g, keys, vals);
+    LAUNCH_CHECK();
+    return hip_ok();
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+HipErr launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, int64_t n) {
+    if (n == 0) return hip_ok();
+    k_fill_u32<<<nblk(n, 256), 256, 0, st>>>(p, v, n);
+    LAUNCH_CHECK();
+    return hip_ok();
+}
+
+// ------------------------------------------------------------------------------------------------
+// LSD radix sort of (u32 key, u32 value), 8-bit digits, stable.  Per pass: tile histograms
+// (digit-major), one exclusive scan over all (digit, tile) counters, then a stable scatter where
+// each 64-lane wave ranks equal digits with 8 ballots and the tile keeps a running offset per
+// digit across its rounds.  Tile = 256 threads x 8 rounds; round r covers items r*256 .. r*256+255.
+// ------------------------------------------------------------------------------------------------
+constexpr int kRsT = 256, kRsR = 8, kRsTile = kRsT * kRsR;
+
+__global__ void k_radix_hist(const uint32_t *__restrict__ keys, int64_t n, int shift, int ntiles,
+                             uint32_t *__restrict__ hist) {
+    __shared__ uint32_t lh[256];
+    lh[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kRsTile;
+#pragma unroll
+    for (int r = 0; r < kRsR; ++r) {
+        const int64_t idx = base + r * kRsT + threadIdx.x;
+        if (idx < n) atomicAdd(&lh[(keys[idx] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = lh[threadIdx.x];
+}
+
+__global__ void k_radix_scatter(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
+                                int64_t n, int shift, int ntiles, const uint32_t *__restrict__ off,
+                                uint32_t *__restrict__ okeys, uint32_t *__restrict__ ovals) {
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wc[kRsT / 64][256];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    run[threadIdx.x] = off[(int64_t)threadIdx.x * ntiles + blockIdx.x];
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int64_t base = (int64_t)blockIdx.x * kRsTile;
+    for (int r = 0; r < kRsR; ++r) {
+#pragma unroll
+        for (int q = 0; q < kRsT / 64; ++q) wc[q][threadIdx.x] = 0;
+        __syncthreads();
+        const int64_t idx = base + r * kRsT + threadIdx.x;
+        const bool valid = idx < n;
+        const uint32_t key = valid ? keys[idx] : 0u;
+        const uint32_t val = valid ? vals[idx] : 0u;
+        const uint32_t d = (key >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            const bool on = (d >> bit) & 1u;
+            const uint64_t m = __ballot(valid && on);
+            peers &= on ? m : ~m;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt);
+        if (valid && rank == 0) wc[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = run[d] + rank;
+            for (int q = 0; q < w; ++q) pos += wc[q][d];
+            okeys[pos] = key;
+            ovals[pos] = val;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+#pragma unroll
+        for (int q = 0; q < kRsT / 64; ++q) add += wc[q][threadIdx.x];
+        run[threadIdx.x] += add;
+        __syncthreads();
+    }
+}
+
+HipErr radix_sort_pairs(hipStream_t st, uint32_t *keys, uint32_t *vals, uint32_t *keys_alt,
+                        uint32_t *vals_alt, int64_t n, int keybits, DBuf<uint32_t> &hist,
+                        ScanScratch &s, bool &result_in_alt) {
+    result_in_alt = false;
+    if (n <= 1 || keybits <= 0) return hip_ok();
+    const int ntiles = (int)((n + kRsTile - 1) / kRsTile);
+    const int64_t nh = (int64_t)256 * ntiles;
+    ALENS_HIP_TRY(hist.ensure((size_t)(2 * nh + 1)));
+    uint32_t *h = hist.p, *ho = hist.p + nh;
+    uint32_t *ka = keys, *va = vals, *kb = keys_alt, *vb = vals_alt;
+    for (int shift = 0; shift < keybits; shift += 8) {
+        k_radix_hist<<<ntiles, kRsT, 0, st>>>(ka, n, shift, ntiles, h);
+        LAUNCH_CHECK();
+        HipErr e = exclusive_scan_u32(st, h, ho, nh, s);
+        if (!e.ok()) return e;
+        k_radix_scatter<<<ntiles, kRsT, 0, st>>>(ka, va, n, shift, ntiles, ho, kb, vb);
+        LAUNCH_CHECK();
+        uint32_t *t = ka;
+        ka = kb;
+        kb = t;
+        t = va;
+        va = vb;
+        vb = t;
+        result_in_alt = !result_in_alt;
+    }
+    return hip_ok();
+}
+
+// ------------------------------------------------------------------------------------------------
+// cell ranges in the sorted entry list (cells without entries keep start = end = 0)
+// ------------------------------------------------------------------------------------------------
+__global__ void k_cell_ranges(const uint32_t *__restrict__ keys, int64_t E,
+                              uint32_t *__restrict__ cstart, uint32_t *__restrict__ cend) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= E) return;
+    const uint32_t k = keys[i];
+    if (i == 0 || keys[i - 1] != k) cstart[k] = (uint32_t)i;
+    if (i == E - 1 || keys[i + 1] != k) cend[k] = (uint32_t)(i + 1);
+}
+
+HipErr launch_cell_ranges(hipStream_t st, const uint32_t *keys, int64_t E, uint32_t *cstart,
+                          uint32_t *cend) {
+    if (E == 0) return hip_ok();
+    k_cell_ranges<<<nblk(E, 256), 256, 0, st>>>(keys, E, cstart, cend);
+    LAUNCH_CHECK();
+    return hip_ok();
+}
+
+// ------------------------------------------------------------------------------------------------
+// Pair sweep: one 64-lane wave per cell.  For entries a < b of the cell (rods i < j) the pair is
+// examined here only if, on every axis, this cell is the first cell of i's or of j's range
+// (unwrapped), which visits each (pair, image) once; the image shift of j follows from the
+// unwrapped cell indices.  AABB reject, then the FP64 segment distance and delta0 < colBuf.
+// Emission is wave-aggregated; per-rod counts for both sides are accumulated for the ordering.
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_sweep(RodsDev r, CfgDev c, Grid g,
+                                               const uint32_t *__restrict__ cstart,
+                                               const uint32_t *__restrict__ cend,
+                                               const uint32_t *__restrict__ vals,
+                                               const int32_t *__restrict__ clo,
+                                               const double *__restrict__ blo,
+                                               const double *__restrict__ bhi,
+                                               uint32_t *__restrict__ pair_count, uint32_t cap,
+                                               uint32_t *__restrict__ pI, uint32_t *__restrict__ pJ,
+                                               uint32_t *__restrict__ pImg,
+                                               uint32_t *__restrict__ cntI,
+                                               uint32_t *__restrict__ cntJ) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int64_t wpb = blockDim.x >> 6;
+    const int64_t gw = (int64_t)blockIdx.x * wpb + (threadIdx.x >> 6);
+    const int64_t nw = (int64_t)gridDim.x * wpb;
+    for (int64_t cell = gw; cell < g.ncell; cell += nw) {
+        const uint32_t s0 = cstart[cell], s1 = cend[cell];
+        if (s1 - s0 < 2) continue;
+        const int cc0 = (int)(cell % g.nc[0]);
+        const int cc1 = (int)((cell / g.nc[0]) % g.nc[1]);
+        const int cc2 = (int)(cell / ((int64_t)g.nc[0] * g.nc[1]));
+        const int cc[3] = {cc0, cc1, cc2};
+        for (uint32_t pa = s0; pa + 1 < s1; ++pa) {
+            const uint32_t i = vals[pa];
+            int cli[3], ui[3];
+            double bli[3], bhi_i[3], ci[3], ti[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                cli[k] = clo[3 * i + k];
+                ui[k] = c.pbc[k] ? cli[k] + pmod_i(cc[k] - cli[k], g.nc[k]) : cc[k];
+                bli[k] = blo[3 * i + k];
+                bhi_i[k] = bhi[3 * i + k];
+                ci[k] = r.center[3 * i + k];
+                ti[k] = r.dir[3 * i + k];
+            }
+            const double hi_ = r.hcol[i], ri = r.rcol[i];
+            for (uint32_t pb0 = pa + 1; pb0 < s1; pb0 += 64) {
+                const uint32_t pb = pb0 + lane;
+                const bool valid = pb < s1;
+                bool pass = false;
+                uint32_t j = 0;
+                int m[3] = {0, 0, 0};
+                if (valid) {
+                    j = vals[pb];
+                    bool ok = true;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const int clj = clo[3 * j + k];
+                        const int uj = c.pbc[k] ? clj + pmod_i(cc[k] - clj, g.nc[k]) : cc[k];
+                        ok = ok && (ui[k] == cli[k] || uj == clj);
+                        m[k] = c.pbc[k] ? (ui[k] - uj) / g.nc[k] : 0;
+                        const double sh = c.pbc[k] ? (double)m[k] * c.len[k] : 0.0;
+                        ok = ok && (blo[3 * j + k] + sh <= bhi_i[k]) && (bli[k] <= bhi[3 * j + k] + sh);
+                    }
+                    if (ok) {
+                        double cj[3];
+#pragma unroll
+                        for (int k = 0; k < 3; ++k)
+                            cj[k] = c.pbc[k] ? r.center[3 * j + k] + (double)m[k] * c.len[k] : r.center[3 * j + k];
+                        const SegRes sg = segseg(ci[0], ci[1], ci[2], ti[0], ti[1], ti[2], hi_, cj[0],
+                                                 cj[1], cj[2], r.dir[3 * j], r.dir[3 * j + 1],
+                                                 r.dir[3 * j + 2], r.hcol[j]);
+                        const double d0 = (sg.dist - ri) - r.rcol[j];
+                        pass = d0 < c.col_buf;
+                    }
+                }
+                const uint64_t mask = __ballot(pass);
+                if (mask) {
+                    const int leader = __ffsll((unsigned long long)mask) - 1;
+                    uint32_t base = 0;
+                    if (lane == leader) {
+                        base = atomicAdd(pair_count, (uint32_t)__popcll(mask));
+                        atomicAdd(&cntI[i], (uint32_t)__popcll(mask));
+                    }
+                    base = __shfl(base, leader, 64);
+                    if (pass) {
+                        const uint32_t slot = base + (uint32_t)__popcll(mask & lt);
+                        if (slot < cap) {
+                            pI[slot] = i;
+                            pJ[slot] = j;
+                            pImg[slot] = img_pack(m[0], m[1], m[2]);
+                        }
+                        atomicAdd(&cntJ[j], 1u);
+                    }
+                }
+            }
+        }
+    }
+}
+
+HipErr launch_sweep(hipStream_t st, const RodsDev &r, const CfgDev &c, const Grid &g,
+                    const uint32_t *cstart, const uint32_t *cend, const uint32_t *vals,
+                    const int32_t *clo, const double *blo, const double *bhi, uint32_t *pair_count,
+                    uint32_t cap, uint32_t *pI, uint32_t *pJ, uint32_t *pImg, uint32_t *cntI,
+                    uint32_t *cntJ, int n_cu) {
+    if (g.ncell == 0) return hip_ok();
+    int64_t want = (g.ncell + 3) / 4;
+    int64_t lim = (int64_t)n_cu * 32;
+    if (want > lim) want = lim;
+    if (want < 1) want = 1;
+    k_sweep<<<(unsigned)want, 256, 0, st>>>(r, c, g, cstart, cend, vals, clo, blo, bhi, pair_count,
+                                            cap, pI, pJ, pImg, cntI, cntJ);
+    LAUNCH_CHECK();
+    return hip_ok();
+}
+
+// ------------------------------------------------------------------------------------------------
+// Canonical order: pairs grouped by I (scatter to rowStart[I] + atomic cursor), then each I-group
+// sorted by J by one thread (groups hold a handful of contacts).  The J-side lists hold constraint
+// indices, sorted ascending per rod.
+// ------------------------------------------------------------------------------------------------
+__global__ void k_scatter_I(uint32_t m, const uint32_t *__restrict__ pI, const uint32_t *__restrict__ pJ,
+                            const uint32_t *__restrict__ pImg, const uint32_t *__restrict__ rowStart,
+                            uint32_t *__restrict__ cur, int32_t *__restrict__ cI,
+                            int32_t *__restrict__ cJ, uint32_t *__restrict__ cImg) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= m) return;
+    const uint32_t i = pI[p];
+    const uint32_t pos = rowStart[i] + atomicAdd(&cur[i], 1u);
+    cI[pos] = (int32_t)i;
+    cJ[pos] = (int32_t)pJ[p];
+    cImg[pos] = pImg[p];
+}
+
+__global__ void k_sort_I(int32_t n, const uint32_t *__restrict__ rowStart, int32_t *__restrict__ cJ,
+                         uint32_t *__restrict__ cImg) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s0 = rowStart[i], s1 = rowStart[i + 1];
+    for (uint32_t a = s0 + 1; a < s1; ++a) {
+        const int32_t kj = cJ[a];
+        const uint32_t km = cImg[a];
+        uint32_t b = a;
+        while (b > s0 && cJ[b - 1] > kj) {
+            cJ[b] = cJ[b - 1];
+            cImg[b] = cImg[b - 1];
+            --b;
+        }
+        cJ[b] = kj;
+        cImg[b] = km;
+    }
+}
+
+HipErr launch_order_I(hipStream_t st, int32_t n, uint32_t m, const uint32_t *pI,
+                      const uint32_t *pJ, const uint32_t *pImg, const uint32_t *rowStart,
+                      uint32_t *cur, int32_t *cI, int32_t *cJ, uint32_t *cImg) {
+    if (m == 0) return hip_ok();
+    k_scatter_I<<<nblk(m, 256), 256, 0, st>>>(m, pI, pJ, pImg, rowStart, cur, cI, cJ, cImg);
+    LAUNCH_CHECK();
+    k_sort_I<<<nblk(n, 256), 256, 0, st>>>(n, rowStart, cJ, cImg);
+    LAUNCH_CHECK();
+    return hip_ok();
+}
+
+__global__ void k_scatter_J(uint32_t m, const int32_t *__restrict__ cJ, const uint32_t *__restrict__ jStart,
+                            uint32_t *__restrict__ cur, uint32_t *__restrict__ jlist) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const uint32_t j = (uint32_t)cJ[k];
+    jlist[jStart[j] + atomicAdd(&cur[j], 1u)] = k;
+}
+
+__global__ void k_sort_J(int32_t n, const uint32_t *__restrict__ jStart, uint32_t *__restrict__ jlist) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s0 = jStart[i], s1 = jStart[i + 1];
+    for (uint32_t a = s0 + 1; a < s1; ++a) {
+        const uint32_t v = jlist[a];
+        uint32_t b = a;
+        while (b > s0 && jlist[b - 1] > v) {
+            jlist[b] = jlist[b - 1];
+            --b;
+        }
+        jlist[b] = v;
+    }
+}
+
+HipErr launch_order_J(hipStream_t st, int32_t n, uint32_t m, const int32_t *cJ,
+                      const uint32_t *jStart, uint32_t *cur, uint32_t *jlist) {
+    if (m == 0) return hip_ok();
+    k_scatter_J<<<nblk(m, 256), 256, 0, st>>>(m, cJ, jStart, cur, jlist);
+    LAUNCH_CHECK();
+    k_sort_J<<<nblk(n, 256), 256, 0, st>>>(n, jStart, jlist);
+    LAUNCH_CHECK();
+    return hip_ok();
+}
+
+// ------------------------------------------------------------------------------------------------
+// Constraint rows: delta0, n (J -> I), posI = s tI, posJ = u tJ, rI = posI x n, rJ = posJ x n,
+// b = delta0 + dt (D^T U_nc).  geo[k] = {n, rI, rJ}.
+// ------------------------------------------------------------------------------------------------
+__global__ void k_rows(RodsDev r, CfgDev c, uint32_t m, const int32_t *__restrict__ cI,
+                       const int32_t *__restrict__ cJ, const uint32_t *__restrict__ cImg,
+                       const double *__restrict__ unc, double *__restrict__ delta0,
+                       double *__restrict__ geo, double *__restrict__ posI,
+                       double *__restrict__ posJ, double *__restrict__ b) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const int32_t i = cI[k], j = cJ[k];
+    const uint32_t im = cImg[k];
+    double cj[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        cj[a] = c.pbc[a] ? r.center[3 * j + a] + (double)img_get(im, a) * c.len[a] : r.center[3 * j + a];
+    const double tI0 = r.dir[3 * i], tI1 = r.dir[3 * i + 1], tI2 = r.dir[3 * i + 2];
+    const double tJ0 = r.dir[3 * j], tJ1 = r.dir[3 * j + 1], tJ2 = r.dir[3 * j + 2];
+    const SegRes sg = segseg(r.center[3 * i], r.center[3 * i + 1], r.center[3 * i + 2], tI0, tI1, tI2,
+                             r.hcol[i], cj[0], cj[1], cj[2], tJ0, tJ1, tJ2, r.hcol[j]);
+    double nk[3];
+    contact_normal(sg, tI0, tI1, tI2, tJ0, tJ1, tJ2, nk);
+    const double pI[3] = {sg.s * tI0, sg.s * tI1, sg.s * tI2};
+    const double pJ[3] = {sg.u * tJ0, sg.u * tJ1, sg.u * tJ2};
+    double g9[9];
+    g9[0] = nk[0];
+    g9[1] = nk[1];
+    g9[2] = nk[2];
+    g9[3] = pI[1] * nk[2] - pI[2] * nk[1];
+    g9[4] = pI[2] * nk[0] - pI[0] * nk[2];
+    g9[5] = pI[0] * nk[1] - pI[1] * nk[0];
+    g9[6] = pJ[1] * nk[2] - pJ[2] * nk[1];
+    g9[7] = pJ[2] * nk[0] - pJ[0] * nk[2];
+    g9[8] = pJ[0] * nk[1] - pJ[1] * nk[0];
+    const double d0 = (sg.dist - r.rcol[i]) - r.rcol[j];
+    delta0[k] = d0;
+#pragma unroll
+    for (int a = 0; a < 9; ++a) geo[9 * (size_t)k + a] = g9[a];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        posI[3 * (size_t)k + a] = pI[a];
+        posJ[3 * (size_t)k + a] = pJ[a];
+    }
+    b[k] = d0 + c.dt * dt_row(g9, unc + 6 * (size_t)i, unc + 6 * (size_t)j);
+}
+
+HipErr launch_rows(hipStream_t st, const RodsDev &r, const CfgDev &c, uint32_t m, const int32_t *cI,
+                   const int32_t *cJ, const uint32_t *cImg, const double *unc, double *delta0,
+                   double *geo, double *posI, double *posJ, double *b) {
+    if (m == 0) return hip_ok();
+    k_rows<<<nblk(m, 256), 256, 0, st>>>(r, c, m, cI, cJ, cImg, unc, delta0, geo, posI, posJ, b);
+    LAUNCH_CHECK();
+    return hip_ok();
+}
+
+// ------------------------------------------------------------------------------------------------
+// Rod-major adjacency in ascending constraint index: the J-side list (constraints with J = i all
+// have I < i, so they precede i's own rows) followed by i's I-side rows.  Each entry carries the
+// rod's column block of D: (n, rI) on the I side, (-n, -rJ) on the J side.
+// ------------------------------------------------------------------------------------------------
+__global__ void k_adjacency(int32_t n, const uint32_t *__restrict__ rowStart,
+                            const uint32_t *__restrict__ jStart, const uint32_t *__restrict__ jlist,
+                            const double *__restrict__ geo, uint32_t *__restrict__ adjStart,
+                            uint32_t *__restrict__ adjK, double *__restrict__ adjW) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    const uint32_t e0 = rowStart[i] + jStart[i];
+    adjStart[i] = e0;
+    if (i == n) return;
+    const uint32_t j0 = jStart[i], j1 = jStart[i + 1];
+    uint32_t e = e0;
+    for (uint32_t t = j0; t < j1; ++t, ++e) {
+        const uint32_t k = jlist[t];
+        const double *gk = geo + 9 * (size_t)k;
+        adjK[e] = k | 0x80000000u;
+        double *w = adjW + 6 * (size_t)e;
+        w[0] = -gk[0];
+        w[1] = -gk[1];
+        w[2] = -gk[2];
+        w[3] = -gk[6];
+        w[4] = -gk[7];
+        w[5] = -gk[8];
+    }
+    for (uint32_t k = rowStart[i]; k < rowStart[i + 1]; ++k, ++e) {
+        const double *gk = geo + 9 * (size_t)k;
+        adjK[e] = k;
+        double *w = adjW + 6 * (size_t)e;
+        w[0] = gk[0];
+        w[1] = gk[1];
+        w[2] = gk[2];
+        w[3] = gk[3];
+        w[4] = gk[4];
+        w[5] = gk[5];
+    }
+}
+
+HipErr launch_adjacency(hipStream_t st, int32_t n, const uint32_t *rowStart,
+                        const uint32_t *jStart, const uint32_t *jlist, const double *geo,
+                        uint32_t *adjStart, uint32_t *adjK, double *adjW) {
+    k_adjacency<<<nblk((int64_t)n + 1, 256), 256, 0, st>>>(n, rowStart, jStart, jlist, geo, adjStart,
+                                                          adjK, adjW);
+    LAUNCH_CHECK();
+    return hip_ok();
+}
+
+}  // namespace alens

@@ -1,0 +1,108 @@
+"""Seeded synthetic rod systems for the benchmark workloads, and a reader/writer for the
+reference's rod file format (TubuleInitial.dat / SylinderAscii lines
+`C|S gid radius minus_x minus_y minus_z plus_x plus_y plus_z [group]`, docs/source/quickstart.md:215-221).
+
+Each generator returns (rods, cfg): rods = dict(gid, center (n,3), dir (n,3), length, radius,
+immovable); cfg = keyword arguments for alens_amd.capi.Context / make_config.
+"""
+import numpy as np
+
+RADIUS = 0.0125  # um (diameter 0.025)
+COMMON = dict(col_buf=0.025, viscosity=0.01, kT=0.00411, dt=1e-4, tol=1e-5, max_ite=100000, seed=1234)
+
+
+def _iso_dirs(rng, n):
+    z = rng.uniform(-1.0, 1.0, n)
+    phi = rng.uniform(0.0, 2.0 * np.pi, n)
+    s = np.sqrt(1.0 - z * z)
+    d = np.stack([s * np.cos(phi), s * np.sin(phi), z], axis=1)
+    return d / np.linalg.norm(d, axis=1, keepdims=True)
+
+
+def _rods(center, dirs, length, radius=RADIUS, gid=None):
+    n = len(length)
+    return dict(gid=np.arange(n, dtype=np.int32) if gid is None else np.asarray(gid, np.int32),
+                center=np.ascontiguousarray(center, np.float64), dir=np.ascontiguousarray(dirs, np.float64),
+                length=np.ascontiguousarray(length, np.float64), radius=np.full(n, float(radius)), immovable=None)
+
+
+def small_3d(n=1000, box=4.0, seed=1234):
+    """1,000 rods, L ~ U[0.5, 1.5] um, isotropic, periodic 4^3 um^3."""
+    rng = np.random.default_rng(seed)
+    L = rng.uniform(0.5, 1.5, n)
+    c = rng.uniform(0.0, box, (n, 3))
+    cfg = dict(COMMON, box_lo=(0, 0, 0), box_hi=(box,) * 3, pbc=(1, 1, 1), monolayer=0, seed=seed)
+    return _rods(c, _iso_dirs(rng, n), L), cfg
+
+
+def monolayer_slab(n=200_000, side=100.0, thick=0.05, kappa=4.0, seed=1234):
+    """Dense quasi-2D layer: L = 1 um, centres in a side x side x thick slab (periodic in x, y),
+    in-plane directions from a von Mises(0, kappa) distribution about x.  Rods move in 3-D (stacked
+    crossings inside the slab are resolved by out-of-plane motion)."""
+    rng = np.random.default_rng(seed)
+    c = np.stack([rng.uniform(0, side, n), rng.uniform(0, side, n), rng.uniform(0, thick, n)], axis=1)
+    th = rng.vonmises(0.0, kappa, n)
+    d = np.stack([np.cos(th), np.sin(th), np.zeros(n)], axis=1)
+    cfg = dict(COMMON, box_lo=(0, 0, 0), box_hi=(side, side, thick), pbc=(1, 1, 0), monolayer=0, seed=seed)
+    return _rods(c, d, np.ones(n)), cfg
+
+
+def lognormal_3d(n=1_000_000, box=25.0, median=1.0, sigma=0.3, lmax=3.0, seed=1234):
+    """1M rods, L ~ lognormal(ln median, sigma) resampled above lmax, periodic box^3, isotropic."""
+    rng = np.random.default_rng(seed)
+    L = rng.lognormal(np.log(median), sigma, n)
+    bad = L > lmax
+    while bad.any():
+        L[bad] = rng.lognormal(np.log(median), sigma, int(bad.sum()))
+        bad = L > lmax
+    c = rng.uniform(0.0, box, (n, 3))
+    cfg = dict(COMMON, box_lo=(0, 0, 0), box_hi=(box,) * 3, pbc=(1, 1, 1), monolayer=0, seed=seed)
+    return _rods(c, _iso_dirs(rng, n), L), cfg
+
+
+def scaled(kind, n, seed=1234):
+    """Same density and shape law as a named workload at a smaller rod count."""
+    if kind == "small_3d":
+        return small_3d(n=n, box=4.0 * (n / 1000.0) ** (1 / 3), seed=seed)
+    if kind == "monolayer_slab":
+        return monolayer_slab(n=n, side=100.0 * (n / 200_000.0) ** 0.5, seed=seed)
+    if kind == "lognormal_3d":
+        return lognormal_3d(n=n, box=25.0 * (n / 1_000_000.0) ** (1 / 3), seed=seed)
+    raise ValueError(kind)
+
+
+def read_rod_file(path):
+    """Parse the reference's rod text format.  'S' rods are stationary (immovable)."""
+    gid, rad, mi, pl, imm = [], [], [], [], []
+    with open(path) as fh:
+        for line in fh:
+            tok = line.split()
+            if len(tok) < 9 or tok[0] not in ("C", "S"):
+                continue
+            gid.append(int(tok[1]))
+            rad.append(float(tok[2]))
+            mi.append([float(x) for x in tok[3:6]])
+            pl.append([float(x) for x in tok[6:9]])
+            imm.append(1 if tok[0] == "S" else 0)
+    mi = np.array(mi, dtype=np.float64).reshape(-1, 3)
+    pl = np.array(pl, dtype=np.float64).reshape(-1, 3)
+    v = pl - mi
+    L = np.linalg.norm(v, axis=1)
+    d = v / np.where(L > 0, L, 1.0)[:, None]
+    d[L == 0] = (0.0, 0.0, 1.0)
+    return dict(gid=np.array(gid, np.int32), center=0.5 * (mi + pl), dir=d, length=L,
+                radius=np.array(rad, dtype=np.float64), immovable=np.array(imm, np.uint8))
+
+
+def write_rod_file(path, rods, header_time=0.0):
+    """Write rods in the reference's text format (minus end, plus end per rod)."""
+    c, d, L = rods["center"], rods["dir"], rods["length"]
+    imm = rods.get("immovable")
+    mi = c - 0.5 * L[:, None] * d
+    pl = c + 0.5 * L[:, None] * d
+    with open(path, "w") as fh:
+        fh.write(f"{len(L)}\n{header_time:f}\n")
+        for k in range(len(L)):
+            t = "S" if (imm is not None and imm[k]) else "C"
+            fh.write(f"{t} {int(rods['gid'][k])} {rods['radius'][k]:.17g} " +
+                     " ".join(f"{x:.17g}" for x in (*mi[k], *pl[k])) + "\n")

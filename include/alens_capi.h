@@ -1,0 +1,136 @@
+/*
+ * alens_capi.h -- C ABI of the MI355X collision-constraint engine (libalens.so).
+ *
+ * One context owns one rod system on one GPU: its device buffers, a HIP stream, the Brownian key
+ * and the previous step's contact impulses.  Every entry point is plain C: host pointers are
+ * borrowed for the duration of the call and copied; outputs are caller-allocated; errors come back
+ * as alens_status with a message from alens_last_error().  The library never exits the process.
+ * A context is used by one thread at a time; distinct contexts are independent.
+ *
+ * The reference reaches this path through SylinderSystem (SimToolbox, absent from the reference
+ * tree): TubuleSystem calls rodSystem.prepareStep() (SRC/TubuleSystem.cpp:120), pushes protein
+ * constraint blocks into the per-thread pool (SRC/TubuleSystem.cpp:694-745), then
+ * rodSystem.runStep() (SRC/TubuleSystem.cpp:223) runs Brownian + non-constraint velocity, collision
+ * collection, the BCQP solve and the write-back, and stepEuler() (SRC/TubuleSystem.cpp:55) moves
+ * the rods.  Each function below names the call it replaces.
+ */
+#ifndef ALENS_CAPI_H
+#define ALENS_CAPI_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct alens_ctx alens_ctx;
+
+typedef enum {
+    ALENS_OK = 0,
+    ALENS_E_ARG = 1,        /* invalid argument or configuration */
+    ALENS_E_HIP = 2,        /* HIP runtime error (message names the call) */
+    ALENS_E_OOM = 3,        /* device allocation failed */
+    ALENS_E_NONFINITE = 4,  /* NaN/Inf in the rod state or in the solver iterate */
+    ALENS_E_NOCONV = 5,     /* BBPGD reached max_ite above tol (results are still written) */
+    ALENS_E_BOX = 6,        /* periodic box shorter than twice the largest interaction range */
+    ALENS_E_STATE = 7       /* call order violated (e.g. step before set_rods) */
+} alens_status;
+
+/* Run configuration: the hot-path subset of SylinderConfig (docs/source/quickstart.md:316-358). */
+typedef struct {
+    double box_lo[3], box_hi[3]; /* simBoxLow / simBoxHigh */
+    int32_t pbc[3];              /* simBoxPBC */
+    int32_t monolayer;           /* monolayer: motion restricted to the x-y plane */
+    double col_buf;              /* sylinderColBuf, absolute (um) */
+    double viscosity;            /* viscosity, pN s / um^2 */
+    double kT;                   /* KBT, pN um; <= 0 disables Brownian motion */
+    double dt;                   /* dt, s */
+    double tol;                  /* conResTol */
+    double diam_col_ratio;       /* sylinderDiameterColRatio */
+    double len_col_ratio;        /* sylinderLengthColRatio */
+    double cell_size;            /* cell-list edge (um); <= 0 picks half the mean collision length */
+    int32_t max_ite;             /* conMaxIte */
+    int32_t solver;              /* conSolverChoice; 0 = BBPGD (the only solver provided) */
+    int32_t precision;           /* 0 = FP64; 1 = FP32 iterations + FP64 residual and refinement */
+    int32_t warm_start;          /* 0 = cold; 1 = previous-step gamma matched by (gid_i, gid_j) */
+    uint64_t seed;               /* rngSeed */
+    int32_t device;              /* HIP device ordinal */
+    int32_t reserved;
+} alens_config;
+
+/* One constraint, field for field the ConstraintBlock built at SRC/TubuleSystem.cpp:727-742. */
+typedef struct {
+    double delta0, gamma, kappa, gamma_lb;
+    int32_t gid_i, gid_j, gidx_i, gidx_j;
+    double norm_i[3], norm_j[3], pos_i[3], pos_j[3], lab_i[3], lab_j[3];
+    double stress[9];
+    int32_t one_side, bilateral;
+} alens_con_block;
+
+typedef struct {
+    int32_t iters;          /* BBPGD iterations of the final (FP64) solve */
+    int32_t matvecs;        /* FP64 applications of A */
+    int32_t converged;
+    int32_t fallback_used;  /* warm start rejected (raised the initial residual), cold restart */
+    int32_t status;         /* alens_status of the solve */
+    int32_t iters_fp32;     /* FP32 iterations before the FP64 refinement (precision = 1) */
+    int64_t n_con;          /* constraints collected this step */
+    int64_t n_entries;      /* cell-list entries */
+    double res0, res;       /* residual at the start point used, final FP64 residual */
+    double t_collect_ms;    /* device time: U_nc + cell list + pair sweep + rows */
+    double t_solve_ms;      /* device time of the BBPGD kernel(s) */
+    double t_step_ms;       /* device time of the whole step_constraints call */
+} alens_solve_report;
+
+/* Create / destroy a context (replaces SylinderSystem construction + initialize(),
+ * SRC/TubuleSystem.cpp:79, minus file IO). */
+alens_status alens_create(const alens_config *cfg, alens_ctx **out);
+void alens_destroy(alens_ctx *ctx);
+const char *alens_last_error(const alens_ctx *ctx);
+const char *alens_version(void);
+
+/* Load the rod state (Sylinder fields copied at SRC/TubuleBind.hpp:54-67: gid, radius, length,
+ * pos, direction = q e_z).  center/dir are 3n (x,y,z per rod), dir unit; immovable may be NULL.
+ * Rod order is kept; constraints are indexed by it. */
+alens_status alens_set_rods(alens_ctx *ctx, int32_t n, const int32_t *gid, const double *center,
+                            const double *dir, const double *length, const double *radius,
+                            const uint8_t *immovable);
+alens_status alens_get_rods(alens_ctx *ctx, double *center, double *dir);
+
+/* Step counter keying the Brownian stream (SylinderSystem's stepCount, SRC/TubuleSystem.cpp:114). */
+alens_status alens_set_step(alens_ctx *ctx, int64_t step);
+int64_t alens_get_step(const alens_ctx *ctx);
+
+/* One constraint step (rodSystem.runStep(), SRC/TubuleSystem.cpp:223): U_nc = M F_ext + U_B,
+ * collision collection, BBPGD to tol, write-back F_con = D gamma, U = U_nc + M F_con.
+ * f_ext is 6n (force, torque per rod) or NULL. */
+alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_solve_report *rep);
+
+/* Move rods with the last step's velocity (stepEuler(), SRC/TubuleSystem.cpp:55) and advance the
+ * step counter. */
+alens_status alens_step_euler(alens_ctx *ctx);
+
+/* Constraints of the last step in canonical order (ascending rod index I, then J; I < J).
+ * get_constraints fills blocks; get_constraint_arrays fills any non-NULL SoA array
+ * (I, J: m; img: 3m periodic image of J; delta0, b: m; nrm, pos_i, pos_j, r_i, r_j: 3m). */
+alens_status alens_get_num_constraints(alens_ctx *ctx, int64_t *m);
+alens_status alens_get_constraints(alens_ctx *ctx, int64_t cap, alens_con_block *out, int64_t *m);
+alens_status alens_get_constraint_arrays(alens_ctx *ctx, int32_t *I, int32_t *J, int32_t *img,
+                                         double *delta0, double *nrm, double *pos_i, double *pos_j,
+                                         double *r_i, double *r_j, double *b);
+alens_status alens_get_gamma(alens_ctx *ctx, double *gamma);
+/* u, u_nc, f_con: 6n (vx,vy,vz,wx,wy,wz per rod); any may be NULL. mob: 6n {t, 1/zperp,
+ * 1/zpar - 1/zperp, 1/zrot} or NULL. */
+alens_status alens_get_velocity(alens_ctx *ctx, double *u, double *u_nc, double *f_con, double *mob);
+
+/* Stand-alone BCQP solve (BCQPSolver::solveBBPGD on A = dt D^T M D, b; reached in the reference
+ * through ConstraintSolver, CMakeLists.txt:109-112).  Rows: I, J (m, I < J not required), nrm, r_i,
+ * r_j (3m), b (m); mob (6n) as in alens_get_velocity; x0 (m) or NULL; gamma_out (m). */
+alens_status alens_bcqp_solve(alens_ctx *ctx, int32_t n, int64_t m, const int32_t *I,
+                              const int32_t *J, const double *nrm, const double *r_i,
+                              const double *r_j, const double *b, const double *mob,
+                              const double *x0, double *gamma_out, alens_solve_report *rep);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

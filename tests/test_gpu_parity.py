@@ -1,0 +1,240 @@
+"""GPU parity: libalens.so (through the C ABI) against the FP64 CPU oracle on the same inputs.
+
+Both sides evaluate the same operation sequence in IEEE binary64 with no contraction and the same
+fixed reduction order, so the bar here is bit-for-bit: the pair set and every row field, the BBPGD
+iteration count, residuals, gamma, U_nc, F_con and U, and the rod state after Euler steps.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from alens_amd import capi, systems
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float64).view(np.uint64)
+
+
+def assert_bits(a, b, what):
+    a = np.asarray(a, np.float64).reshape(-1)
+    b = np.asarray(b, np.float64).reshape(-1)
+    assert a.shape == b.shape, f"{what}: shape {a.shape} vs {b.shape}"
+    bad = np.nonzero(bits(a) != bits(b))[0]
+    if bad.size:
+        k = bad[0]
+        raise AssertionError(f"{what}: {bad.size} of {a.size} differ; first at {k}: {a[k]!r} vs {b[k]!r}")
+
+
+def gpu_step(rods, cfg, **extra):
+    ctx = capi.Context(**dict(cfg, **extra))
+    ctx.set_rods(rods)
+    rep = ctx.step()
+    out = ctx.constraint_arrays()
+    out["gamma"] = ctx.gamma()
+    out.update(ctx.velocity())
+    out["report"] = rep
+    return ctx, out
+
+
+def compare_full(rods, cfg, g, o):
+    assert np.array_equal(g["I"], o["I"]) and np.array_equal(g["J"], o["J"]), "pair set differs"
+    assert np.array_equal(g["img"], o["img"]), "periodic images differ"
+    rw = o["rows"]
+    assert_bits(g["delta0"], rw["delta0"], "delta0")
+    assert_bits(g["nrm"], rw["nrm"], "normal")
+    assert_bits(g["posI"], rw["posI"], "posI")
+    assert_bits(g["posJ"], rw["posJ"], "posJ")
+    assert_bits(g["rI"], rw["rI"], "posI x n")
+    assert_bits(g["rJ"], rw["rJ"], "posJ x n")
+    assert_bits(g["mob"], o["mob"], "mobility")
+    assert_bits(g["unc"], o["unc"], "U_nc")
+    assert_bits(g["b"], rw["b"], "b")
+    gr, orp = g["report"], o["report"]
+    assert gr["iters"] == orp["iters"] and gr["matvecs"] == orp["matvecs"], (gr, orp)
+    assert gr["converged"] == orp["converged"]
+    assert_bits([gr["res0"], gr["res"]], [orp["res0"], orp["res"]], "residuals")
+    assert_bits(g["gamma"], o["gamma"], "gamma")
+    assert_bits(g["fcon"], o["fcon"], "F_con")
+    assert_bits(g["u"], o["u"], "U")
+
+
+def test_small_3d_step_bitexact():
+    rods, cfg = systems.small_3d()
+    ctx, g = gpu_step(rods, cfg)
+    o = O.step(rods, O.cfg_from_dict(cfg))
+    assert o["I"].size > 300
+    compare_full(rods, cfg, g, o)
+    assert g["report"]["converged"] == 1 and g["report"]["res"] <= cfg["tol"]
+
+
+def test_small_3d_trajectory_bitexact():
+    """Five full steps (constraints + Euler) stay bit-identical to the oracle's trajectory."""
+    rods, cfg = systems.small_3d(n=600, box=3.4, seed=77)
+    ctx = capi.Context(**cfg)
+    ctx.set_rods(rods)
+    oc = O.cfg_from_dict(cfg)
+    cur = dict(rods)
+    for step in range(5):
+        oc.step = step
+        o = O.step(cur, oc)
+        ctx.step()
+        g = ctx.constraint_arrays()
+        g["gamma"] = ctx.gamma()
+        g.update(ctx.velocity())
+        assert np.array_equal(g["I"], o["I"]) and np.array_equal(g["J"], o["J"])
+        assert_bits(g["gamma"], o["gamma"], f"gamma step {step}")
+        assert_bits(g["u"], o["u"], f"U step {step}")
+        ctx.euler()
+        c, t = O.euler(cur, o["u"], oc)
+        cur = dict(cur, center=c, dir=t)
+        gc, gt = ctx.get_rods()
+        assert_bits(gc, c, f"centers after step {step}")
+        assert_bits(gt, t, f"directions after step {step}")
+    assert ctx.step_count == 5
+
+
+def test_dense_monolayer_example_bitexact():
+    """The reference's DenseMonoLayer input (9,700 rods, 11 x 11 periodic, monolayer dynamics)."""
+    rods = systems.read_rod_file(os.path.join(GOLDEN, "dense_monolayer_TubuleInitial.dat"))
+    cfg = dict(box_lo=(0, 0, 0), box_hi=(11, 11, 11), pbc=(1, 1, 0), monolayer=1, col_buf=0.025, viscosity=1.0,
+               kT=0.00411, dt=1e-5, tol=1e-6, max_ite=100000, seed=1234)
+    ctx, g = gpu_step(rods, cfg)
+    o = O.step(rods, O.cfg_from_dict(cfg))
+    assert o["I"].size > 1000
+    compare_full(rods, cfg, g, o)
+    # monolayer dynamics: no out-of-plane motion
+    u = g["u"].reshape(-1, 6)
+    assert np.all(u[:, 2] == 0) and np.all(u[:, 3] == 0) and np.all(u[:, 4] == 0)
+
+
+def test_dense_straining_example_bitexact():
+    rods = systems.read_rod_file(os.path.join(GOLDEN, "dense_straining_TubuleInitial.dat"))
+    cfg = dict(box_lo=(0, 0, 0), box_hi=(3, 1, 1), pbc=(1, 0, 0), monolayer=0, col_buf=0.025, viscosity=1.0,
+               kT=-0.00411, dt=1e-4, tol=1e-5, max_ite=1000000, seed=1234)
+    ctx, g = gpu_step(rods, cfg)
+    o = O.step(rods, O.cfg_from_dict(cfg))
+    compare_full(rods, cfg, g, o)
+
+
+def test_mix_motor_sliding_has_no_contacts():
+    """Two parallel rods 0.07 um apart (gap 0.045 > colBuf 0.025): no constraint."""
+    rods = systems.read_rod_file(os.path.join(GOLDEN, "mix_motor_sliding_TubuleInitial.dat"))
+    cfg = dict(box_lo=(0, 0, 0), box_hi=(20, 1, 1), pbc=(0, 0, 0), col_buf=0.025, viscosity=1.0, kT=-0.00411,
+               dt=1e-5, tol=1e-5, max_ite=10000)
+    ctx, g = gpu_step(rods, cfg)
+    assert g["I"].size == 0
+    assert np.all(g["u"] == 0.0)  # no noise, no force, no contact
+    assert g["report"]["converged"] == 1
+
+
+@pytest.mark.parametrize("kind,n", [("monolayer_slab", 20000), ("lognormal_3d", 20000)])
+def test_scaled_workloads_bitexact(kind, n):
+    rods, cfg = systems.scaled(kind, n)
+    ctx, g = gpu_step(rods, cfg)
+    o = O.step(rods, O.cfg_from_dict(cfg))
+    compare_full(rods, cfg, g, o)
+
+
+def test_rerun_is_bit_identical():
+    rods, cfg = systems.scaled("lognormal_3d", 30000, seed=5)
+    _, a = gpu_step(rods, cfg)
+    _, b = gpu_step(rods, cfg)
+    assert np.array_equal(a["I"], b["I"])
+    assert_bits(a["gamma"], b["gamma"], "gamma rerun")
+    assert_bits(a["u"], b["u"], "U rerun")
+
+
+def test_bcqp_solve_matches_exact_lcp():
+    """Stand-alone BCQP entry on the 50-digit active-set solutions (tests/golden/lcp_small.json)."""
+    cases = json.load(open(os.path.join(GOLDEN, "lcp_small.json")))["cases"]
+    ctx = capi.Context(box_lo=(0, 0, 0), box_hi=(1, 1, 1), pbc=(0, 0, 0), tol=1e-13, max_ite=200000, dt=1e-4)
+    for cs in cases:
+        I = np.array(cs["I"], np.int32)
+        J = np.array(cs["J"], np.int32)
+        rw = {k: np.array(cs[k]) for k in ("nrm", "rI", "rJ", "b")}
+        mob = np.array(cs["mob"])
+        gam, rep = ctx.bcqp_solve(cs["n"], I, J, rw["nrm"], rw["rI"], rw["rJ"], rw["b"], mob)
+        exact = np.array([float(x) for x in cs["gamma"]])
+        assert np.max(np.abs(gam - exact)) <= 1e-9 * max(1.0, np.max(np.abs(exact))), cs
+        og, orep = O.bbpgd(cs["n"], I, J, rw, mob, O.make_cfg(dt=cs["dt"], tol=1e-13, max_ite=200000))
+        assert_bits(gam, og, "bcqp gamma vs oracle")
+        assert rep["iters"] == orep["iters"]
+
+
+def test_warm_start_previous_gamma_bitexact_and_fallback():
+    """Warm start from the previous step's gamma: same iterate path as the oracle given the same x0."""
+    rods, cfg = systems.small_3d(n=800, box=3.7, seed=11)
+    ctx = capi.Context(**dict(cfg, warm_start=1))
+    ctx.set_rods(rods)
+    oc = O.cfg_from_dict(cfg)
+    r0 = ctx.step()
+    g0 = ctx.constraint_arrays()
+    gam0 = ctx.gamma()
+    o0 = O.step(rods, oc)
+    assert_bits(gam0, o0["gamma"], "step 0 gamma")
+    ctx.euler()
+    c, t = O.euler(rods, o0["u"], oc)
+    rods1 = dict(rods, center=c, dir=t)
+    oc.step = 1
+    r1 = ctx.step()
+    g1 = ctx.constraint_arrays()
+    # the warm start the library used: previous gamma matched on (I, J)
+    key0 = {(int(i), int(j)): v for i, j, v in zip(g0["I"], g0["J"], gam0)}
+    x0 = np.array([key0.get((int(i), int(j)), 0.0) for i, j in zip(g1["I"], g1["J"])])
+    mob = O.mobility(rods1, oc)
+    un = O.unc(rods1, mob, oc)
+    I, J, img = O.pairs(rods1, oc)
+    rw = O.rows(rods1, oc, I, J, img, un)
+    og, orep = O.bbpgd(len(rods1["length"]), I, J, rw, mob, oc, x0=x0)
+    assert_bits(ctx.gamma(), og, "warm gamma")
+    assert r1["iters"] == orep["iters"] and r1["fallback_used"] == orep["fallback_used"]
+    assert r1["iters"] < r0["iters"]
+
+
+def test_error_paths():
+    rods, cfg = systems.small_3d(n=100, box=3.2)
+    ctx = capi.Context(**cfg)
+    with pytest.raises(capi.AlensError) as e:
+        ctx.step()
+    assert e.value.status == capi.ALENS_E_STATE
+    bad = dict(rods, center=rods["center"].copy())
+    bad["center"][3, 1] = np.nan
+    with pytest.raises(capi.AlensError) as e:
+        ctx.set_rods(bad)
+    assert e.value.status == capi.ALENS_E_NONFINITE
+    tiny = capi.Context(**dict(cfg, box_hi=(2.0, 2.0, 2.0)))
+    with pytest.raises(capi.AlensError) as e:
+        tiny.set_rods(rods)
+    assert e.value.status == capi.ALENS_E_BOX
+    # a 2-iteration cap reports NOCONV but still writes a result
+    dense, dcfg = systems.small_3d(n=1000)
+    c2 = capi.Context(**dict(dcfg, max_ite=2))
+    c2.set_rods(dense)
+    rep = c2.step()
+    assert rep["status_code"] == capi.ALENS_E_NOCONV and rep["iters"] == 2 and rep["converged"] == 0
+    o = O.step(dense, O.cfg_from_dict(dict(dcfg, max_ite=2)))
+    assert_bits(c2.gamma(), o["gamma"], "gamma at the iteration cap")
+
+
+def test_constraint_blocks_follow_reference_layout():
+    rods, cfg = systems.small_3d(n=500, box=3.2, seed=3)
+    ctx, g = gpu_step(rods, cfg)
+    blocks = ctx.constraint_blocks()
+    assert len(blocks) == g["I"].size
+    for k, cb in enumerate(blocks[:50]):
+        i, j = g["I"][k], g["J"][k]
+        assert cb.gid_i == rods["gid"][i] and cb.gid_j == rods["gid"][j]
+        n = np.array(cb.norm_i)
+        assert np.allclose(np.array(cb.norm_j), -n)
+        # lab points are the closest points; their difference is (delta0 + radii) along the normal
+        d = np.array(cb.lab_i) - np.array(cb.lab_j)
+        assert np.allclose(d, (cb.delta0 + 2 * rods["radius"][i]) * n, atol=1e-12)
+        assert np.allclose(np.array(cb.lab_i) - rods["center"][i], np.array(cb.pos_i), atol=1e-12)
+        assert cb.bilateral == 0 and cb.gamma_lb == 0.0 and cb.gamma == g["gamma"][k]
