@@ -14,7 +14,7 @@ constexpr double kPi = 3.14159265358979323846;
 constexpr double kEpsPar = 1e-30;      // |tI x tJ|^2 <= kEpsPar * a * c  =>  parallel segments
 constexpr double kAabbMargin = 1e-6;   // um added to AABB half-widths (rounding safety)
 constexpr int kRedThreads = 256;       // reduction lanes per block (4 waves of 64)
-constexpr int kRedChunk = 2048;        // elements per reduction chunk
+constexpr int kRedChunk = 512;         // elements per reduction chunk (2 passes of 256 lanes)
 constexpr int kRedPasses = kRedChunk / kRedThreads;
 
 // ---------------------------------------------------------------------------------------------

@@ -118,30 +118,50 @@ HipErr launch_adjacency(hipStream_t st, int32_t n, const uint32_t *rowStart,
                         uint32_t *adjStart, uint32_t *adjK, double *adjW);
 
 // solve kernels (k_solve.hip)
-struct SolveArgs {
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<double> {
+    using type = double2;
+};
+template <>
+struct Vec2<float> {
+    using type = float2;
+};
+
+// Arguments of the persistent BBPGD kernel; T is the storage/arithmetic type of the iterate,
+// operator data and velocities (double: plain method; float: precision form).  Sums of products are
+// accumulated in double in both cases.
+template <typename T>
+struct SolveArgsT {
     int32_t n;
     uint32_t m;
     const uint32_t *adjStart;
     const uint32_t *adjK;
-    const double *adjW;
-    const double *mob;
+    const T *adjW;
+    const T *mob;
     const int32_t *cI;
     const int32_t *cJ;
-    const double *geo;
-    const double *b;
-    const double *lb;  // NULL = all zero
-    double2 *xg;       // (x, g) per constraint; x holds the start point on entry
-    double *U;         // 6n scratch
-    double *part;      // kRedQ * nchunks partials
-    uint32_t *bar;     // barrier words (zeroed before launch)
-    double *out;       // report words
+    const T *geo;
+    const T *b;
+    const T *lb;                 // NULL = all zero
+    typename Vec2<T>::type *xg;  // (x, g) per constraint; x holds the start point on entry
+    T *U;                        // 6n scratch
+    double *part;                // kRedQ * nchunks partials
+    uint32_t *bar;               // barrier words (zeroed before launch)
+    double *out;                 // report words
     double dt, tol;
     int32_t max_ite;
     int32_t mono;
     int32_t warm;
     int32_t nchunks;
 };
-HipErr launch_bbpgd64(hipStream_t st, SolveArgs a, int max_blocks, int *grid_used);
+using SolveArgs = SolveArgsT<double>;
+HipErr launch_bbpgd64(hipStream_t st, SolveArgsT<double> a, int max_blocks, int *grid_used);
+HipErr launch_bbpgd32(hipStream_t st, SolveArgsT<float> a, int max_blocks, int *grid_used);
+HipErr launch_to_f32(hipStream_t st, const double *src, float *dst, size_t count);
+HipErr launch_xg32_to_x64(hipStream_t st, const float2 *xg32, double *x64, uint32_t m);
+HipErr launch_init_xg32(hipStream_t st, float2 *xg, const double2 *xg64, uint32_t m);
 HipErr launch_writeback(hipStream_t st, int32_t n, const uint32_t *adjStart, const uint32_t *adjK,
                         const double *adjW, const double2 *xg, const double *mob,
                         const double *unc, int mono, double *F, double *U, double *gamma,

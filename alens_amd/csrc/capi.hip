@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -49,6 +50,9 @@ struct alens_ctx {
 
     DBuf<double2> xg;
     DBuf<double> gamma, part, out;
+    // precision form: FP32 copies of the operator data and the FP32 iterate
+    DBuf<float> geo32, adjW32, mob32, b32, U32;
+    DBuf<float2> xg32;
     DBuf<uint32_t> bar;
 
     DBuf<uint64_t> pkey;
@@ -191,6 +195,7 @@ void alens_destroy(alens_ctx *ctx) {
     rel(ctx->cImg); rel(ctx->delta0); rel(ctx->geo); rel(ctx->posI); rel(ctx->posJ); rel(ctx->b);
     rel(ctx->adjStart); rel(ctx->adjK); rel(ctx->adjW); rel(ctx->xg); rel(ctx->gamma); rel(ctx->part);
     rel(ctx->out); rel(ctx->bar); rel(ctx->pkey); rel(ctx->pgam);
+    rel(ctx->geo32); rel(ctx->adjW32); rel(ctx->mob32); rel(ctx->b32); rel(ctx->U32); rel(ctx->xg32);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->st) (void)hipStreamDestroy(ctx->st);
@@ -322,25 +327,14 @@ alens_status alens_set_step(alens_ctx *ctx, int64_t step) {
 }
 int64_t alens_get_step(const alens_ctx *ctx) { return ctx ? ctx->step : -1; }
 
-static alens_status run_solver(alens_ctx *ctx, int32_t n, uint32_t m, const double *mob, bool warm,
-                               alens_solve_report *rep, float *ms_solve) {
-    const uint32_t nchunks = (m + kRedChunk - 1) / kRedChunk;
+}  // extern "C"
+
+// Launches one persistent BBPGD solve on the context's stream and reads its report words.
+template <typename T>
+static alens_status run_bbpgd(alens_ctx *ctx, SolveArgsT<T> a, alens_solve_report *rep, float *ms_solve) {
+    const uint32_t nchunks = (a.m + kRedChunk - 1) / kRedChunk;
     TRY_HIP(ctx->part.ensure((size_t)kRedQ * std::max<uint32_t>(nchunks, 1)));
     TRY_HIP(hipMemsetAsync(ctx->bar.p, 0, 4 * sizeof(uint32_t), ctx->st));
-    SolveArgs a{};
-    a.n = n;
-    a.m = m;
-    a.adjStart = ctx->adjStart.p;
-    a.adjK = ctx->adjK.p;
-    a.adjW = ctx->adjW.p;
-    a.mob = mob;
-    a.cI = ctx->cI.p;
-    a.cJ = ctx->cJ.p;
-    a.geo = ctx->geo.p;
-    a.b = ctx->b.p;
-    a.lb = nullptr;
-    a.xg = ctx->xg.p;
-    a.U = ctx->Us.p;
     a.part = ctx->part.p;
     a.bar = ctx->bar.p;
     a.out = ctx->out.p;
@@ -348,10 +342,15 @@ static alens_status run_solver(alens_ctx *ctx, int32_t n, uint32_t m, const doub
     a.tol = ctx->cfg.tol;
     a.max_ite = ctx->cfg.max_ite;
     a.mono = ctx->cfg.monolayer ? 1 : 0;
-    a.warm = warm ? 1 : 0;
     a.nchunks = (int32_t)nchunks;
+    // ALENS_SOLVER_MAX_BLOCKS caps the persistent solver's grid (tuning; default: all co-resident)
+    const char *mb = getenv("ALENS_SOLVER_MAX_BLOCKS");
+    const int max_blocks = mb ? atoi(mb) : 0;
     TRY_HIP(hipEventRecord(ctx->ev[2], ctx->st));
-    TRY_K(launch_bbpgd64(ctx->st, a, 0, &ctx->grid_used));
+    if constexpr (sizeof(T) == 8)
+        TRY_K(launch_bbpgd64(ctx->st, a, max_blocks, &ctx->grid_used));
+    else
+        TRY_K(launch_bbpgd32(ctx->st, a, max_blocks, &ctx->grid_used));
     TRY_HIP(hipEventRecord(ctx->ev[3], ctx->st));
     double o[8];
     uint32_t barw[4];
@@ -369,6 +368,74 @@ static alens_status run_solver(alens_ctx *ctx, int32_t n, uint32_t m, const doub
     rep->res = o[6];
     return ALENS_OK;
 }
+
+// FP64 solve on the step buffers; xg holds the start point
+static alens_status run_solver(alens_ctx *ctx, int32_t n, uint32_t m, const double *mob, bool warm,
+                               alens_solve_report *rep, float *ms_solve) {
+    SolveArgsT<double> a{};
+    a.n = n;
+    a.m = m;
+    a.adjStart = ctx->adjStart.p;
+    a.adjK = ctx->adjK.p;
+    a.adjW = ctx->adjW.p;
+    a.mob = mob;
+    a.cI = ctx->cI.p;
+    a.cJ = ctx->cJ.p;
+    a.geo = ctx->geo.p;
+    a.b = ctx->b.p;
+    a.lb = nullptr;
+    a.xg = ctx->xg.p;
+    a.U = ctx->Us.p;
+    a.warm = warm ? 1 : 0;
+    return run_bbpgd<double>(ctx, a, rep, ms_solve);
+}
+
+// Precision form: FP32 BBPGD from the start point in xg, then the FP64 solve started from the
+// FP32 iterate (its residual is the FP64 residual; it refines until the FP64 residual meets tol).
+static alens_status run_solver_mixed(alens_ctx *ctx, int32_t n, uint32_t m, bool warm, alens_solve_report *rep,
+                                     float *ms_solve) {
+    hipStream_t st = ctx->st;
+    const size_t M = std::max<uint32_t>(m, 1), N = (size_t)std::max(n, 1);
+    TRY_HIP(ctx->geo32.ensure(9 * M));
+    TRY_HIP(ctx->adjW32.ensure(12 * M));
+    TRY_HIP(ctx->b32.ensure(M));
+    TRY_HIP(ctx->xg32.ensure(M));
+    TRY_HIP(ctx->mob32.ensure(6 * N));
+    TRY_HIP(ctx->U32.ensure(6 * N));
+    TRY_K(launch_to_f32(st, ctx->geo.p, ctx->geo32.p, 9 * (size_t)m));
+    TRY_K(launch_to_f32(st, ctx->adjW.p, ctx->adjW32.p, 12 * (size_t)m));
+    TRY_K(launch_to_f32(st, ctx->b.p, ctx->b32.p, (size_t)m));
+    TRY_K(launch_to_f32(st, ctx->mob.p, ctx->mob32.p, 6 * (size_t)n));
+    TRY_K(launch_init_xg32(st, ctx->xg32.p, ctx->xg.p, m));
+    SolveArgsT<float> a{};
+    a.n = n;
+    a.m = m;
+    a.adjStart = ctx->adjStart.p;
+    a.adjK = ctx->adjK.p;
+    a.adjW = ctx->adjW32.p;
+    a.mob = ctx->mob32.p;
+    a.cI = ctx->cI.p;
+    a.cJ = ctx->cJ.p;
+    a.geo = ctx->geo32.p;
+    a.b = ctx->b32.p;
+    a.lb = nullptr;
+    a.xg = ctx->xg32.p;
+    a.U = ctx->U32.p;
+    a.warm = warm ? 1 : 0;
+    alens_solve_report r32{};
+    float ms32 = 0.f, ms64 = 0.f;
+    alens_status s = run_bbpgd<float>(ctx, a, &r32, &ms32);
+    if (s != ALENS_OK) return s;
+    TRY_K(launch_xg32_to_x64(st, ctx->xg32.p, ctx->gamma.p, m));
+    TRY_K(launch_init_xg(st, ctx->xg.p, ctx->gamma.p, m));
+    s = run_solver(ctx, n, m, ctx->mob.p, true, rep, &ms64);
+    if (s != ALENS_OK) return s;
+    rep->iters_fp32 = r32.iters;
+    *ms_solve = ms32 + ms64;
+    return ALENS_OK;
+}
+
+extern "C" {
 
 alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_solve_report *rep_out) {
     if (!ctx) return ALENS_E_ARG;
@@ -472,7 +539,8 @@ alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_s
                                     ctx->xg.p));
         else
             TRY_K(launch_init_xg(st, ctx->xg.p, nullptr, M));
-        alens_status s = run_solver(ctx, n, M, ctx->mob.p, warm, &rep, &ms_solve);
+        alens_status s = ctx->cfg.precision == 1 ? run_solver_mixed(ctx, n, M, warm, &rep, &ms_solve)
+                                                 : run_solver(ctx, n, M, ctx->mob.p, warm, &rep, &ms_solve);
         if (s != ALENS_OK) return s;
     } else {
         rep.converged = 1;

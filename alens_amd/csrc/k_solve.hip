@@ -5,7 +5,7 @@
 // rodSystem.runStep() / stepEuler(), SRC/TubuleSystem.cpp:223, :55).
 //
 // One iteration = phase A (rod-parallel: x+ = P(x - alpha g) gathered per incident constraint,
-// F = D x+, U = M F), grid barrier, phase B (constraint-parallel in 2048-element chunks:
+// F = D x+, U = M F), grid barrier, phase B (constraint-parallel in 512-element chunks:
 // g+ = dt D^T U + b, residual, s = x+ - x, y = g+ - g, chunk partial sums), grid barrier, and every
 // block folds the chunk partials in the same fixed order, so all blocks hold identical step lengths
 // and stop together.
@@ -23,132 +23,216 @@ namespace alens {
 static inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 constexpr uint32_t kSpinLimit = 1u << 26;  // barrier wait bound: sets the abort word, never hangs
+constexpr int kBlk = 1024;                  // threads per block: 4 reduction groups of 256
+constexpr int kGrp = kBlk / kRedThreads;
 
 enum : int { Q_SS = 0, Q_SY = 1, Q_YY = 2, Q_RMAX = 3, Q_NF = 4, Q_RC = 5, Q_NFC = 6 };
 
-// Grid barrier over co-resident blocks (cooperative launch).  bar[0] = arrivals, bar[1] = generation,
-// bar[2] = abort.  Returns false once any block gave up waiting.
-__device__ inline bool grid_barrier(uint32_t *bar, int *s_flag) {
+// Grid barrier over co-resident blocks (cooperative launch).  bar[0] counts arrivals monotonically:
+// barrier number e completes when it reaches (e + 1) * gridDim.x.  bar[2] = abort.  Thread 0 of
+// each block publishes the block's writes with one agent-scope release fence, polls with relaxed
+// loads, and takes one agent-scope acquire fence after the wait.
+__device__ inline bool grid_barrier(uint32_t *bar, uint32_t &epoch, int *s_flag) {
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
-        const uint32_t gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        const uint32_t prev = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == gridDim.x - 1) {
-            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&bar[1], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            uint32_t spins = 0;
-            while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > kSpinLimit) {
-                    __hip_atomic_store(&bar[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
-                }
-                if ((spins & 1023u) == 0 &&
-                    __hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                    break;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const uint32_t target = (epoch + 1u) * gridDim.x;
+        __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t spins = 0;
+        while (__hip_atomic_load(&bar[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > kSpinLimit) {
+                __hip_atomic_store(&bar[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
             }
+            if ((spins & 1023u) == 0 && __hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                break;
         }
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         *s_flag = (int)__hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    ++epoch;
     __syncthreads();
     return *s_flag == 0;
 }
 
-// Sum in the fixed order: per-thread partial already accumulated left to right; fold each wave by
-// halving (lane l += lane l + h, h = 32 .. 1); add the four wave totals left to right.
-__device__ inline double block_sum_fixed(double v, double *sh) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+// Fixed-order sum within one 256-lane reduction group: each lane's partial is already accumulated
+// left to right; each wave folds by halving (lane l += lane l + h, h = 32 .. 1); the group's four
+// wave totals are added left to right.  Called by every thread of the block (all groups together).
+template <int NQ>
+__device__ inline void group_sums(double (&v)[NQ], double *sh /* kGrp * NQ * 4 */, double (&tot)[NQ]) {
+    const int lane = threadIdx.x & 63, w = (threadIdx.x >> 6) & 3, g = threadIdx.x >> 8;
 #pragma unroll
-    for (int h = 32; h >= 1; h >>= 1) v = v + __shfl_down(v, h, 64);
-    if (lane == 0) sh[w] = v;
+    for (int q = 0; q < NQ; ++q) {
+        double x = v[q];
+#pragma unroll
+        for (int h = 32; h >= 1; h >>= 1) x = x + __shfl_down(x, h, 64);
+        if (lane == 0) sh[(g * NQ + q) * 4 + w] = x;
+    }
     __syncthreads();
-    const double t = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const double *s4 = sh + (g * NQ + q) * 4;
+        tot[q] = ((s4[0] + s4[1]) + s4[2]) + s4[3];
+    }
     __syncthreads();
-    return t;
 }
 
-__device__ inline double block_max(double v, double *sh) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+template <int NQ>
+__device__ inline void group_maxes(double (&v)[NQ], double *sh, double (&tot)[NQ]) {
+    const int lane = threadIdx.x & 63, w = (threadIdx.x >> 6) & 3, g = threadIdx.x >> 8;
 #pragma unroll
-    for (int h = 32; h >= 1; h >>= 1) {
-        const double o = __shfl_down(v, h, 64);
-        v = o > v ? o : v;
+    for (int q = 0; q < NQ; ++q) {
+        double x = v[q];
+#pragma unroll
+        for (int h = 32; h >= 1; h >>= 1) {
+            const double o = __shfl_down(x, h, 64);
+            x = o > x ? o : x;
+        }
+        if (lane == 0) sh[(g * NQ + q) * 4 + w] = x;
     }
-    if (lane == 0) sh[w] = v;
     __syncthreads();
-    double t = sh[0];
-    t = sh[1] > t ? sh[1] : t;
-    t = sh[2] > t ? sh[2] : t;
-    t = sh[3] > t ? sh[3] : t;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const double *s4 = sh + (g * NQ + q) * 4;
+        double t = s4[0];
+        t = s4[1] > t ? s4[1] : t;
+        t = s4[2] > t ? s4[2] : t;
+        t = s4[3] > t ? s4[3] : t;
+        tot[q] = t;
+    }
     __syncthreads();
-    return t;
 }
 
-// total of partial slot q over all chunks, fixed order
-__device__ inline double fold_sum(const double *part, int nchunks, int q, double *sh) {
-    double acc = 0.0;
-    for (int c = threadIdx.x; c < nchunks; c += kRedThreads) acc = acc + part[(size_t)q * nchunks + c];
-    return block_sum_fixed(acc, sh);
-}
-__device__ inline double fold_max(const double *part, int nchunks, int q, double *sh) {
-    double acc = 0.0;
-    for (int c = threadIdx.x; c < nchunks; c += kRedThreads) {
-        const double v = part[(size_t)q * nchunks + c];
-        acc = v > acc ? v : acc;
+// Totals over all chunk partials: sums of slots S[0..NS) in the fixed order (lane c % 256 of group 0
+// accumulates partials c, c + 256, ... left to right, then the group fold), maxes of slots X[0..NX).
+// Every block computes the same values; results land in tot_s / tot_x for every thread.
+template <int NS, int NX>
+__device__ inline void fold_partials(const double *part, int nchunks, const int (&S)[NS], const int (&X)[NX],
+                                     double *sh, double (&tot_s)[NS], double (&tot_x)[NX]) {
+    double vs[NS], vx[NX];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) vs[q] = 0.0;
+#pragma unroll
+    for (int q = 0; q < NX; ++q) vx[q] = 0.0;
+    if (threadIdx.x < kRedThreads) {
+        for (int c = threadIdx.x; c < nchunks; c += kRedThreads) {
+#pragma unroll
+            for (int q = 0; q < NS; ++q) vs[q] = vs[q] + part[(size_t)S[q] * nchunks + c];
+#pragma unroll
+            for (int q = 0; q < NX; ++q) {
+                const double v = part[(size_t)X[q] * nchunks + c];
+                vx[q] = v > vx[q] ? v : vx[q];
+            }
+        }
     }
-    return block_max(acc, sh);
+    double ts[NS], tx[NX];
+    group_sums<NS>(vs, sh, ts);
+    group_maxes<NX>(vx, sh, tx);
+    // group 0 holds the totals; broadcast them through LDS
+    __shared__ double bc[16];
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) bc[q] = ts[q];
+#pragma unroll
+        for (int q = 0; q < NX; ++q) bc[8 + q] = tx[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NS; ++q) tot_s[q] = bc[q];
+#pragma unroll
+    for (int q = 0; q < NX; ++q) tot_x[q] = bc[8 + q];
+    __syncthreads();
 }
 
 enum : int { A_PROJ = 0, A_GRAD = 1 };
 
-template <int MODE>
-__device__ inline void phase_A(const SolveArgs &a, double alpha) {
+template <typename T>
+__device__ inline typename Vec2<T>::type mk2(T x, T y);
+template <>
+__device__ inline double2 mk2<double>(double x, double y) {
+    return make_double2(x, y);
+}
+template <>
+__device__ inline float2 mk2<float>(float x, float y) {
+    return make_float2(x, y);
+}
+
+template <int MODE, typename T>
+__device__ inline T a_value(const SolveArgsT<T> &a, uint32_t k, T alpha) {
+    const typename Vec2<T>::type xg = a.xg[k];
+    if (MODE == A_GRAD) return xg.y;
+    return proj(xg.x - alpha * xg.y, a.lb ? a.lb[k] : T(0));
+}
+
+// phase A: per rod, F = sum over incident constraints (ascending k) of v_k * w_e; U = M F
+template <int MODE, typename T>
+__device__ inline void phase_A(const SolveArgsT<T> &a, T alpha) {
+    using V2 = typename Vec2<T>::type;
     const int stride = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
-        double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        T f0 = T(0), f1 = T(0), f2 = T(0), f3 = T(0), f4 = T(0), f5 = T(0);
+        uint32_t e = a.adjStart[i];
         const uint32_t e1 = a.adjStart[i + 1];
-        for (uint32_t e = a.adjStart[i]; e < e1; ++e) {
-            const uint32_t k = a.adjK[e] & 0x7fffffffu;
-            const double2 xg = a.xg[k];
-            double v;
-            if (MODE == A_GRAD) {
-                v = xg.y;
-            } else {
-                v = proj(xg.x - alpha * xg.y, a.lb ? a.lb[k] : 0.0);
-            }
-            const double2 *w2 = reinterpret_cast<const double2 *>(a.adjW + 6 * (size_t)e);
-            const double2 w01 = w2[0], w23 = w2[1], w45 = w2[2];
-            f[0] = f[0] + v * w01.x;
-            f[1] = f[1] + v * w01.y;
-            f[2] = f[2] + v * w23.x;
-            f[3] = f[3] + v * w23.y;
-            f[4] = f[4] + v * w45.x;
-            f[5] = f[5] + v * w45.y;
+        for (; e + 2 <= e1; e += 2) {
+            const uint32_t k0 = a.adjK[e] & 0x7fffffffu, k1 = a.adjK[e + 1] & 0x7fffffffu;
+            const V2 *w2 = reinterpret_cast<const V2 *>(a.adjW + 6 * (size_t)e);
+            const V2 p0 = w2[0], p1 = w2[1], p2 = w2[2], q0 = w2[3], q1 = w2[4], q2 = w2[5];
+            const T v0 = a_value<MODE, T>(a, k0, alpha);
+            const T v1 = a_value<MODE, T>(a, k1, alpha);
+            f0 = f0 + v0 * p0.x;
+            f1 = f1 + v0 * p0.y;
+            f2 = f2 + v0 * p1.x;
+            f3 = f3 + v0 * p1.y;
+            f4 = f4 + v0 * p2.x;
+            f5 = f5 + v0 * p2.y;
+            f0 = f0 + v1 * q0.x;
+            f1 = f1 + v1 * q0.y;
+            f2 = f2 + v1 * q1.x;
+            f3 = f3 + v1 * q1.y;
+            f4 = f4 + v1 * q2.x;
+            f5 = f5 + v1 * q2.y;
         }
-        double mb[6], u[6];
-#pragma unroll
-        for (int q = 0; q < 6; ++q) mb[q] = a.mob[6 * (size_t)i + q];
+        if (e < e1) {
+            const uint32_t k0 = a.adjK[e] & 0x7fffffffu;
+            const V2 *w2 = reinterpret_cast<const V2 *>(a.adjW + 6 * (size_t)e);
+            const V2 p0 = w2[0], p1 = w2[1], p2 = w2[2];
+            const T v0 = a_value<MODE, T>(a, k0, alpha);
+            f0 = f0 + v0 * p0.x;
+            f1 = f1 + v0 * p0.y;
+            f2 = f2 + v0 * p1.x;
+            f3 = f3 + v0 * p1.y;
+            f4 = f4 + v0 * p2.x;
+            f5 = f5 + v0 * p2.y;
+        }
+        const V2 *m2 = reinterpret_cast<const V2 *>(a.mob + 6 * (size_t)i);
+        const V2 m01 = m2[0], m23 = m2[1], m45 = m2[2];
+        const T mb[6] = {m01.x, m01.y, m23.x, m23.y, m45.x, m45.y};
+        const T f[6] = {f0, f1, f2, f3, f4, f5};
+        T u[6];
         apply_M(mb, f, a.mono, u);
-        double2 *U2 = reinterpret_cast<double2 *>(a.U + 6 * (size_t)i);
-        U2[0] = make_double2(u[0], u[1]);
-        U2[1] = make_double2(u[2], u[3]);
-        U2[2] = make_double2(u[4], u[5]);
+        V2 *U2 = reinterpret_cast<V2 *>(a.U + 6 * (size_t)i);
+        U2[0] = mk2<T>(u[0], u[1]);
+        U2[1] = mk2<T>(u[2], u[3]);
+        U2[2] = mk2<T>(u[4], u[5]);
     }
 }
 
-__device__ inline double du_row(const SolveArgs &a, uint32_t k) {
-    double gk[9], uI[6], uJ[6];
+template <typename T>
+__device__ inline T du_row(const SolveArgsT<T> &a, uint32_t k) {
+    using V2 = typename Vec2<T>::type;
+    const T *gp = a.geo + 9 * (size_t)k;
+    T gk[9];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) gk[q] = a.geo[9 * (size_t)k + q];
+    for (int q = 0; q < 9; ++q) gk[q] = gp[q];
     const size_t I = (size_t)a.cI[k], J = (size_t)a.cJ[k];
-    const double2 *UI = reinterpret_cast<const double2 *>(a.U + 6 * I);
-    const double2 *UJ = reinterpret_cast<const double2 *>(a.U + 6 * J);
+    const V2 *UI = reinterpret_cast<const V2 *>(a.U + 6 * I);
+    const V2 *UJ = reinterpret_cast<const V2 *>(a.U + 6 * J);
+    T uI[6], uJ[6];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        const double2 p = UI[q], r = UJ[q];
+        const V2 p = UI[q], r = UJ[q];
         uI[2 * q] = p.x;
         uI[2 * q + 1] = p.y;
         uJ[2 * q] = r.x;
@@ -159,75 +243,92 @@ __device__ inline double du_row(const SolveArgs &a, uint32_t k) {
 
 enum : int { B_INIT = 0, B_ALPHA0 = 1, B_ITER = 2 };
 
-template <int MODE>
-__device__ inline void phase_B(const SolveArgs &a, double alpha, double *sh) {
-    for (int c = blockIdx.x; c < a.nchunks; c += gridDim.x) {
+// phase B: constraints in 512-element chunks, one chunk per 256-lane group (element e of the chunk
+// is lane e % 256, pass e / 256); chunk partials go to part[q * nchunks + c]
+template <int MODE, typename T>
+__device__ inline void phase_B(const SolveArgsT<T> &a, T alpha, double *sh) {
+    const int g = threadIdx.x >> 8, lane = threadIdx.x & 255;
+    const int ngroups = gridDim.x * kGrp;
+    const T dt = (T)a.dt;
+    for (int base = blockIdx.x * kGrp; base < a.nchunks; base += ngroups) {
+        const int c = base + g;
         double s0 = 0.0, s1 = 0.0, s2 = 0.0, rmax = 0.0, nf = 0.0, rc = 0.0, nfc = 0.0;
-        for (int p = 0; p < kRedPasses; ++p) {
-            const uint32_t k = (uint32_t)c * kRedChunk + p * kRedThreads + threadIdx.x;
-            if (k >= a.m) break;
-            const double2 xg = a.xg[k];
-            const double l = a.lb ? a.lb[k] : 0.0;
-            const double du = du_row(a, k);
-            if (MODE == B_ALPHA0) {
-                const double ag = a.dt * du;
-                s0 = s0 + xg.y * xg.y;
-                s1 = s1 + xg.y * ag;
-            } else {
-                const double xn = proj(xg.x - alpha * xg.y, l);
-                const double bk = a.b[k];
-                const double gn = a.dt * du + bk;
-                const double res = fabs(xn - proj(xn - gn, l));
-                if (!isfinite(res))
-                    nf = nf + 1.0;
-                else if (res > rmax)
-                    rmax = res;
-                if (MODE == B_INIT) {
-                    if (a.warm) {
-                        const double r0 = fabs(0.0 - proj(0.0 - bk, l));
-                        if (!isfinite(r0))
-                            nfc = nfc + 1.0;
-                        else if (r0 > rc)
-                            rc = r0;
-                    }
-                } else {
-                    const double s = xn - xg.x;
-                    const double y = gn - xg.y;
-                    s0 = s0 + s * s;
-                    s1 = s1 + s * y;
-                    s2 = s2 + y * y;
+        if (c < a.nchunks) {
+            T xo[kRedPasses], go[kRedPasses], du[kRedPasses];
+            uint32_t kk[kRedPasses];
+            bool ok[kRedPasses];
+#pragma unroll
+            for (int p = 0; p < kRedPasses; ++p) {
+                kk[p] = (uint32_t)c * kRedChunk + p * kRedThreads + lane;
+                ok[p] = kk[p] < a.m;
+                if (ok[p]) {
+                    const typename Vec2<T>::type xg = a.xg[kk[p]];
+                    xo[p] = xg.x;
+                    go[p] = xg.y;
+                    du[p] = du_row<T>(a, kk[p]);
                 }
-                a.xg[k] = make_double2(xn, gn);
+            }
+#pragma unroll
+            for (int p = 0; p < kRedPasses; ++p) {
+                if (!ok[p]) continue;
+                const uint32_t k = kk[p];
+                const T l = a.lb ? a.lb[k] : T(0);
+                if (MODE == B_ALPHA0) {
+                    const T ag = dt * du[p];
+                    s0 = s0 + (double)(go[p] * go[p]);
+                    s1 = s1 + (double)(go[p] * ag);
+                } else {
+                    const T xn = proj(xo[p] - alpha * go[p], l);
+                    const T bk = a.b[k];
+                    const T gn = dt * du[p] + bk;
+                    const double res = (double)fabs(xn - proj(xn - gn, l));
+                    if (!isfinite(res))
+                        nf = nf + 1.0;
+                    else if (res > rmax)
+                        rmax = res;
+                    if (MODE == B_INIT) {
+                        if (a.warm) {
+                            const double r0 = (double)fabs(T(0) - proj(T(0) - bk, l));
+                            if (!isfinite(r0))
+                                nfc = nfc + 1.0;
+                            else if (r0 > rc)
+                                rc = r0;
+                        }
+                    } else {
+                        const T s = xn - xo[p];
+                        const T y = gn - go[p];
+                        s0 = s0 + (double)(s * s);
+                        s1 = s1 + (double)(s * y);
+                        s2 = s2 + (double)(y * y);
+                    }
+                    a.xg[k] = mk2<T>(xn, gn);
+                }
             }
         }
+        double vs[4] = {s0, s1, s2, nf}, ts[4];
+        double vx[1] = {rmax}, tx[1];
+        group_sums<4>(vs, sh, ts);
+        group_maxes<1>(vx, sh, tx);
         double *pc = a.part + c;
         const int nc = a.nchunks;
-        if (MODE != B_INIT) {
-            const double t0 = block_sum_fixed(s0, sh);
-            const double t1 = block_sum_fixed(s1, sh);
-            if (threadIdx.x == 0) {
-                pc[(size_t)Q_SS * nc] = t0;
-                pc[(size_t)Q_SY * nc] = t1;
+        if (c < nc && lane == 0) {
+            if (MODE != B_INIT) {
+                pc[(size_t)Q_SS * nc] = ts[0];
+                pc[(size_t)Q_SY * nc] = ts[1];
             }
-        }
-        if (MODE == B_ITER) {
-            const double t2 = block_sum_fixed(s2, sh);
-            if (threadIdx.x == 0) pc[(size_t)Q_YY * nc] = t2;
-        }
-        if (MODE != B_ALPHA0) {
-            const double tm = block_max(rmax, sh);
-            const double tn = block_sum_fixed(nf, sh);
-            if (threadIdx.x == 0) {
-                pc[(size_t)Q_RMAX * nc] = tm;
-                pc[(size_t)Q_NF * nc] = tn;
+            if (MODE == B_ITER) pc[(size_t)Q_YY * nc] = ts[2];
+            if (MODE != B_ALPHA0) {
+                pc[(size_t)Q_RMAX * nc] = tx[0];
+                pc[(size_t)Q_NF * nc] = ts[3];
             }
         }
         if (MODE == B_INIT && a.warm) {
-            const double tm = block_max(rc, sh);
-            const double tn = block_sum_fixed(nfc, sh);
-            if (threadIdx.x == 0) {
-                pc[(size_t)Q_RC * nc] = tm;
-                pc[(size_t)Q_NFC * nc] = tn;
+            double v2[1] = {nfc}, t2[1], v3[1] = {rc}, t3[1];
+            group_sums<1>(v2, sh, t2);
+            group_maxes<1>(v3, sh, t3);
+            if (c < nc && lane == 0) {
+                pc[(size_t)Q_RC * nc] = t3[0];
+                pc[(size_t)Q_NFC * nc] = t2[0];
             }
         }
     }
@@ -235,31 +336,41 @@ __device__ inline void phase_B(const SolveArgs &a, double alpha, double *sh) {
 
 // out: [0] iters, [1] matvecs, [2] converged, [3] fallback, [4] status, [5] res0, [6] res,
 //      [7] alpha at exit
-__global__ __launch_bounds__(256) void k_bbpgd64(SolveArgs a) {
-    __shared__ double sh[8];
+template <typename T>
+__global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
+    __shared__ double sh[kGrp * 8 * 4];
     __shared__ int s_flag;
+    uint32_t epoch = 0;
     double alpha = 0.0;
     int iters = 0, matvecs = 1, converged = 0, fallback = 0, status = 0;
     double res0 = 0.0, res = 0.0;
 
     // start point: x = P(x0), g = A x + b
-    phase_A<A_PROJ>(a, 0.0);
-    if (!grid_barrier(a.bar, &s_flag)) return;
-    phase_B<B_INIT>(a, 0.0, sh);
-    if (!grid_barrier(a.bar, &s_flag)) return;
-    double R0 = fold_max(a.part, a.nchunks, Q_RMAX, sh);
-    double NF0 = fold_sum(a.part, a.nchunks, Q_NF, sh);
+    phase_A<A_PROJ, T>(a, T(0));
+    if (!grid_barrier(a.bar, epoch, &s_flag)) return;
+    phase_B<B_INIT, T>(a, T(0), sh);
+    if (!grid_barrier(a.bar, epoch, &s_flag)) return;
+    double R0, NF0;
+    {
+        const int S[1] = {Q_NF}, X[1] = {Q_RMAX};
+        double ts[1], tx[1];
+        fold_partials<1, 1>(a.part, a.nchunks, S, X, sh, ts, tx);
+        R0 = tx[0];
+        NF0 = ts[0];
+    }
     if (a.warm) {
-        const double Rc = fold_max(a.part, a.nchunks, Q_RC, sh);
-        const double NFc = fold_sum(a.part, a.nchunks, Q_NFC, sh);
+        const int S[1] = {Q_NFC}, X[1] = {Q_RC};
+        double ts[1], tx[1];
+        fold_partials<1, 1>(a.part, a.nchunks, S, X, sh, ts, tx);
+        const double Rc = tx[0], NFc = ts[0];
         if (NF0 > 0.0 || R0 > Rc) {
             // warm start rejected: x = 0, g = 0 + b
             for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < a.m; k += gridDim.x * blockDim.x)
-                a.xg[k] = make_double2(0.0, 0.0 + a.b[k]);
+                a.xg[k] = mk2<T>(T(0), T(0) + a.b[k]);
             R0 = Rc;
             NF0 = NFc;
             fallback = 1;
-            if (!grid_barrier(a.bar, &s_flag)) return;
+            if (!grid_barrier(a.bar, epoch, &s_flag)) return;
         }
     }
     res0 = R0;
@@ -270,39 +381,40 @@ __global__ __launch_bounds__(256) void k_bbpgd64(SolveArgs a) {
         converged = 1;
     } else {
         // alpha0 = g^T g / g^T A g
-        phase_A<A_GRAD>(a, 0.0);
-        if (!grid_barrier(a.bar, &s_flag)) return;
-        phase_B<B_ALPHA0>(a, 0.0, sh);
-        if (!grid_barrier(a.bar, &s_flag)) return;
+        phase_A<A_GRAD, T>(a, T(0));
+        if (!grid_barrier(a.bar, epoch, &s_flag)) return;
+        phase_B<B_ALPHA0, T>(a, T(0), sh);
+        if (!grid_barrier(a.bar, epoch, &s_flag)) return;
         ++matvecs;
-        const double gg = fold_sum(a.part, a.nchunks, Q_SS, sh);
-        const double gag = fold_sum(a.part, a.nchunks, Q_SY, sh);
-        alpha = gg / gag;
+        {
+            const int S[2] = {Q_SS, Q_SY}, X[1] = {Q_RMAX};
+            double ts[2], tx[1];
+            fold_partials<2, 1>(a.part, a.nchunks, S, X, sh, ts, tx);
+            alpha = ts[0] / ts[1];
+        }
         if (!(alpha > 0.0 && alpha < INFINITY)) alpha = 1.0;
         status = ALENS_E_NOCONV;
         for (int it = 1; it <= a.max_ite; ++it) {
-            phase_A<A_PROJ>(a, alpha);
-            if (!grid_barrier(a.bar, &s_flag)) return;
-            phase_B<B_ITER>(a, alpha, sh);
-            if (!grid_barrier(a.bar, &s_flag)) return;
+            phase_A<A_PROJ, T>(a, (T)alpha);
+            if (!grid_barrier(a.bar, epoch, &s_flag)) return;
+            phase_B<B_ITER, T>(a, (T)alpha, sh);
+            if (!grid_barrier(a.bar, epoch, &s_flag)) return;
             ++matvecs;
-            const double ss = fold_sum(a.part, a.nchunks, Q_SS, sh);
-            const double sy = fold_sum(a.part, a.nchunks, Q_SY, sh);
-            const double yy = fold_sum(a.part, a.nchunks, Q_YY, sh);
-            const double R = fold_max(a.part, a.nchunks, Q_RMAX, sh);
-            const double NF = fold_sum(a.part, a.nchunks, Q_NF, sh);
+            const int S[4] = {Q_SS, Q_SY, Q_YY, Q_NF}, X[1] = {Q_RMAX};
+            double ts[4], tx[1];
+            fold_partials<4, 1>(a.part, a.nchunks, S, X, sh, ts, tx);
             iters = it;
-            res = R;
-            if (NF > 0.0) {
+            res = tx[0];
+            if (ts[3] > 0.0) {
                 status = ALENS_E_NONFINITE;
                 break;
             }
-            if (R <= a.tol) {
+            if (tx[0] <= a.tol) {
                 converged = 1;
                 status = 0;
                 break;
             }
-            const double an = (it & 1) ? ss / sy : sy / yy;
+            const double an = (it & 1) ? ts[0] / ts[1] : ts[1] / ts[2];
             if (an > 0.0 && an < INFINITY) alpha = an;
         }
     }
@@ -318,21 +430,62 @@ __global__ __launch_bounds__(256) void k_bbpgd64(SolveArgs a) {
     }
 }
 
-HipErr launch_bbpgd64(hipStream_t st, SolveArgs a, int max_blocks, int *grid_used) {
+template <typename T>
+static HipErr launch_bbpgd(hipStream_t st, SolveArgsT<T> a, int max_blocks, int *grid_used) {
     int dev = 0, ncu = 0, per_cu = 0;
     ALENS_HIP_TRY(hipGetDevice(&dev));
     ALENS_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    ALENS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_bbpgd64, 256, 0));
+    ALENS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_bbpgd<T>, kBlk, 0));
     if (per_cu < 1) per_cu = 1;
-    int64_t need = (int64_t)nblk(a.n, 256);
-    if ((int64_t)a.nchunks > need) need = a.nchunks;
+    int64_t need = (int64_t)nblk(a.n, kBlk);
+    const int64_t needB = ((int64_t)a.nchunks + kGrp - 1) / kGrp;
+    if (needB > need) need = needB;
     if (need < 1) need = 1;
     int64_t lim = (int64_t)ncu * per_cu;
     if (max_blocks > 0 && max_blocks < lim) lim = max_blocks;
     const int grid = (int)(need < lim ? need : lim);
     *grid_used = grid;
     void *args[] = {&a};
-    ALENS_HIP_TRY(hipLaunchCooperativeKernel((const void *)k_bbpgd64, dim3(grid), dim3(256), args, 0, st));
+    ALENS_HIP_TRY(hipLaunchCooperativeKernel((const void *)k_bbpgd<T>, dim3(grid), dim3(kBlk), args, 0, st));
+    return hip_ok();
+}
+
+HipErr launch_bbpgd64(hipStream_t st, SolveArgsT<double> a, int max_blocks, int *grid_used) {
+    return launch_bbpgd<double>(st, a, max_blocks, grid_used);
+}
+HipErr launch_bbpgd32(hipStream_t st, SolveArgsT<float> a, int max_blocks, int *grid_used) {
+    return launch_bbpgd<float>(st, a, max_blocks, grid_used);
+}
+
+// precision-form conversions
+__global__ void k_to_f32(const double *__restrict__ s, float *__restrict__ d, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = (float)s[i];
+}
+HipErr launch_to_f32(hipStream_t st, const double *src, float *dst, size_t count) {
+    if (count == 0) return hip_ok();
+    k_to_f32<<<nblk((int64_t)count, 256), 256, 0, st>>>(src, dst, count);
+    LAUNCH_CHECK();
+    return hip_ok();
+}
+__global__ void k_xg32_to_x64(const float2 *__restrict__ xg, double *__restrict__ x, uint32_t m) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < m) x[k] = (double)xg[k].x;
+}
+HipErr launch_xg32_to_x64(hipStream_t st, const float2 *xg32, double *x64, uint32_t m) {
+    if (m == 0) return hip_ok();
+    k_xg32_to_x64<<<nblk(m, 256), 256, 0, st>>>(xg32, x64, m);
+    LAUNCH_CHECK();
+    return hip_ok();
+}
+__global__ void k_init_xg32(float2 *__restrict__ xg, const double2 *__restrict__ xg64, uint32_t m) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < m) xg[k] = make_float2((float)xg64[k].x, 0.0f);
+}
+HipErr launch_init_xg32(hipStream_t st, float2 *xg, const double2 *xg64, uint32_t m) {
+    if (m == 0) return hip_ok();
+    k_init_xg32<<<nblk(m, 256), 256, 0, st>>>(xg, xg64, m);
+    LAUNCH_CHECK();
     return hip_ok();
 }
 

@@ -49,6 +49,7 @@ def main():
                     choices=["monolayer_slab", "lognormal_3d", "small_3d"])
     ap.add_argument("--n", type=int, default=0, help="rod count (default: the workload's own)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-form", action="store_true", help="skip the chosen-form (FP32 + warm start) pass")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--progress", action="store_true")
     args = ap.parse_args()
@@ -130,6 +131,35 @@ def main():
             "algorithmic_bytes_per_launch": float(alg / K),
         },
     }
+
+    # chosen form on the same workload: FP32 iterations + FP64 residual/refinement, warm start from
+    # the previous step's impulses
+    if not args.no_form:
+        fctx = capi.Context(**dict(cfg, precision=1, warm_start=1))
+        fctx.set_rods(rods)
+        for w in range(args.warmup):
+            fctx.step()
+            fctx.euler()
+        freps = []
+        f0 = time.perf_counter()
+        for s in range(args.steps):
+            freps.append(fctx.step())
+            fctx.euler()
+        f1 = time.perf_counter()
+        fit32 = np.array([r["iters_fp32"] for r in freps])
+        fit64 = np.array([r["iters"] for r in freps])
+        out["chosen_form"] = {
+            "form": "FP32 BBPGD + FP64 residual/refinement, warm start from previous-step gamma",
+            "value": n * K / (f1 - f0),
+            "unit": "rod-steps/s",
+            "ms_per_step": 1e3 * (f1 - f0) / K,
+            "bbpgd_iters_per_step": float((fit32 + fit64).mean()),
+            "fp32_iters_per_step": float(fit32.mean()),
+            "fp64_refine_iters_per_step": float(fit64.mean()),
+            "fallbacks": int(sum(r["fallback_used"] for r in freps)),
+            "iteration_ratio_plain_over_form": float(iters.mean() / max((fit32 + fit64).mean(), 1e-9)),
+        }
+        fctx.close()
 
     if not args.no_cpu:
         from oracle import oracle as O

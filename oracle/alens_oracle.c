@@ -35,7 +35,7 @@
 #define ORC_EPS_PAR 1e-30     /* |tI x tJ|^2 <= EPS_PAR * a * c  =>  treated as parallel */
 #define ORC_AABB_MARGIN 1e-6  /* um added to every AABB half-width so rounding never drops a pair */
 #define ORC_RED_THREADS 256   /* reduction layout: 256 lanes = 4 waves of 64 */
-#define ORC_RED_CHUNK 2048    /* elements per reduction chunk */
+#define ORC_RED_CHUNK 512     /* elements per reduction chunk (2 passes of 256 lanes) */
 
 static int g_threads = 0;
 
@@ -264,7 +264,7 @@ static void contact_normal(const double D[3], double dist, const double tI[3], c
  * element e of a list goes to lane e % 256 as its (e / 256)-th term; each lane accumulates its
  * terms left to right from 0.0; each 64-lane wave folds by halving (lane l += lane l + h for
  * h = 32, 16, ..., 1); the four wave totals are added left to right.  Long vectors are cut into
- * 2048-element chunks, each chunk is summed this way, and the chunk totals are summed the same way.
+ * 512-element chunks, each chunk is summed this way, and the chunk totals are summed the same way.
  * ------------------------------------------------------------------------------------------------ */
 double orc_block_sum(const double *v, int64_t n) {
     double acc[ORC_RED_THREADS];

@@ -238,3 +238,32 @@ def test_constraint_blocks_follow_reference_layout():
         assert np.allclose(d, (cb.delta0 + 2 * rods["radius"][i]) * n, atol=1e-12)
         assert np.allclose(np.array(cb.lab_i) - rods["center"][i], np.array(cb.pos_i), atol=1e-12)
         assert cb.bilateral == 0 and cb.gamma_lb == 0.0 and cb.gamma == g["gamma"][k]
+
+
+def _kkt_residual(n, I, J, rw, mob, cfg_o, gamma):
+    """max |min(gamma, A gamma + b)| with A applied by the oracle's FP64 operators."""
+    _, U = O.apply_D_M(n, I, J, rw, mob, cfg_o, gamma)
+    g = cfg_o.dt * O.apply_DT(I, J, rw, U) + rw["b"]
+    return float(np.max(np.abs(np.minimum(gamma, g)))) if gamma.size else 0.0
+
+
+@pytest.mark.parametrize("kind,n", [("small_3d", 1000), ("lognormal_3d", 20000), ("monolayer_slab", 20000)])
+def test_precision_form_meets_fp64_residual(kind, n):
+    """FP32 iterations + FP64 residual/refinement: the returned gamma satisfies the FP64 KKT residual
+    tol, and its error against a tight FP64 solve is of the same order as the plain FP64 method's."""
+    rods, cfg = systems.scaled(kind, n) if kind != "small_3d" else systems.small_3d()
+    ctx, g = gpu_step(rods, cfg, precision=1)
+    rep = g["report"]
+    assert rep["converged"] == 1 and rep["res"] <= cfg["tol"] and rep["iters_fp32"] > 0
+    oc = O.cfg_from_dict(cfg)
+    o = O.step(rods, oc)
+    assert np.array_equal(g["I"], o["I"]) and np.array_equal(g["J"], o["J"])
+    nn = len(rods["length"])
+    assert _kkt_residual(nn, o["I"], o["J"], o["rows"], o["mob"], oc, g["gamma"]) <= cfg["tol"]
+    tight, _ = O.bbpgd(nn, o["I"], o["J"], o["rows"], o["mob"], O.cfg_from_dict(dict(cfg, tol=1e-11)))
+    scale = max(np.max(np.abs(tight)), 1e-300)
+    err_form = np.max(np.abs(g["gamma"] - tight)) / scale
+    err_plain = np.max(np.abs(o["gamma"] - tight)) / scale
+    print(f"{kind}: gamma rel err vs tight FP64 solve: precision form {err_form:.2e}, plain FP64 {err_plain:.2e}; "
+          f"iters fp32 {rep['iters_fp32']} + fp64 {rep['iters']} vs plain {o['report']['iters']}")
+    assert err_form <= max(20 * err_plain, 1e-6)

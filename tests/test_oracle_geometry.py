@@ -1,0 +1,158 @@
+"""Pins of the oracle's geometry: segment-segment distance against exact 50-digit solutions,
+pair sets against brute force and invariances, cutoff boundary handling, reference inputs."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from alens_amd import systems
+from oracle import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_segment_distance_golden():
+    cases = json.load(open(os.path.join(GOLDEN, "segdist.json")))["cases"]
+    assert len(cases) >= 400
+    for cs in cases:
+        o = O.segseg(cs["cI"], cs["tI"], cs["hI"], cs["cJ"], cs["tJ"], cs["hJ"])
+        scale = 1.0 + max(abs(x) for x in cs["cI"] + cs["cJ"])
+        assert abs(o[2] - float(cs["dist"])) <= 4e-16 * scale, cs["kind"]
+        if cs["unique"] and cs["kind"] in ("random", "sphere", "far_origin"):
+            assert abs(o[0] - float(cs["s"])) <= 1e-12 * scale
+            assert abs(o[1] - float(cs["u"])) <= 1e-12 * scale
+
+
+def test_segment_distance_symmetric_and_consistent():
+    rng = np.random.default_rng(1)
+    for _ in range(500):
+        tI, tJ = [v / np.linalg.norm(v) for v in rng.normal(size=(2, 3))]
+        cI, cJ = rng.uniform(-1, 1, (2, 3))
+        hI, hJ = rng.uniform(0, 0.8, 2)
+        a = O.segseg(cI, tI, hI, cJ, tJ, hJ)
+        b = O.segseg(cJ, tJ, hJ, cI, tI, hI)
+        assert abs(a[2] - b[2]) <= 1e-15
+        # D = P - Q with P, Q the returned parameters
+        P = cI + a[0] * tI
+        Q = cJ + a[1] * tJ
+        assert np.allclose(P - Q, a[3:6], atol=1e-15)
+        assert abs(a[0]) <= hI and abs(a[1]) <= hJ
+        # a coarse grid never beats the solution
+        s = np.linspace(-hI, hI, 21)
+        u = np.linspace(-hJ, hJ, 21)
+        d = np.linalg.norm((cI + s[:, None, None] * tI) - (cJ + u[None, :, None] * tJ), axis=2)
+        assert d.min() >= a[2] - 1e-14
+
+
+WORKLOADS = [("small_3d", 1000), ("monolayer_slab", 6000), ("lognormal_3d", 16000)]
+
+
+@pytest.mark.parametrize("kind,n", WORKLOADS)
+def test_cell_list_matches_brute_force(kind, n):
+    rods, cfg = systems.scaled(kind, n, seed=21)
+    oc = O.cfg_from_dict(cfg)
+    a = O.pairs(rods, oc, "cells")
+    b = O.pairs(rods, oc, "brute")
+    assert a[0].size > 0
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("fname,cfg", [
+    ("dense_monolayer_TubuleInitial.dat", dict(box_hi=(11, 11, 11), pbc=(1, 1, 0), monolayer=1)),
+    ("dense_straining_TubuleInitial.dat", dict(box_hi=(3, 1, 1), pbc=(1, 0, 0))),
+])
+def test_reference_inputs_cells_vs_brute(fname, cfg):
+    rods = systems.read_rod_file(os.path.join(GOLDEN, fname))
+    oc = O.make_cfg(**cfg)
+    a = O.pairs(rods, oc, "cells")
+    b = O.pairs(rods, oc, "brute")
+    assert a[0].size > 100
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_mix_motor_sliding_known_answer_no_contact():
+    rods = systems.read_rod_file(os.path.join(GOLDEN, "mix_motor_sliding_TubuleInitial.dat"))
+    assert rods["length"].size == 2 and rods["immovable"][0] == 1
+    oc = O.make_cfg(box_hi=(20, 1, 1), pbc=(0, 0, 0))
+    assert O.pairs(rods, oc, "brute")[0].size == 0
+    # closing the gap to below colBuf creates exactly one contact
+    r2 = dict(rods, center=rods["center"].copy())
+    r2["center"][1, 2] -= 0.03
+    I, J, _ = O.pairs(r2, oc, "brute")
+    assert list(zip(I, J)) == [(0, 1)]
+
+
+def _gid_pairs(rods, res):
+    I, J, _ = res
+    g = rods["gid"]
+    return {(min(g[i], g[j]), max(g[i], g[j])) for i, j in zip(I, J)}
+
+
+def test_pair_set_invariant_under_permutation_and_lattice_shift():
+    rods, cfg = systems.small_3d(n=1500, box=4.6, seed=8)
+    oc = O.cfg_from_dict(cfg)
+    base = _gid_pairs(rods, O.pairs(rods, oc))
+    perm = np.random.default_rng(2).permutation(1500)
+    rp = {k: (v[perm] if isinstance(v, np.ndarray) else v) for k, v in rods.items()}
+    assert _gid_pairs(rp, O.pairs(rp, oc)) == base
+    rs = dict(rods, center=rods["center"] + np.array([4.6, -9.2, 0.0]))
+    assert _gid_pairs(rs, O.pairs(rs, oc)) == base
+
+
+def test_cutoff_is_strict_at_colbuf():
+    """A pair is recorded iff delta0 < colBuf: find the last z-offset whose delta0 is below colBuf and
+    check that it is recorded while the next representable offset is not."""
+    oc = O.make_cfg(box_hi=(4, 4, 4), pbc=(0, 0, 0), col_buf=0.025)
+    r = 0.0125
+
+    def rods_at(z):
+        return dict(gid=np.array([0, 1], np.int32), center=np.array([[2.0, 2.0, 2.0], [2.0, 2.0, z]]),
+                    dir=np.array([[1.0, 0, 0], [1.0, 0, 0]]), length=np.array([1.0, 1.0]), radius=np.array([r, r]))
+
+    def delta0(z):
+        return (O.segseg([2.0, 2.0, 2.0], [1.0, 0, 0], 0.5, [2.0, 2.0, z], [1.0, 0, 0], 0.5)[2] - r) - r
+
+    z = 2.05  # centre offset 2 r + colBuf
+    for _ in range(64):
+        if delta0(z) >= 0.025:
+            break
+        z = np.nextafter(z, 3.0)
+    for _ in range(128):
+        if delta0(z) < 0.025:
+            break
+        z = np.nextafter(z, 0.0)
+    above = np.nextafter(z, 3.0)
+    assert delta0(z) < 0.025 <= delta0(above)
+    assert O.pairs(rods_at(z), oc, "brute")[0].size == 1
+    assert O.pairs(rods_at(above), oc, "brute")[0].size == 0
+    assert O.pairs(rods_at(2.05), oc, "cells")[0].size == 1
+    assert O.pairs(rods_at(2.1), oc, "cells")[0].size == 0
+
+
+def test_rows_follow_reference_block_conventions():
+    """normal from J to I, pos relative to each centre, lab points differ by the gap along n
+    (SRC/TubuleSystem.cpp:719-726)."""
+    rods, cfg = systems.small_3d(n=800, box=3.8, seed=12)
+    oc = O.cfg_from_dict(cfg)
+    I, J, img = O.pairs(rods, oc)
+    rw = O.rows(rods, oc, I, J, img, np.zeros(6 * 800))
+    n = rw["nrm"].reshape(-1, 3)
+    assert np.allclose(np.linalg.norm(n, axis=1), 1.0, atol=1e-15)
+    box = np.array(cfg["box_hi"])
+    labI = rods["center"][I] + rw["posI"].reshape(-1, 3)
+    labJ = rods["center"][J] + img * box + rw["posJ"].reshape(-1, 3)
+    gap = rw["delta0"] + 2 * 0.0125
+    assert np.allclose(labI - labJ, gap[:, None] * n, atol=1e-13)
+    assert np.allclose(rw["rI"].reshape(-1, 3), np.cross(rw["posI"].reshape(-1, 3), n), atol=1e-15)
+    assert np.all(rw["delta0"] < 0.025)
+    assert np.array_equal(rw["b"], rw["delta0"])  # zero U_nc => b = delta0
+
+
+def test_small_periodic_box_is_rejected():
+    rods, cfg = systems.small_3d(n=50, box=3.2)
+    oc = O.cfg_from_dict(dict(cfg, box_hi=(2.5, 2.5, 2.5)))
+    with pytest.raises(ValueError):
+        O.pairs(rods, oc)
