@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(LIBDIR, "libalens.so")
-SOURCES = ["k_collide.hip", "k_solve.hip", "capi.hip"]
+SOURCES = ["k_collide.hip", "k_solve.hip", "k_mlp.hip", "capi.hip"]
 ARCH = os.environ.get("ALENS_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -20,10 +20,15 @@ STATUS_NAMES = {0: "OK", 1: "E_ARG", 2: "E_HIP", 3: "E_OOM", 4: "E_NONFINITE", 5
 # every entry point declared in include/alens_capi.h
 EXPORTS = [
     "alens_create", "alens_destroy", "alens_last_error", "alens_version", "alens_set_rods", "alens_get_rods",
+    "alens_sort_rods",
     "alens_set_step", "alens_get_step", "alens_step_constraints", "alens_step_euler",
     "alens_get_num_constraints", "alens_get_constraints", "alens_get_constraint_arrays", "alens_get_gamma",
     "alens_get_velocity", "alens_bcqp_solve",
+    "alens_mlp_set", "alens_mlp_get", "alens_get_features", "alens_mlp_predict", "alens_mlp_gradient",
+    "alens_mlp_train",
 ]
+MLP_PARAMS = 9473
+MLP_NORM = 25
 
 
 class Config(ctypes.Structure):
@@ -103,6 +108,7 @@ def lib():
     L.alens_version.restype = ctypes.c_char_p
     L.alens_set_rods.argtypes = [vp, ctypes.c_int32, _ip, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_uint8)]
     L.alens_get_rods.argtypes = [vp, _dp, _dp]
+    L.alens_sort_rods.argtypes = [vp, _ip]
     L.alens_set_step.argtypes = [vp, ctypes.c_int64]
     L.alens_get_step.argtypes = [vp]
     L.alens_get_step.restype = ctypes.c_int64
@@ -115,6 +121,14 @@ def lib():
     L.alens_get_velocity.argtypes = [vp, _dp, _dp, _dp, _dp]
     L.alens_bcqp_solve.argtypes = [vp, ctypes.c_int32, ctypes.c_int64, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                    ctypes.POINTER(Report)]
+    _fp = ctypes.POINTER(ctypes.c_float)
+    L.alens_mlp_set.argtypes = [vp, _fp, _fp]
+    L.alens_mlp_get.argtypes = [vp, _fp, _fp]
+    L.alens_get_features.argtypes = [vp, _fp]
+    L.alens_mlp_predict.argtypes = [vp, ctypes.c_int64, _fp, _fp]
+    L.alens_mlp_gradient.argtypes = [vp, ctypes.c_int64, _fp, _fp, _fp, _fp]
+    L.alens_mlp_train.argtypes = [vp, ctypes.c_int64, _fp, _fp, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                                  ctypes.c_uint64, _fp]
     for name in EXPORTS:
         fn = getattr(L, name)
         if name not in ("alens_destroy", "alens_last_error", "alens_version", "alens_get_step"):
@@ -207,6 +221,12 @@ class Context:
         self._check(lib().alens_get_rods(self._h, _p(c, ctypes.c_double), _p(t, ctypes.c_double)))
         return c, t
 
+    def sort_rods(self):
+        """Reorder the rods spatially on the device; returns perm (new index -> previous index)."""
+        perm = np.zeros(max(self.n, 1), np.int32)
+        self._check(lib().alens_sort_rods(self._h, _p(perm, ctypes.c_int32)))
+        return perm[: self.n]
+
     def set_step(self, step):
         self._check(lib().alens_set_step(self._h, int(step)))
 
@@ -284,3 +304,46 @@ class Context:
         d = rep.as_dict()
         d["status_code"] = st
         return g[:m], d
+
+    # ---- learned warm start ----
+    def mlp_set(self, params, norm):
+        P = np.ascontiguousarray(params, np.float32)
+        N = np.ascontiguousarray(norm, np.float32)
+        assert P.size == MLP_PARAMS and N.size == MLP_NORM
+        self._check(lib().alens_mlp_set(self._h, _p(P, ctypes.c_float), _p(N, ctypes.c_float)))
+
+    def mlp_get(self):
+        P = np.zeros(MLP_PARAMS, np.float32)
+        N = np.zeros(MLP_NORM, np.float32)
+        self._check(lib().alens_mlp_get(self._h, _p(P, ctypes.c_float), _p(N, ctypes.c_float)))
+        return P, N
+
+    def features(self):
+        m = self.num_constraints()
+        X = np.zeros((max(m, 1), 16), np.float32)
+        self._check(lib().alens_get_features(self._h, _p(X, ctypes.c_float)))
+        return X[:m]
+
+    def mlp_predict(self, X):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, 16)
+        y = np.zeros(max(len(X), 1), np.float32)
+        self._check(lib().alens_mlp_predict(self._h, len(X), _p(X, ctypes.c_float), _p(y, ctypes.c_float)))
+        return y[: len(X)]
+
+    def mlp_gradient(self, X, target):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, 16)
+        t = np.ascontiguousarray(target, np.float32)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = ctypes.c_float()
+        self._check(lib().alens_mlp_gradient(self._h, len(X), _p(X, ctypes.c_float), _p(t, ctypes.c_float),
+                                             _p(g, ctypes.c_float), ctypes.byref(loss)))
+        return g, loss.value
+
+    def mlp_train(self, X, gamma, epochs=10, batch=4096, lr=2e-3, seed=1):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, 16)
+        gm = np.ascontiguousarray(gamma, np.float32)
+        losses = np.zeros(max(epochs, 1), np.float32)
+        self._check(lib().alens_mlp_train(self._h, len(X), _p(X, ctypes.c_float), _p(gm, ctypes.c_float),
+                                          int(epochs), int(batch), float(lr), int(seed),
+                                          _p(losses, ctypes.c_float)))
+        return losses[:epochs]

@@ -100,11 +100,17 @@ HipErr radix_sort_pairs(hipStream_t st, uint32_t *keys, uint32_t *vals, uint32_t
 HipErr launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, int64_t n);
 HipErr launch_cell_ranges(hipStream_t st, const uint32_t *keys, int64_t E, uint32_t *cstart,
                           uint32_t *cend);
-HipErr launch_sweep(hipStream_t st, const RodsDev &r, const CfgDev &c, const Grid &g,
-                    const uint32_t *cstart, const uint32_t *cend, const uint32_t *vals,
-                    const int32_t *clo, const double *blo, const double *bhi, uint32_t *pair_count,
-                    uint32_t cap, uint32_t *pI, uint32_t *pJ, uint32_t *pImg, uint32_t *cntI,
-                    uint32_t *cntJ, int n_cu);
+HipErr launch_sweep(hipStream_t st, const RodsDev &r, const CfgDev &c, const Grid &g, const uint32_t *keys,
+                    int64_t E, const uint32_t *cend, const uint32_t *vals, const int32_t *clo, const double *blo,
+                    const double *bhi, uint32_t *pair_count, uint32_t cap, uint32_t *pI, uint32_t *pJ,
+                    uint32_t *pImg, uint32_t *cntI, uint32_t *cntJ);
+HipErr launch_morton_keys(hipStream_t st, int32_t n, const double *center, const CfgDev &c, const Grid &g,
+                          uint32_t *keys, uint32_t *vals);
+HipErr launch_permute_rods(hipStream_t st, int32_t n, const uint32_t *perm, const int32_t *gid,
+                           const double *center, const double *dir, const double *length, const double *radius,
+                           const double *hcol, const double *rcol, const uint8_t *immov, int32_t *gid2,
+                           double *center2, double *dir2, double *length2, double *radius2, double *hcol2,
+                           double *rcol2, uint8_t *immov2);
 HipErr launch_order_I(hipStream_t st, int32_t n, uint32_t m, const uint32_t *pI,
                       const uint32_t *pJ, const uint32_t *pImg, const uint32_t *rowStart,
                       uint32_t *cur, int32_t *cI, int32_t *cJ, uint32_t *cImg);
@@ -176,5 +182,29 @@ HipErr launch_make_keys(hipStream_t st, uint32_t m, const int32_t *cI, const int
                         const int32_t *gid, const double2 *xg, uint64_t *key, double *gam);
 
 constexpr int kRedQ = 8;  // partial-sum slots per chunk
+
+// learned warm start (k_mlp.hip): 12 -> 64 -> 64 -> 64 -> 1; W1 stored 64 x 16 (columns 12..15 zero)
+constexpr int kMlpIn = 12;
+constexpr int kPW1 = 0, kPB1 = 1024, kPW2 = 1088, kPB2 = 5184, kPW3 = 5248, kPB3 = 9344, kPW4 = 9408,
+              kPB4 = 9472;
+constexpr int kMlpParams = 9473;
+constexpr int kMlpNorm = 2 * kMlpIn + 1;  // mean[12], 1/std[12], gamma scale
+struct MlpScratch {
+    float *XT, *Z1, *H1, *Z2, *H2, *Z3, *H3, *dO, *dZ1, *dZ2, *dZ3, *loss, *lossv;
+};
+HipErr launch_features(hipStream_t st, uint32_t m, const int32_t *cI, const int32_t *cJ, const double *delta0,
+                       const double *geo, const double *posI, const double *posJ, const double *dir,
+                       const double *length, const double *unc, const double2 *xprev, double dt, float *X);
+HipErr launch_mlp_infer_f32(hipStream_t st, uint32_t m, const float *X, const float *P, const float *norm,
+                            float *out);
+HipErr launch_mlp_infer_xg(hipStream_t st, uint32_t m, const float *X, const float *P, const float *norm,
+                           double2 *xg);
+HipErr launch_gemm_bf16(hipStream_t st, int M, int N, int K, const float *A, int lda, int tA, const float *B,
+                        int ldb, int tB, float *C, int ldc, int ksplit);
+HipErr launch_gather_batch(hipStream_t st, int B, const int32_t *idx, const float *X, const float *t, float *XB,
+                           float *tB);
+HipErr mlp_gradient(hipStream_t st, int B, const float *X, const float *t, const float *P, const float *norm,
+                    MlpScratch &s, float *grad);
+HipErr mlp_adam_step(hipStream_t st, float *P, const float *grad, float *m1, float *m2, float lr, int step);
 
 }  // namespace alens

@@ -53,6 +53,11 @@ struct alens_ctx {
     // precision form: FP32 copies of the operator data and the FP32 iterate
     DBuf<float> geo32, adjW32, mob32, b32, U32;
     DBuf<float2> xg32;
+    // learned warm start: FP32 master parameters, normalisation, features of the step
+    DBuf<float> mlpP, mlpNorm, feat;
+    bool mlp_ready = false;
+    DBuf<double2> xprev;  // previous-step gamma matched to this step's rows (x component)
+    bool have_xprev = false;
     DBuf<uint32_t> bar;
 
     DBuf<uint64_t> pkey;
@@ -157,7 +162,7 @@ alens_status alens_create(const alens_config *cfg, alens_ctx **out) {
         return bad("params");
     if (cfg->solver != 0) return bad("solver");
     if (cfg->precision != 0 && cfg->precision != 1) return bad("precision");
-    if (cfg->warm_start < 0 || cfg->warm_start > 1) return bad("warm_start");
+    if (cfg->warm_start < 0 || cfg->warm_start > 2) return bad("warm_start");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
         delete ctx;
@@ -196,6 +201,7 @@ void alens_destroy(alens_ctx *ctx) {
     rel(ctx->adjStart); rel(ctx->adjK); rel(ctx->adjW); rel(ctx->xg); rel(ctx->gamma); rel(ctx->part);
     rel(ctx->out); rel(ctx->bar); rel(ctx->pkey); rel(ctx->pgam);
     rel(ctx->geo32); rel(ctx->adjW32); rel(ctx->mob32); rel(ctx->b32); rel(ctx->U32); rel(ctx->xg32);
+    rel(ctx->mlpP); rel(ctx->mlpNorm); rel(ctx->feat); rel(ctx->xprev);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->st) (void)hipStreamDestroy(ctx->st);
@@ -317,6 +323,71 @@ alens_status alens_get_rods(alens_ctx *ctx, double *center, double *dir) {
         if (dir) TRY_HIP(hipMemcpyAsync(dir, ctx->dir.p, sizeof(double) * 3 * ctx->n, hipMemcpyDeviceToHost, ctx->st));
     }
     TRY_HIP(hipStreamSynchronize(ctx->st));
+    return ALENS_OK;
+}
+
+alens_status alens_sort_rods(alens_ctx *ctx, int32_t *perm_out) {
+    if (!ctx) return ALENS_E_ARG;
+    if (!ctx->have_rods) return fail(ctx, ALENS_E_STATE, "sort_rods before set_rods");
+    TRY_HIP(hipSetDevice(ctx->dev));
+    hipStream_t st = ctx->st;
+    const int32_t n = ctx->n;
+    if (n <= 1) {
+        if (perm_out && n == 1) perm_out[0] = 0;
+        return ALENS_OK;
+    }
+    const size_t N = (size_t)n;
+    TRY_HIP(ctx->keys.ensure(N));
+    TRY_HIP(ctx->vals.ensure(N));
+    TRY_HIP(ctx->keys2.ensure(N));
+    TRY_HIP(ctx->vals2.ensure(N));
+    const CfgDev cd = make_cfgdev(ctx);
+    TRY_K(launch_morton_keys(st, n, ctx->center.p, cd, ctx->grid, ctx->keys.p, ctx->vals.p));
+    bool alt = false;
+    TRY_K(radix_sort_pairs(st, ctx->keys.p, ctx->vals.p, ctx->keys2.p, ctx->vals2.p, n, 30, ctx->hist, ctx->scan, alt));
+    const uint32_t *perm = alt ? ctx->vals2.p : ctx->vals.p;
+    DBuf<int32_t> gid2;
+    DBuf<double> center2, dir2, length2, radius2, hcol2, rcol2;
+    DBuf<uint8_t> immov2;
+    TRY_HIP(gid2.ensure(N));
+    TRY_HIP(center2.ensure(3 * N));
+    TRY_HIP(dir2.ensure(3 * N));
+    TRY_HIP(length2.ensure(N));
+    TRY_HIP(radius2.ensure(N));
+    TRY_HIP(hcol2.ensure(N));
+    TRY_HIP(rcol2.ensure(N));
+    if (ctx->has_immov) TRY_HIP(immov2.ensure(N));
+    TRY_K(launch_permute_rods(st, n, perm, ctx->gid.p, ctx->center.p, ctx->dir.p, ctx->length.p, ctx->radius.p,
+                              ctx->hcol.p, ctx->rcol.p, ctx->has_immov ? ctx->immov.p : nullptr, gid2.p, center2.p,
+                              dir2.p, length2.p, radius2.p, hcol2.p, rcol2.p,
+                              ctx->has_immov ? immov2.p : nullptr));
+    std::vector<uint32_t> hp(N);
+    TRY_HIP(hipMemcpyAsync(hp.data(), perm, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, st));
+    TRY_HIP(hipStreamSynchronize(st));
+    std::swap(ctx->gid, gid2);
+    std::swap(ctx->center, center2);
+    std::swap(ctx->dir, dir2);
+    std::swap(ctx->length, length2);
+    std::swap(ctx->radius, radius2);
+    std::swap(ctx->hcol, hcol2);
+    std::swap(ctx->rcol, rcol2);
+    if (ctx->has_immov) std::swap(ctx->immov, immov2);
+    gid2.release();
+    center2.release();
+    dir2.release();
+    length2.release();
+    radius2.release();
+    hcol2.release();
+    rcol2.release();
+    immov2.release();
+    std::vector<int32_t> ng(N);
+    for (size_t i = 0; i < N; ++i) {
+        ng[i] = ctx->h_gid[hp[i]];
+        if (perm_out) perm_out[i] = (int32_t)hp[i];
+    }
+    ctx->h_gid.swap(ng);
+    ctx->have_prev = false;
+    ctx->have_step = false;
     return ALENS_OK;
 }
 
@@ -489,9 +560,9 @@ alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_s
         TRY_HIP(hipMemsetAsync(ctx->counter.p, 0, sizeof(uint32_t), st));
         TRY_HIP(hipMemsetAsync(ctx->cntI.p, 0, sizeof(uint32_t) * (n + 1), st));
         TRY_HIP(hipMemsetAsync(ctx->cntJ.p, 0, sizeof(uint32_t) * (n + 1), st));
-        TRY_K(launch_sweep(st, rd, cd, g, ctx->cstart.p, ctx->cend.p, svals, ctx->clo.p, ctx->blo.p, ctx->bhi.p,
+        TRY_K(launch_sweep(st, rd, cd, g, skeys, E, ctx->cend.p, svals, ctx->clo.p, ctx->blo.p, ctx->bhi.p,
                            ctx->counter.p, ctx->pcap, ctx->pI.p, ctx->pJ.p, ctx->pImg.p, ctx->cntI.p,
-                           ctx->cntJ.p, ctx->ncu));
+                           ctx->cntJ.p));
         TRY_HIP(hipMemcpyAsync(&M, ctx->counter.p, sizeof M, hipMemcpyDeviceToHost, st));
         TRY_HIP(hipStreamSynchronize(st));
         if (M <= ctx->pcap) break;
@@ -530,15 +601,33 @@ alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_s
     TRY_K(launch_adjacency(st, n, ctx->rowStart.p, ctx->jStart.p, ctx->jlist.p, ctx->geo.p, ctx->adjStart.p,
                            ctx->adjK.p, ctx->adjW.p));
     TRY_HIP(hipEventRecord(ctx->ev[1], st));
-    // solve
+    // solve: start point = 0 (cold), the previous step's gamma (warm_start 1) or the network's
+    // prediction from this step's features and the previous gamma (warm_start 2)
     float ms_solve = 0.0f;
-    const bool warm = ctx->cfg.warm_start == 1 && ctx->have_prev;
+    const int ws = ctx->cfg.warm_start;
+    if (ws == 2 && !ctx->mlp_ready) return fail(ctx, ALENS_E_STATE, "warm_start = 2 needs network parameters");
+    bool warm = false;
+    ctx->have_xprev = false;
     if (M > 0) {
-        if (warm)
+        if (ws >= 1 && ctx->have_prev) {
             TRY_K(launch_warm_match(st, M, ctx->cI.p, ctx->cJ.p, ctx->gid.p, ctx->mprev, ctx->pkey.p, ctx->pgam.p,
                                     ctx->xg.p));
-        else
+            TRY_HIP(ctx->xprev.ensure(Mc));
+            TRY_HIP(hipMemcpyAsync(ctx->xprev.p, ctx->xg.p, sizeof(double2) * M, hipMemcpyDeviceToDevice, st));
+            ctx->have_xprev = true;
+            warm = true;
+        } else {
             TRY_K(launch_init_xg(st, ctx->xg.p, nullptr, M));
+        }
+        if (ws == 2) {
+            TRY_HIP(ctx->feat.ensure(16 * Mc));
+            TRY_K(launch_features(st, M, ctx->cI.p, ctx->cJ.p, ctx->delta0.p, ctx->geo.p, ctx->posI.p, ctx->posJ.p,
+                                  ctx->dir.p, ctx->length.p, ctx->unc.p, ctx->have_xprev ? ctx->xprev.p : nullptr,
+                                  ctx->cfg.dt, ctx->feat.p));
+            TRY_K(launch_init_xg(st, ctx->xg.p, nullptr, M));
+            TRY_K(launch_mlp_infer_xg(st, M, ctx->feat.p, ctx->mlpP.p, ctx->mlpNorm.p, ctx->xg.p));
+            warm = true;
+        }
         alens_status s = ctx->cfg.precision == 1 ? run_solver_mixed(ctx, n, M, warm, &rep, &ms_solve)
                                                  : run_solver(ctx, n, M, ctx->mob.p, warm, &rep, &ms_solve);
         if (s != ALENS_OK) return s;
@@ -547,7 +636,7 @@ alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_s
     }
     TRY_K(launch_writeback(st, n, ctx->adjStart.p, ctx->adjK.p, ctx->adjW.p, ctx->xg.p, ctx->mob.p, ctx->unc.p,
                            ctx->cfg.monolayer ? 1 : 0, ctx->F.p, ctx->U.p, ctx->gamma.p, M));
-    if (ctx->cfg.warm_start == 1) {
+    if (ctx->cfg.warm_start >= 1) {
         TRY_HIP(ctx->pkey.ensure(Mc));
         TRY_HIP(ctx->pgam.ensure(Mc));
         TRY_K(launch_make_keys(st, M, ctx->cI.p, ctx->cJ.p, ctx->gid.p, ctx->xg.p, ctx->pkey.p, ctx->pgam.p));
@@ -775,6 +864,217 @@ alens_status alens_bcqp_solve(alens_ctx *ctx, int32_t n, int64_t m, const int32_
     if (rep_out) *rep_out = rep;
     if (rep.status == ALENS_E_NONFINITE) return fail(ctx, ALENS_E_NONFINITE, "non-finite BBPGD residual");
     if (rep.status == ALENS_E_NOCONV) return fail(ctx, ALENS_E_NOCONV, "BBPGD did not reach tol within max_ite");
+    return ALENS_OK;
+}
+
+alens_status alens_mlp_set(alens_ctx *ctx, const float *params, const float *norm) {
+    if (!ctx || !params || !norm) return ALENS_E_ARG;
+    for (int i = 0; i < kMlpNorm; ++i)
+        if (!std::isfinite(norm[i])) return fail(ctx, ALENS_E_NONFINITE, "mlp_set: non-finite normalisation");
+    for (int i = 0; i < kMlpParams; ++i)
+        if (!std::isfinite(params[i])) return fail(ctx, ALENS_E_NONFINITE, "mlp_set: non-finite parameter");
+    TRY_HIP(hipSetDevice(ctx->dev));
+    TRY_HIP(ctx->mlpP.ensure(kMlpParams));
+    TRY_HIP(ctx->mlpNorm.ensure(kMlpNorm));
+    TRY_HIP(hipMemcpyAsync(ctx->mlpP.p, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, ctx->st));
+    TRY_HIP(hipMemcpyAsync(ctx->mlpNorm.p, norm, sizeof(float) * kMlpNorm, hipMemcpyHostToDevice, ctx->st));
+    TRY_HIP(hipStreamSynchronize(ctx->st));
+    ctx->mlp_ready = true;
+    return ALENS_OK;
+}
+
+alens_status alens_mlp_get(alens_ctx *ctx, float *params, float *norm) {
+    if (!ctx) return ALENS_E_ARG;
+    if (!ctx->mlp_ready) return fail(ctx, ALENS_E_STATE, "no network parameters");
+    TRY_HIP(hipSetDevice(ctx->dev));
+    if (params) TRY_HIP(hipMemcpyAsync(params, ctx->mlpP.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, ctx->st));
+    if (norm) TRY_HIP(hipMemcpyAsync(norm, ctx->mlpNorm.p, sizeof(float) * kMlpNorm, hipMemcpyDeviceToHost, ctx->st));
+    TRY_HIP(hipStreamSynchronize(ctx->st));
+    return ALENS_OK;
+}
+
+alens_status alens_get_features(alens_ctx *ctx, float *X) {
+    if (!ctx || (!X && ctx->m)) return ALENS_E_ARG;
+    if (!ctx->have_step) return fail(ctx, ALENS_E_STATE, "no step has been run");
+    TRY_HIP(hipSetDevice(ctx->dev));
+    const uint32_t M = (uint32_t)ctx->m;
+    if (M == 0) return ALENS_OK;
+    TRY_HIP(ctx->feat.ensure(16 * (size_t)M));
+    TRY_K(launch_features(ctx->st, M, ctx->cI.p, ctx->cJ.p, ctx->delta0.p, ctx->geo.p, ctx->posI.p, ctx->posJ.p,
+                          ctx->dir.p, ctx->length.p, ctx->unc.p, ctx->have_xprev ? ctx->xprev.p : nullptr,
+                          ctx->cfg.dt, ctx->feat.p));
+    TRY_HIP(hipMemcpyAsync(X, ctx->feat.p, sizeof(float) * 16 * M, hipMemcpyDeviceToHost, ctx->st));
+    TRY_HIP(hipStreamSynchronize(ctx->st));
+    return ALENS_OK;
+}
+
+alens_status alens_mlp_predict(alens_ctx *ctx, int64_t m, const float *X, float *gamma0) {
+    if (!ctx || m < 0 || m > 0x7fffffff || (m > 0 && (!X || !gamma0))) return ALENS_E_ARG;
+    if (!ctx->mlp_ready) return fail(ctx, ALENS_E_STATE, "no network parameters");
+    if (m == 0) return ALENS_OK;
+    TRY_HIP(hipSetDevice(ctx->dev));
+    DBuf<float> dX, dY;
+    TRY_HIP(dX.ensure(16 * (size_t)m));
+    TRY_HIP(dY.ensure((size_t)m));
+    TRY_HIP(hipMemcpyAsync(dX.p, X, sizeof(float) * 16 * m, hipMemcpyHostToDevice, ctx->st));
+    TRY_K(launch_mlp_infer_f32(ctx->st, (uint32_t)m, dX.p, ctx->mlpP.p, ctx->mlpNorm.p, dY.p));
+    TRY_HIP(hipMemcpyAsync(gamma0, dY.p, sizeof(float) * m, hipMemcpyDeviceToHost, ctx->st));
+    TRY_HIP(hipStreamSynchronize(ctx->st));
+    dX.release();
+    dY.release();
+    return ALENS_OK;
+}
+
+namespace {
+struct TrainBufs {
+    DBuf<float> XT, Z1, H1, Z2, H2, Z3, H3, dO, dZ1, dZ2, dZ3, loss, lossv;
+    hipError_t ensure(int B) {
+        const size_t nb = (size_t)64 * B;
+        hipError_t e;
+        if ((e = XT.ensure(16 * (size_t)B)) != hipSuccess) return e;
+        for (auto *b : {&Z1, &H1, &Z2, &H2, &Z3, &H3, &dZ1, &dZ2, &dZ3})
+            if ((e = b->ensure(nb)) != hipSuccess) return e;
+        if ((e = dO.ensure((size_t)B)) != hipSuccess) return e;
+        if ((e = lossv.ensure((size_t)B)) != hipSuccess) return e;
+        return loss.ensure(4);
+    }
+    MlpScratch view() {
+        return MlpScratch{XT.p, Z1.p, H1.p, Z2.p, H2.p, Z3.p, H3.p, dO.p, dZ1.p, dZ2.p, dZ3.p, loss.p, lossv.p};
+    }
+    void release() {
+        for (auto *b : {&XT, &Z1, &H1, &Z2, &H2, &Z3, &H3, &dO, &dZ1, &dZ2, &dZ3, &loss, &lossv}) b->release();
+    }
+};
+}  // namespace
+
+alens_status alens_mlp_gradient(alens_ctx *ctx, int64_t m, const float *X, const float *target, float *grad,
+                                float *loss) {
+    if (!ctx || m <= 0 || m % 16 || m > (1 << 26) || !X || !target || !grad) return ALENS_E_ARG;
+    if (!ctx->mlp_ready) return fail(ctx, ALENS_E_STATE, "no network parameters");
+    TRY_HIP(hipSetDevice(ctx->dev));
+    TrainBufs tb;
+    DBuf<float> dX, dT, dG;
+    TRY_HIP(tb.ensure((int)m));
+    TRY_HIP(dX.ensure(16 * (size_t)m));
+    TRY_HIP(dT.ensure((size_t)m));
+    TRY_HIP(dG.ensure(kMlpParams));
+    TRY_HIP(hipMemcpyAsync(dX.p, X, sizeof(float) * 16 * m, hipMemcpyHostToDevice, ctx->st));
+    TRY_HIP(hipMemcpyAsync(dT.p, target, sizeof(float) * m, hipMemcpyHostToDevice, ctx->st));
+    MlpScratch sv = tb.view();
+    TRY_K(mlp_gradient(ctx->st, (int)m, dX.p, dT.p, ctx->mlpP.p, ctx->mlpNorm.p, sv, dG.p));
+    TRY_HIP(hipMemcpyAsync(grad, dG.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, ctx->st));
+    float l = 0.f;
+    TRY_HIP(hipMemcpyAsync(&l, tb.loss.p, sizeof(float), hipMemcpyDeviceToHost, ctx->st));
+    TRY_HIP(hipStreamSynchronize(ctx->st));
+    if (loss) *loss = l;
+    tb.release();
+    dX.release();
+    dT.release();
+    dG.release();
+    return ALENS_OK;
+}
+
+alens_status alens_mlp_train(alens_ctx *ctx, int64_t m, const float *X, const float *gamma, int32_t epochs,
+                             int32_t batch, float lr, uint64_t seed, float *loss_per_epoch) {
+    if (!ctx || m <= 0 || m > (1 << 28) || !X || !gamma || epochs < 0 || batch < 16 || batch % 16 || !(lr > 0.f))
+        return ALENS_E_ARG;
+    if (batch > m) batch = (int32_t)(m / 16 * 16);
+    if (batch < 16) return fail(ctx, ALENS_E_ARG, "mlp_train: fewer than 16 samples");
+    TRY_HIP(hipSetDevice(ctx->dev));
+    hipStream_t st = ctx->st;
+    // normalisation from the data unless a network is already loaded
+    std::vector<float> norm(kMlpNorm), P(kMlpParams, 0.0f);
+    if (!ctx->mlp_ready) {
+        for (int f = 0; f < kMlpIn; ++f) {
+            double s = 0, s2 = 0;
+            for (int64_t i = 0; i < m; ++i) {
+                const double v = X[16 * i + f];
+                s += v;
+                s2 += v * v;
+            }
+            const double mu = s / m, var = std::max(s2 / m - mu * mu, 0.0);
+            norm[f] = (float)mu;
+            norm[kMlpIn + f] = (float)(1.0 / std::max(std::sqrt(var), 1e-12));
+        }
+        double g2 = 0;
+        for (int64_t i = 0; i < m; ++i) g2 += (double)gamma[i] * gamma[i];
+        norm[2 * kMlpIn] = (float)(g2 > 0 ? std::sqrt(g2 / m) : 1.0);
+        // Glorot-uniform weights (xorshift64*), zero biases, output bias at softplus^{-1}(1)
+        uint64_t rs = seed ? seed : 0x9E3779B97F4A7C15ull;
+        auto urand = [&]() {
+            rs ^= rs >> 12;
+            rs ^= rs << 25;
+            rs ^= rs >> 27;
+            return (double)((rs * 2685821657736338717ull) >> 11) * 0x1p-53;
+        };
+        auto fill = [&](int off, int rows, int cols, int fan_in) {
+            const double a = std::sqrt(6.0 / (fan_in + rows));
+            for (int r = 0; r < rows; ++r)
+                for (int c = 0; c < cols; ++c)
+                    P[off + r * cols + c] = (c < fan_in) ? (float)((2.0 * urand() - 1.0) * a) : 0.0f;
+        };
+        fill(kPW1, 64, 16, kMlpIn);
+        fill(kPW2, 64, 64, 64);
+        fill(kPW3, 64, 64, 64);
+        fill(kPW4, 1, 64, 64);
+        P[kPB4] = 0.5413248546f;
+        alens_status s = alens_mlp_set(ctx, P.data(), norm.data());
+        if (s != ALENS_OK) return s;
+    } else {
+        alens_status s = alens_mlp_get(ctx, nullptr, norm.data());
+        if (s != ALENS_OK) return s;
+    }
+    const float gs = norm[2 * kMlpIn];
+    std::vector<float> t(m);
+    for (int64_t i = 0; i < m; ++i) t[i] = gamma[i] / gs;
+    TrainBufs tb;
+    DBuf<float> dX, dT, dXB, dTB, dG, dM1, dM2, dL;
+    DBuf<int32_t> dIdx;
+    const int64_t nb = m / batch;
+    TRY_HIP(tb.ensure(batch));
+    TRY_HIP(dX.ensure(16 * (size_t)m));
+    TRY_HIP(dT.ensure((size_t)m));
+    TRY_HIP(dXB.ensure(16 * (size_t)batch));
+    TRY_HIP(dTB.ensure((size_t)batch));
+    TRY_HIP(dG.ensure(kMlpParams));
+    TRY_HIP(dM1.ensure(kMlpParams));
+    TRY_HIP(dM2.ensure(kMlpParams));
+    TRY_HIP(dL.ensure((size_t)std::max<int64_t>(nb, 1)));
+    TRY_HIP(dIdx.ensure((size_t)m));
+    TRY_HIP(hipMemcpyAsync(dX.p, X, sizeof(float) * 16 * m, hipMemcpyHostToDevice, st));
+    TRY_HIP(hipMemcpyAsync(dT.p, t.data(), sizeof(float) * m, hipMemcpyHostToDevice, st));
+    TRY_HIP(hipMemsetAsync(dM1.p, 0, sizeof(float) * kMlpParams, st));
+    TRY_HIP(hipMemsetAsync(dM2.p, 0, sizeof(float) * kMlpParams, st));
+    std::vector<int32_t> idx(m);
+    std::vector<float> lh((size_t)std::max<int64_t>(nb, 1));
+    MlpScratch sv = tb.view();
+    uint64_t rs = seed ^ 0xD1B54A32D192ED03ull;
+    int step = 0;
+    for (int ep = 0; ep < epochs; ++ep) {
+        for (int64_t i = 0; i < m; ++i) idx[i] = (int32_t)i;
+        for (int64_t i = m - 1; i > 0; --i) {  // Fisher-Yates with xorshift64*
+            rs ^= rs >> 12;
+            rs ^= rs << 25;
+            rs ^= rs >> 27;
+            const int64_t j = (int64_t)(((rs * 2685821657736338717ull) >> 11) % (uint64_t)(i + 1));
+            std::swap(idx[i], idx[j]);
+        }
+        TRY_HIP(hipMemcpyAsync(dIdx.p, idx.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, st));
+        for (int64_t b = 0; b < nb; ++b) {
+            TRY_K(launch_gather_batch(st, batch, dIdx.p + b * batch, dX.p, dT.p, dXB.p, dTB.p));
+            TRY_K(mlp_gradient(st, batch, dXB.p, dTB.p, ctx->mlpP.p, ctx->mlpNorm.p, sv, dG.p));
+            TRY_K(mlp_adam_step(st, ctx->mlpP.p, dG.p, dM1.p, dM2.p, lr, ++step));
+            TRY_HIP(hipMemcpyAsync(dL.p + b, tb.loss.p, sizeof(float), hipMemcpyDeviceToDevice, st));
+        }
+        TRY_HIP(hipMemcpyAsync(lh.data(), dL.p, sizeof(float) * nb, hipMemcpyDeviceToHost, st));
+        TRY_HIP(hipStreamSynchronize(st));
+        double acc = 0;
+        for (int64_t b = 0; b < nb; ++b) acc += lh[b];
+        if (loss_per_epoch) loss_per_epoch[ep] = (float)(acc / std::max<int64_t>(nb, 1));
+    }
+    tb.release();
+    for (auto *b : {&dX, &dT, &dXB, &dTB, &dG, &dM1, &dM2, &dL}) b->release();
+    dIdx.release();
     return ALENS_OK;
 }
 

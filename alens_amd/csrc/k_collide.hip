@@ -411,40 +411,40 @@ HipErr launch_cell_ranges(hipStream_t st, const uint32_t *keys, int64_t E, uint3
 }
 
 // ------------------------------------------------------------------------------------------------
-// Pair sweep: one 64-lane wave per cell.  For entries a < b of the cell (rods i < j) the pair is
-// examined here only if, on every axis, this cell is the first cell of i's or of j's range
-// (unwrapped), which visits each (pair, image) once; the image shift of j follows from the
-// unwrapped cell indices.  AABB reject, then the FP64 segment distance and delta0 < colBuf.
-// Emission is wave-aggregated; per-rod counts for both sides are accumulated for the ordering.
+// Pair sweep.  For entries a < b of one cell (rods i < j) the pair is examined there only if, on every
+// axis, the cell is the first cell of i's or of j's range (unwrapped), which visits each
+// (pair, image) once; the image shift of j follows from the unwrapped cell indices.  AABB reject, then
+// the FP64 segment distance and delta0 < colBuf.  Per-rod counts of both sides feed the ordering.
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_sweep(RodsDev r, CfgDev c, Grid g,
-                                               const uint32_t *__restrict__ cstart,
-                                               const uint32_t *__restrict__ cend,
-                                               const uint32_t *__restrict__ vals,
-                                               const int32_t *__restrict__ clo,
-                                               const double *__restrict__ blo,
-                                               const double *__restrict__ bhi,
-                                               uint32_t *__restrict__ pair_count, uint32_t cap,
-                                               uint32_t *__restrict__ pI, uint32_t *__restrict__ pJ,
-                                               uint32_t *__restrict__ pImg,
-                                               uint32_t *__restrict__ cntI,
-                                               uint32_t *__restrict__ cntJ) {
+// One thread per sorted entry e (rod i in cell c): pairs with the later entries of the same cell.
+// Lanes of a wave run the same trip count (the wave maximum) so emission stays wave-aggregated.
+__global__ __launch_bounds__(256) void k_sweep_entries(RodsDev r, CfgDev c, Grid g, const uint32_t *__restrict__ keys,
+                                                       int64_t E, const uint32_t *__restrict__ cend,
+                                                       const uint32_t *__restrict__ vals,
+                                                       const int32_t *__restrict__ clo,
+                                                       const double *__restrict__ blo,
+                                                       const double *__restrict__ bhi,
+                                                       uint32_t *__restrict__ pair_count, uint32_t cap,
+                                                       uint32_t *__restrict__ pI, uint32_t *__restrict__ pJ,
+                                                       uint32_t *__restrict__ pImg,
+                                                       uint32_t *__restrict__ cntI,
+                                                       uint32_t *__restrict__ cntJ) {
     const int lane = threadIdx.x & 63;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const int64_t wpb = blockDim.x >> 6;
-    const int64_t gw = (int64_t)blockIdx.x * wpb + (threadIdx.x >> 6);
-    const int64_t nw = (int64_t)gridDim.x * wpb;
-    for (int64_t cell = gw; cell < g.ncell; cell += nw) {
-        const uint32_t s0 = cstart[cell], s1 = cend[cell];
-        if (s1 - s0 < 2) continue;
-        const int cc0 = (int)(cell % g.nc[0]);
-        const int cc1 = (int)((cell / g.nc[0]) % g.nc[1]);
-        const int cc2 = (int)(cell / ((int64_t)g.nc[0] * g.nc[1]));
-        const int cc[3] = {cc0, cc1, cc2};
-        for (uint32_t pa = s0; pa + 1 < s1; ++pa) {
-            const uint32_t i = vals[pa];
-            int cli[3], ui[3];
-            double bli[3], bhi_i[3], ci[3], ti[3];
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t cnt = 0, i = 0, s1 = 0;
+    int cc[3] = {0, 0, 0}, cli[3] = {0, 0, 0}, ui[3] = {0, 0, 0};
+    double bli[3] = {0, 0, 0}, bhi_i[3] = {0, 0, 0}, ci[3] = {0, 0, 0}, ti[3] = {0, 0, 0};
+    double hi_ = 0.0, ri = 0.0;
+    if (e < E) {
+        const uint32_t cell = keys[e];
+        s1 = cend[cell];
+        cnt = s1 - (uint32_t)e - 1u;
+        i = vals[e];
+        cc[0] = (int)(cell % (uint32_t)g.nc[0]);
+        cc[1] = (int)((cell / (uint32_t)g.nc[0]) % (uint32_t)g.nc[1]);
+        cc[2] = (int)(cell / ((uint32_t)g.nc[0] * (uint32_t)g.nc[1]));
+        if (cnt) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 cli[k] = clo[3 * i + k];
@@ -454,73 +454,70 @@ __global__ __launch_bounds__(256) void k_sweep(RodsDev r, CfgDev c, Grid g,
                 ci[k] = r.center[3 * i + k];
                 ti[k] = r.dir[3 * i + k];
             }
-            const double hi_ = r.hcol[i], ri = r.rcol[i];
-            for (uint32_t pb0 = pa + 1; pb0 < s1; pb0 += 64) {
-                const uint32_t pb = pb0 + lane;
-                const bool valid = pb < s1;
-                bool pass = false;
-                uint32_t j = 0;
-                int m[3] = {0, 0, 0};
-                if (valid) {
-                    j = vals[pb];
-                    bool ok = true;
+            hi_ = r.hcol[i];
+            ri = r.rcol[i];
+        }
+    }
+    uint32_t maxc = cnt;
 #pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        const int clj = clo[3 * j + k];
-                        const int uj = c.pbc[k] ? clj + pmod_i(cc[k] - clj, g.nc[k]) : cc[k];
-                        ok = ok && (ui[k] == cli[k] || uj == clj);
-                        m[k] = c.pbc[k] ? (ui[k] - uj) / g.nc[k] : 0;
-                        const double sh = c.pbc[k] ? (double)m[k] * c.len[k] : 0.0;
-                        ok = ok && (blo[3 * j + k] + sh <= bhi_i[k]) && (bli[k] <= bhi[3 * j + k] + sh);
-                    }
-                    if (ok) {
-                        double cj[3];
+    for (int o = 32; o >= 1; o >>= 1) {
+        const uint32_t v = __shfl_xor(maxc, o, 64);
+        maxc = v > maxc ? v : maxc;
+    }
+    for (uint32_t t = 0; t < maxc; ++t) {
+        bool pass = false;
+        uint32_t j = 0;
+        int m[3] = {0, 0, 0};
+        if (t < cnt) {
+            j = vals[e + 1 + t];
+            bool ok = true;
 #pragma unroll
-                        for (int k = 0; k < 3; ++k)
-                            cj[k] = c.pbc[k] ? r.center[3 * j + k] + (double)m[k] * c.len[k] : r.center[3 * j + k];
-                        const SegRes sg = segseg(ci[0], ci[1], ci[2], ti[0], ti[1], ti[2], hi_, cj[0],
-                                                 cj[1], cj[2], r.dir[3 * j], r.dir[3 * j + 1],
-                                                 r.dir[3 * j + 2], r.hcol[j]);
-                        const double d0 = (sg.dist - ri) - r.rcol[j];
-                        pass = d0 < c.col_buf;
-                    }
+            for (int k = 0; k < 3; ++k) {
+                const int clj = clo[3 * j + k];
+                const int uj = c.pbc[k] ? clj + pmod_i(cc[k] - clj, g.nc[k]) : cc[k];
+                ok = ok && (ui[k] == cli[k] || uj == clj);
+                m[k] = c.pbc[k] ? (ui[k] - uj) / g.nc[k] : 0;
+                const double sh = c.pbc[k] ? (double)m[k] * c.len[k] : 0.0;
+                ok = ok && (blo[3 * j + k] + sh <= bhi_i[k]) && (bli[k] <= bhi[3 * j + k] + sh);
+            }
+            if (ok) {
+                double cj[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    cj[k] = c.pbc[k] ? r.center[3 * j + k] + (double)m[k] * c.len[k] : r.center[3 * j + k];
+                const SegRes sg = segseg(ci[0], ci[1], ci[2], ti[0], ti[1], ti[2], hi_, cj[0], cj[1], cj[2],
+                                         r.dir[3 * j], r.dir[3 * j + 1], r.dir[3 * j + 2], r.hcol[j]);
+                const double d0 = (sg.dist - ri) - r.rcol[j];
+                pass = d0 < c.col_buf;
+            }
+        }
+        const uint64_t mask = __ballot(pass);
+        if (mask) {
+            const int leader = __ffsll((unsigned long long)mask) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(pair_count, (uint32_t)__popcll(mask));
+            base = __shfl(base, leader, 64);
+            if (pass) {
+                const uint32_t slot = base + (uint32_t)__popcll(mask & lt);
+                if (slot < cap) {
+                    pI[slot] = i;
+                    pJ[slot] = j;
+                    pImg[slot] = img_pack(m[0], m[1], m[2]);
                 }
-                const uint64_t mask = __ballot(pass);
-                if (mask) {
-                    const int leader = __ffsll((unsigned long long)mask) - 1;
-                    uint32_t base = 0;
-                    if (lane == leader) {
-                        base = atomicAdd(pair_count, (uint32_t)__popcll(mask));
-                        atomicAdd(&cntI[i], (uint32_t)__popcll(mask));
-                    }
-                    base = __shfl(base, leader, 64);
-                    if (pass) {
-                        const uint32_t slot = base + (uint32_t)__popcll(mask & lt);
-                        if (slot < cap) {
-                            pI[slot] = i;
-                            pJ[slot] = j;
-                            pImg[slot] = img_pack(m[0], m[1], m[2]);
-                        }
-                        atomicAdd(&cntJ[j], 1u);
-                    }
-                }
+                atomicAdd(&cntI[i], 1u);
+                atomicAdd(&cntJ[j], 1u);
             }
         }
     }
 }
 
-HipErr launch_sweep(hipStream_t st, const RodsDev &r, const CfgDev &c, const Grid &g,
-                    const uint32_t *cstart, const uint32_t *cend, const uint32_t *vals,
-                    const int32_t *clo, const double *blo, const double *bhi, uint32_t *pair_count,
-                    uint32_t cap, uint32_t *pI, uint32_t *pJ, uint32_t *pImg, uint32_t *cntI,
-                    uint32_t *cntJ, int n_cu) {
-    if (g.ncell == 0) return hip_ok();
-    int64_t want = (g.ncell + 3) / 4;
-    int64_t lim = (int64_t)n_cu * 32;
-    if (want > lim) want = lim;
-    if (want < 1) want = 1;
-    k_sweep<<<(unsigned)want, 256, 0, st>>>(r, c, g, cstart, cend, vals, clo, blo, bhi, pair_count,
-                                            cap, pI, pJ, pImg, cntI, cntJ);
+HipErr launch_sweep(hipStream_t st, const RodsDev &r, const CfgDev &c, const Grid &g, const uint32_t *keys,
+                    int64_t E, const uint32_t *cend, const uint32_t *vals, const int32_t *clo, const double *blo,
+                    const double *bhi, uint32_t *pair_count, uint32_t cap, uint32_t *pI, uint32_t *pJ,
+                    uint32_t *pImg, uint32_t *cntI, uint32_t *cntJ) {
+    if (E == 0) return hip_ok();
+    k_sweep_entries<<<nblk(E, 256), 256, 0, st>>>(r, c, g, keys, E, cend, vals, clo, blo, bhi, pair_count, cap, pI,
+                                                  pJ, pImg, cntI, cntJ);
     LAUNCH_CHECK();
     return hip_ok();
 }
@@ -708,6 +705,78 @@ HipErr launch_adjacency(hipStream_t st, int32_t n, const uint32_t *rowStart,
                         uint32_t *adjStart, uint32_t *adjK, double *adjW) {
     k_adjacency<<<nblk((int64_t)n + 1, 256), 256, 0, st>>>(n, rowStart, jStart, jlist, geo, adjStart,
                                                           adjK, adjW);
+    LAUNCH_CHECK();
+    return hip_ok();
+}
+
+// ------------------------------------------------------------------------------------------------
+// Spatial reordering: key = Morton code of the rod centre's cell (10 bits per axis), stable sort,
+// then every per-rod array is gathered into the new order.
+// ------------------------------------------------------------------------------------------------
+__device__ inline uint32_t spread3(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ void k_morton_keys(int32_t n, const double *__restrict__ center, CfgDev c, Grid g,
+                              uint32_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t q[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        int a = (int)floor((center[3 * i + k] - c.box_lo[k]) / g.h[k]);
+        a = c.pbc[k] ? pmod_i(a, g.nc[k]) : (a < 0 ? 0 : (a >= g.nc[k] ? g.nc[k] - 1 : a));
+        q[k] = (uint32_t)a;
+    }
+    keys[i] = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
+    vals[i] = (uint32_t)i;
+}
+
+HipErr launch_morton_keys(hipStream_t st, int32_t n, const double *center, const CfgDev &c, const Grid &g,
+                          uint32_t *keys, uint32_t *vals) {
+    if (n == 0) return hip_ok();
+    k_morton_keys<<<nblk(n, 256), 256, 0, st>>>(n, center, c, g, keys, vals);
+    LAUNCH_CHECK();
+    return hip_ok();
+}
+
+__global__ void k_permute_rods(int32_t n, const uint32_t *__restrict__ perm, const int32_t *__restrict__ gid,
+                               const double *__restrict__ center, const double *__restrict__ dir,
+                               const double *__restrict__ length, const double *__restrict__ radius,
+                               const double *__restrict__ hcol, const double *__restrict__ rcol,
+                               const uint8_t *__restrict__ immov, int32_t *__restrict__ gid2,
+                               double *__restrict__ center2, double *__restrict__ dir2, double *__restrict__ length2,
+                               double *__restrict__ radius2, double *__restrict__ hcol2, double *__restrict__ rcol2,
+                               uint8_t *__restrict__ immov2) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = perm[i];
+    gid2[i] = gid[s];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        center2[3 * i + k] = center[3 * (size_t)s + k];
+        dir2[3 * i + k] = dir[3 * (size_t)s + k];
+    }
+    length2[i] = length[s];
+    radius2[i] = radius[s];
+    hcol2[i] = hcol[s];
+    rcol2[i] = rcol[s];
+    if (immov) immov2[i] = immov[s];
+}
+
+HipErr launch_permute_rods(hipStream_t st, int32_t n, const uint32_t *perm, const int32_t *gid,
+                           const double *center, const double *dir, const double *length, const double *radius,
+                           const double *hcol, const double *rcol, const uint8_t *immov, int32_t *gid2,
+                           double *center2, double *dir2, double *length2, double *radius2, double *hcol2,
+                           double *rcol2, uint8_t *immov2) {
+    if (n == 0) return hip_ok();
+    k_permute_rods<<<nblk(n, 256), 256, 0, st>>>(n, perm, gid, center, dir, length, radius, hcol, rcol, immov, gid2,
+                                                 center2, dir2, length2, radius2, hcol2, rcol2, immov2);
     LAUNCH_CHECK();
     return hip_ok();
 }

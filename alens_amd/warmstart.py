@@ -1,0 +1,127 @@
+"""Learned warm start: data collection from solver runs, GPU training, evaluation.
+
+The network maps 12 per-constraint inputs to an initial impulse gamma0; BBPGD always finishes the
+solve from it (and restarts cold if it raises the initial residual).  Everything here drives
+libalens.so through the C ABI: features and gamma come from real steps, training runs the library's
+MFMA forward/backward kernels and Adam on the GPU.
+
+  python -m alens_amd.warmstart --out alens_amd/data/warmstart_mlp.npz   (on a GPU)
+"""
+import argparse
+import os
+import time
+
+import numpy as np
+
+from . import capi, systems
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_MODEL = os.path.join(HERE, "data", "warmstart_mlp.npz")
+
+
+def collect(kind, n, steps, seed=1234, skip=2, sort=True):
+    """Run `steps` constraint steps (warm start from the previous gamma) and return the features and
+    converged gamma of every step after the first `skip` (initial overlap resolution)."""
+    rods, cfg = systems.scaled(kind, n, seed=seed) if kind != "small_3d" else systems.small_3d(seed=seed)
+    ctx = capi.Context(**dict(cfg, warm_start=1))
+    ctx.set_rods(rods)
+    if sort:
+        ctx.sort_rods()
+    Xs, gs = [], []
+    for s in range(steps):
+        ctx.step()
+        if s >= skip:
+            Xs.append(ctx.features())
+            gs.append(ctx.gamma().astype(np.float32))
+        ctx.euler()
+    ctx.close()
+    return np.concatenate(Xs), np.concatenate(gs)
+
+
+def train(X, gamma, epochs=12, batch=4096, lr=2e-3, seed=1):
+    ctx = capi.Context()
+    losses = ctx.mlp_train(X, gamma, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    P, norm = ctx.mlp_get()
+    ctx.close()
+    return P, norm, losses
+
+
+def evaluate(kind, n, steps, P, norm, seed=99, warmup=2, precision=0):
+    """Same rod states fed to three contexts (cold / previous gamma / network) at every step;
+    returns the per-step BBPGD iteration counts of each."""
+    rods, cfg = systems.scaled(kind, n, seed=seed) if kind != "small_3d" else systems.small_3d(seed=seed)
+    driver = capi.Context(**cfg)
+    driver.set_rods(rods)
+    perm = driver.sort_rods()
+    rods = {k: (v[perm] if isinstance(v, np.ndarray) else v) for k, v in rods.items()}
+    ctxs = {}
+    for name, ws in (("cold", 0), ("prev", 1), ("mlp", 2)):
+        c = capi.Context(**dict(cfg, warm_start=ws, precision=precision))
+        c.set_rods(rods)
+        if ws == 2:
+            c.mlp_set(P, norm)
+        ctxs[name] = c
+    out = {k: [] for k in ctxs}
+    out["fallback_mlp"] = 0
+    for s in range(warmup + steps):
+        c, t = driver.get_rods()
+        state = dict(rods, center=c, dir=t)
+        for name, ctx in ctxs.items():
+            ctx.set_rods(state)
+            ctx.set_step(s)
+            r = ctx.step()
+            if s >= warmup:
+                out[name].append(r["iters"] + r["iters_fp32"])
+                if name == "mlp":
+                    out["fallback_mlp"] += r["fallback_used"]
+        driver.set_step(s)
+        driver.step()
+        driver.euler()
+    for ctx in list(ctxs.values()) + [driver]:
+        ctx.close()
+    return {k: (np.array(v) if isinstance(v, list) else v) for k, v in out.items()}
+
+
+def save(path, P, norm, meta):
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    np.savez(path, params=np.asarray(P, np.float32), norm=np.asarray(norm, np.float32),
+             meta=np.array(repr(meta)))
+
+
+def load(path=DEFAULT_MODEL):
+    d = np.load(path, allow_pickle=False)
+    return d["params"], d["norm"]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=DEFAULT_MODEL)
+    ap.add_argument("--pairs", type=int, default=2_000_000, help="training pairs to collect")
+    ap.add_argument("--epochs", type=int, default=12)
+    args = ap.parse_args()
+    t0 = time.time()
+    Xs, gs = [], []
+    # the small 3-D system (many short runs) and the dense monolayer
+    X1, g1 = collect("small_3d", 1000, 60, seed=11, skip=2)
+    Xs.append(X1)
+    gs.append(g1)
+    have = len(g1)
+    steps = 2
+    while have < args.pairs:
+        X2, g2 = collect("monolayer_slab", 200_000, steps + 2, seed=1000 + steps, skip=2)
+        Xs.append(X2)
+        gs.append(g2)
+        have += len(g2)
+        steps += 1
+    X = np.concatenate(Xs)[: args.pairs]
+    g = np.concatenate(gs)[: args.pairs]
+    print(f"collected {len(g)} pairs in {time.time() - t0:.1f} s", flush=True)
+    P, norm, losses = train(X, g, epochs=args.epochs)
+    print("loss per epoch", np.round(losses, 5).tolist(), flush=True)
+    save(args.out, P, norm, dict(pairs=len(g), epochs=args.epochs, losses=[float(v) for v in losses]))
+    ev = evaluate("monolayer_slab", 20000, 8, P, norm)
+    print({k: (v.mean() if hasattr(v, "mean") else v) for k, v in ev.items()}, flush=True)
+
+
+if __name__ == "__main__":
+    main()

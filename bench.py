@@ -66,6 +66,9 @@ def main():
     n = len(rods["length"])
     ctx = capi.Context(**cfg)
     ctx.set_rods(rods)
+    # spatial (Morton cell) order, as the reference's FDPS containers keep it; later arrays follow it
+    perm = ctx.sort_rods()
+    rods = {k: (v[perm] if isinstance(v, np.ndarray) else v) for k, v in rods.items()}
 
     def one_step():
         rep = ctx.step()
@@ -137,6 +140,7 @@ def main():
     if not args.no_form:
         fctx = capi.Context(**dict(cfg, precision=1, warm_start=1))
         fctx.set_rods(rods)
+        fctx.sort_rods()
         for w in range(args.warmup):
             fctx.step()
             fctx.euler()

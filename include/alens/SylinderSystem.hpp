@@ -1,0 +1,170 @@
+// SylinderSystem.hpp -- C++ call surface of the reference's rod engine over libalens.so's C ABI.
+//
+// The reference calls into SimToolbox's SylinderSystem / ConstraintCollector / BCQPSolver from
+// TubuleSystem (SRC/TubuleSystem.cpp:79, 120, 223, 55, 697-742).  These header-only classes keep those
+// names and call shapes so that code written against them moves onto the GPU engine unchanged:
+//   SylinderSystem::initialize / prepareStep / runStep / stepEuler / getConstraintPoolNonConst /
+//   getStepCount, the per-thread ConstraintBlockPool (pool.size() == thread count, one queue per
+//   thread), ConstraintBlock's 16-argument constructor and setStress, and BCQPSolver::solveBBPGD.
+// Errors surface as alens::Error exceptions carrying the alens_status and message.
+#pragma once
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../alens_capi.h"
+
+namespace alens {
+
+struct Error : std::runtime_error {
+    alens_status status;
+    Error(alens_status s, const std::string &m) : std::runtime_error(m), status(s) {}
+};
+
+inline void check(const alens_ctx *ctx, alens_status s, const char *what) {
+    if (s != ALENS_OK) throw Error(s, std::string(what) + ": " + alens_last_error(ctx));
+}
+
+// Field order of the reference's ConstraintBlock constructor (SRC/TubuleSystem.cpp:727-736).
+struct ConstraintBlock {
+    double delta0 = 0, gamma = 0;
+    int gidI = 0, gidJ = 0, globalIndexI = 0, globalIndexJ = 0;
+    double normI[3] = {}, normJ[3] = {}, posI[3] = {}, posJ[3] = {}, labI[3] = {}, labJ[3] = {};
+    bool oneSide = false, bilateral = false;
+    double kappa = 0, gammaLB = 0;
+    double stress[9] = {};
+
+    ConstraintBlock() = default;
+    ConstraintBlock(double delta0_, double gamma_, int gidI_, int gidJ_, int globalIndexI_, int globalIndexJ_,
+                    const double normI_[3], const double normJ_[3], const double posI_[3], const double posJ_[3],
+                    const double labI_[3], const double labJ_[3], bool oneSide_, bool bilateral_, double kappa_,
+                    double gammaLB_)
+        : delta0(delta0_), gamma(gamma_), gidI(gidI_), gidJ(gidJ_), globalIndexI(globalIndexI_),
+          globalIndexJ(globalIndexJ_), oneSide(oneSide_), bilateral(bilateral_), kappa(kappa_), gammaLB(gammaLB_) {
+        std::memcpy(normI, normI_, sizeof normI);
+        std::memcpy(normJ, normJ_, sizeof normJ);
+        std::memcpy(posI, posI_, sizeof posI);
+        std::memcpy(posJ, posJ_, sizeof posJ);
+        std::memcpy(labI, labI_, sizeof labI);
+        std::memcpy(labJ, labJ_, sizeof labJ);
+    }
+    // row-major 3 x 3 (Emat3 in the reference, SRC/TubuleSystem.cpp:742)
+    void setStress(const double s[9]) { std::memcpy(stress, s, sizeof stress); }
+
+    static ConstraintBlock fromC(const alens_con_block &c) {
+        ConstraintBlock b(c.delta0, c.gamma, c.gid_i, c.gid_j, c.gidx_i, c.gidx_j, c.norm_i, c.norm_j, c.pos_i,
+                          c.pos_j, c.lab_i, c.lab_j, c.one_side != 0, c.bilateral != 0, c.kappa, c.gamma_lb);
+        b.setStress(c.stress);
+        return b;
+    }
+};
+
+using ConstraintBlockQue = std::vector<ConstraintBlock>;
+using ConstraintBlockPool = std::vector<ConstraintBlockQue>;
+
+struct Rods {
+    std::vector<int32_t> gid;
+    std::vector<double> center, direction, length, radius;  // center/direction: 3 per rod
+    std::vector<uint8_t> immovable;                          // empty = all movable
+    int32_t size() const { return (int32_t)gid.size(); }
+};
+
+class SylinderSystem {
+  public:
+    SylinderSystem(const alens_config &cfg, int nThreads = 1) : pool_((size_t)(nThreads > 0 ? nThreads : 1)) {
+        alens_status s = alens_create(&cfg, &ctx_);
+        if (s != ALENS_OK) throw Error(s, "alens_create failed");
+    }
+    ~SylinderSystem() { alens_destroy(ctx_); }
+    SylinderSystem(const SylinderSystem &) = delete;
+    SylinderSystem &operator=(const SylinderSystem &) = delete;
+
+    // initialize(): load rods; spatialSort reorders them the way FDPS keeps its particle arrays
+    void initialize(const Rods &r, bool spatialSort = true) {
+        n_ = r.size();
+        check(ctx_, alens_set_rods(ctx_, n_, r.gid.data(), r.center.data(), r.direction.data(), r.length.data(),
+                                   r.radius.data(), r.immovable.empty() ? nullptr : r.immovable.data()),
+              "set_rods");
+        if (spatialSort) check(ctx_, alens_sort_rods(ctx_, nullptr), "sort_rods");
+    }
+    // prepareStep(): clears the constraint pool for this step's external blocks
+    void prepareStep() {
+        for (auto &q : pool_) q.clear();
+    }
+    // runStep(): Brownian + non-constraint velocity, collision collection, BBPGD, write-back
+    void runStep(const double *forceTorque = nullptr) {
+        for (const auto &q : pool_)
+            if (!q.empty()) throw Error(ALENS_E_ARG, "runStep: external constraint blocks are not supported yet");
+        alens_status s = alens_step_constraints(ctx_, forceTorque, &report_);
+        if (s != ALENS_OK && s != ALENS_E_NOCONV) check(ctx_, s, "step_constraints");
+    }
+    void stepEuler() { check(ctx_, alens_step_euler(ctx_), "step_euler"); }
+
+    ConstraintBlockPool &getConstraintPoolNonConst() { return pool_; }
+    int64_t getStepCount() const { return alens_get_step(ctx_); }
+    const alens_solve_report &getReport() const { return report_; }
+    alens_ctx *handle() { return ctx_; }
+
+    // this step's collision constraints with their impulses, in canonical order
+    std::vector<ConstraintBlock> getCollisionBlocks() {
+        int64_t m = 0;
+        check(ctx_, alens_get_num_constraints(ctx_, &m), "get_num_constraints");
+        std::vector<alens_con_block> raw((size_t)m);
+        check(ctx_, alens_get_constraints(ctx_, m, raw.data(), &m), "get_constraints");
+        std::vector<ConstraintBlock> out;
+        out.reserve(raw.size());
+        for (const auto &c : raw) out.push_back(ConstraintBlock::fromC(c));
+        return out;
+    }
+    // velocity (6 per rod: v, omega) after the step
+    std::vector<double> getVelocity() {
+        std::vector<double> u(6 * (size_t)n_);
+        check(ctx_, alens_get_velocity(ctx_, u.data(), nullptr, nullptr, nullptr), "get_velocity");
+        return u;
+    }
+    void getPositions(std::vector<double> &center, std::vector<double> &direction) {
+        center.resize(3 * (size_t)n_);
+        direction.resize(3 * (size_t)n_);
+        check(ctx_, alens_get_rods(ctx_, center.data(), direction.data()), "get_rods");
+    }
+
+  private:
+    alens_ctx *ctx_ = nullptr;
+    int32_t n_ = 0;
+    ConstraintBlockPool pool_;
+    alens_solve_report report_{};
+};
+
+// BCQPSolver: min 1/2 x^T A x + b^T x, x >= 0, A = dt D^T M D given by constraint rows (normal and the
+// two moment arms posI x n, posJ x n) and per-rod mobility {t, 1/zperp, 1/zpar - 1/zperp, 1/zrot}.
+class BCQPSolver {
+  public:
+    BCQPSolver(alens_ctx *ctx, int32_t nRods, std::vector<int32_t> I, std::vector<int32_t> J,
+               std::vector<double> normal, std::vector<double> armI, std::vector<double> armJ, std::vector<double> b,
+               std::vector<double> mobility)
+        : ctx_(ctx), n_(nRods), I_(std::move(I)), J_(std::move(J)), nrm_(std::move(normal)), rI_(std::move(armI)),
+          rJ_(std::move(armJ)), b_(std::move(b)), mob_(std::move(mobility)) {}
+    // returns the iteration count; xsol holds the start point on entry (empty = cold)
+    int solveBBPGD(std::vector<double> &xsol, alens_solve_report *history = nullptr) {
+        const int64_t m = (int64_t)I_.size();
+        std::vector<double> out((size_t)m);
+        alens_solve_report rep{};
+        alens_status s = alens_bcqp_solve(ctx_, n_, m, I_.data(), J_.data(), nrm_.data(), rI_.data(), rJ_.data(),
+                                          b_.data(), mob_.data(), xsol.size() == (size_t)m ? xsol.data() : nullptr,
+                                          out.data(), &rep);
+        if (s != ALENS_OK && s != ALENS_E_NOCONV) check(ctx_, s, "bcqp_solve");
+        xsol.swap(out);
+        if (history) *history = rep;
+        return rep.iters;
+    }
+
+  private:
+    alens_ctx *ctx_;
+    int32_t n_;
+    std::vector<int32_t> I_, J_;
+    std::vector<double> nrm_, rI_, rJ_, b_, mob_;
+};
+
+}  // namespace alens

@@ -51,7 +51,8 @@ typedef struct {
     int32_t max_ite;             /* conMaxIte */
     int32_t solver;              /* conSolverChoice; 0 = BBPGD (the only solver provided) */
     int32_t precision;           /* 0 = FP64; 1 = FP32 iterations + FP64 residual and refinement */
-    int32_t warm_start;          /* 0 = cold; 1 = previous-step gamma matched by (gid_i, gid_j) */
+    int32_t warm_start;          /* 0 = cold; 1 = previous-step gamma matched by rod pair;
+                                    2 = learned network (alens_mlp_*) fed with the previous gamma */
     uint64_t seed;               /* rngSeed */
     int32_t device;              /* HIP device ordinal */
     int32_t reserved;
@@ -96,6 +97,13 @@ alens_status alens_set_rods(alens_ctx *ctx, int32_t n, const int32_t *gid, const
                             const uint8_t *immovable);
 alens_status alens_get_rods(alens_ctx *ctx, double *center, double *dir);
 
+/* Reorder the rods on the device by the Morton code of their centre's cell so that neighbours sit
+ * close in memory (the reference keeps its particle arrays in FDPS tree order for the same reason;
+ * repartitioning happens in prepareStep, SRC/TubuleSystem.cpp:120-125).  perm_out (n, may be NULL)
+ * receives, for each new index i, the rod's previous index.  Rod and constraint indices refer to the
+ * new order afterwards; the previous-step impulse cache is cleared. */
+alens_status alens_sort_rods(alens_ctx *ctx, int32_t *perm_out);
+
 /* Step counter keying the Brownian stream (SylinderSystem's stepCount, SRC/TubuleSystem.cpp:114). */
 alens_status alens_set_step(alens_ctx *ctx, int64_t step);
 int64_t alens_get_step(const alens_ctx *ctx);
@@ -129,6 +137,29 @@ alens_status alens_bcqp_solve(alens_ctx *ctx, int32_t n, int64_t m, const int32_
                               const int32_t *J, const double *nrm, const double *r_i,
                               const double *r_j, const double *b, const double *mob,
                               const double *x0, double *gamma_out, alens_solve_report *rep);
+
+/* ---- learned warm start (no counterpart in the reference: the reference starts its BBPGD from the
+ * constraint blocks' gamma guess, SRC/TubuleSystem.cpp:722,728) ----
+ * Network 12 -> 64 -> 64 -> 64 -> 1, SiLU hidden layers, softplus output, BF16 MFMA with FP32
+ * accumulation.  params: ALENS_MLP_PARAMS floats (W1 64x16 row-major with input columns 12..15 zero,
+ * b1 64, W2 64x64, b2 64, W3 64x64, b3 64, w4 64, b4); norm: ALENS_MLP_NORM floats (feature mean[12],
+ * 1/std[12], gamma scale).  Features X are 16 floats per constraint (12 used): delta0, normal(3),
+ * s_I/(L_I/2), s_J/(L_J/2), dt * relative contact velocity of U_nc (3), L_I, L_J, previous gamma. */
+#define ALENS_MLP_PARAMS 9473
+#define ALENS_MLP_NORM 25
+alens_status alens_mlp_set(alens_ctx *ctx, const float *params, const float *norm);
+alens_status alens_mlp_get(alens_ctx *ctx, float *params, float *norm);
+/* features of the last step's constraints (16 m floats) */
+alens_status alens_get_features(alens_ctx *ctx, float *X);
+/* gamma0 = gamma_scale * network(X) for m feature rows */
+alens_status alens_mlp_predict(alens_ctx *ctx, int64_t m, const float *X, float *gamma0);
+/* full-batch gradient of mean((network(X)/gamma_scale - target)^2); m a multiple of 16 */
+alens_status alens_mlp_gradient(alens_ctx *ctx, int64_t m, const float *X, const float *target, float *grad,
+                                float *loss);
+/* Adam training on the GPU on (X, gamma) pairs; sets the normalisation from the data and initialises
+ * the weights when no network is loaded.  batch is a multiple of 16; loss_per_epoch may be NULL. */
+alens_status alens_mlp_train(alens_ctx *ctx, int64_t m, const float *X, const float *gamma, int32_t epochs,
+                             int32_t batch, float lr, uint64_t seed, float *loss_per_epoch);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,78 @@
+"""GPU checks of the learned warm start: the fused BF16-MFMA inference kernel and the MFMA training
+gradient against the FP64 oracle (with bfloat16 operand rounding emulated), GPU training progress, and
+the iteration count the trained network saves."""
+import numpy as np
+import pytest
+
+from alens_amd import capi, systems
+from oracle import mlp_oracle as MO
+
+pytestmark = pytest.mark.gpu
+
+
+def _data(B, seed):
+    rng = np.random.default_rng(seed)
+    X = np.zeros((B, 16), np.float32)
+    X[:, :12] = rng.normal(size=(B, 12)).astype(np.float32)
+    norm = np.concatenate([rng.normal(size=12) * 0.1, 1.0 + rng.random(12), [0.7]]).astype(np.float32)
+    return X, norm
+
+
+def test_inference_kernel_matches_oracle():
+    X, norm = _data(1000, 1)  # not a multiple of 16: tail tile
+    P = MO.init_params(5)
+    ctx = capi.Context()
+    ctx.mlp_set(P, norm)
+    got = ctx.mlp_predict(X)
+    ref = MO.predict(P, norm, X, bf=True)
+    assert np.max(np.abs(got - ref)) <= 2e-4 * np.max(np.abs(ref)), np.max(np.abs(got - ref))
+    # and the plain FP64 network is close (BF16 operands)
+    ref64 = MO.predict(P, norm, X, bf=False)
+    assert np.max(np.abs(got - ref64)) <= 5e-2 * np.max(np.abs(ref64))
+
+
+def test_training_gradient_matches_oracle():
+    X, norm = _data(2048, 2)
+    P = MO.init_params(6)
+    t = np.random.default_rng(3).random(2048).astype(np.float32)
+    ctx = capi.Context()
+    ctx.mlp_set(P, norm)
+    g, loss = ctx.mlp_gradient(X, t)
+    rl, rg = MO.loss_and_grad(P, norm, X, t, bf=True)
+    assert abs(loss - rl) <= 1e-4 * rl
+    blocks = [(MO.PW1, MO.PB1), (MO.PB1, MO.PW2), (MO.PW2, MO.PB2), (MO.PB2, MO.PW3), (MO.PW3, MO.PB3),
+              (MO.PB3, MO.PW4), (MO.PW4, MO.PB4), (MO.PB4, MO.NPARAM)]
+    for a, b in blocks:
+        err = np.linalg.norm(g[a:b] - rg[a:b]) / max(np.linalg.norm(rg[a:b]), 1e-30)
+        assert err <= 2e-3, (a, b, err)
+
+
+def test_gpu_training_fits_a_smooth_target():
+    rng = np.random.default_rng(4)
+    B = 32768
+    X = np.zeros((B, 16), np.float32)
+    X[:, :12] = rng.normal(size=(B, 12))
+    gam = np.log1p(np.exp(X[:, 0] - 0.5 * X[:, 3] + 0.3 * X[:, 7] * X[:, 2])).astype(np.float32)
+    ctx = capi.Context()
+    losses = ctx.mlp_train(X, gam, epochs=15, batch=2048, lr=3e-3, seed=7)
+    assert losses[-1] < 0.1 * losses[0], losses
+    P, norm = ctx.mlp_get()
+    pred = ctx.mlp_predict(X[:4096])
+    assert np.allclose(pred, MO.predict(P, norm, X[:4096], bf=True), rtol=5e-4, atol=5e-4 * gam.max())
+
+
+def test_learned_warm_start_reduces_iterations():
+    """Train on solver output of one monolayer run, evaluate on another (same rod states fed to cold,
+    previous-gamma and network warm starts).  The network must beat the cold start; the measured
+    ratios are printed."""
+    from alens_amd import warmstart as W
+
+    X, g = W.collect("monolayer_slab", 20000, 14, seed=21, skip=2)
+    assert len(g) > 200_000
+    P, norm, losses = W.train(X, g, epochs=8, batch=4096, lr=2e-3, seed=3)
+    assert losses[-1] < losses[0]
+    ev = W.evaluate("monolayer_slab", 20000, 6, P, norm, seed=77, warmup=2)
+    cold, prev, mlp = ev["cold"].mean(), ev["prev"].mean(), ev["mlp"].mean()
+    print(f"BBPGD iterations/step: cold {cold:.1f}, previous gamma {prev:.1f}, network {mlp:.1f} "
+          f"(cold/network = {cold / mlp:.2f}), fallbacks {ev['fallback_mlp']}")
+    assert mlp < cold

@@ -267,3 +267,27 @@ def test_precision_form_meets_fp64_residual(kind, n):
     print(f"{kind}: gamma rel err vs tight FP64 solve: precision form {err_form:.2e}, plain FP64 {err_plain:.2e}; "
           f"iters fp32 {rep['iters_fp32']} + fp64 {rep['iters']} vs plain {o['report']['iters']}")
     assert err_form <= max(20 * err_plain, 1e-6)
+
+
+def test_sort_rods_spatial_order_bitexact():
+    """After the Morton reorder the step is still bit-identical to the oracle run on the permuted rods."""
+    rods, cfg = systems.scaled("lognormal_3d", 20000, seed=9)
+    ctx = capi.Context(**cfg)
+    ctx.set_rods(rods)
+    perm = ctx.sort_rods()
+    assert np.array_equal(np.sort(perm), np.arange(20000))
+    pr = {k: (v[perm] if isinstance(v, np.ndarray) else v) for k, v in rods.items()}
+    c, t = ctx.get_rods()
+    assert_bits(c, pr["center"], "permuted centers")
+    # neighbours are close in memory: mean index distance of contact pairs drops sharply
+    rep = ctx.step()
+    g = ctx.constraint_arrays()
+    g["gamma"] = ctx.gamma()
+    g.update(ctx.velocity())
+    g["report"] = rep
+    o = O.step(pr, O.cfg_from_dict(cfg))
+    compare_full(pr, cfg, g, o)
+    blocks = ctx.constraint_blocks()
+    assert all(b.gid_i == pr["gid"][b.gidx_i] for b in blocks[:100])
+    o_unsorted = O.pairs(rods, O.cfg_from_dict(cfg))
+    assert np.mean(np.abs(g["J"] - g["I"])) < 0.25 * np.mean(np.abs(o_unsorted[1] - o_unsorted[0]))
