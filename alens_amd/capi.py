@@ -20,7 +20,7 @@ STATUS_NAMES = {0: "OK", 1: "E_ARG", 2: "E_HIP", 3: "E_OOM", 4: "E_NONFINITE", 5
 # every entry point declared in include/alens_capi.h
 EXPORTS = [
     "alens_create", "alens_destroy", "alens_last_error", "alens_version", "alens_set_rods", "alens_get_rods",
-    "alens_sort_rods",
+    "alens_sort_rods", "alens_set_systems", "alens_get_num_systems", "alens_get_system_reports",
     "alens_set_step", "alens_get_step", "alens_step_constraints", "alens_step_euler",
     "alens_get_num_constraints", "alens_get_constraints", "alens_get_constraint_arrays", "alens_get_gamma",
     "alens_get_velocity", "alens_bcqp_solve",
@@ -53,6 +53,11 @@ class Config(ctypes.Structure):
         ("device", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
     ]
+
+
+class Box(ctypes.Structure):
+    _fields_ = [("box_lo", ctypes.c_double * 3), ("box_hi", ctypes.c_double * 3), ("pbc", ctypes.c_int32 * 3),
+                ("monolayer", ctypes.c_int32)]
 
 
 class ConBlock(ctypes.Structure):
@@ -109,6 +114,10 @@ def lib():
     L.alens_set_rods.argtypes = [vp, ctypes.c_int32, _ip, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_uint8)]
     L.alens_get_rods.argtypes = [vp, _dp, _dp]
     L.alens_sort_rods.argtypes = [vp, _ip]
+    L.alens_set_systems.argtypes = [vp, ctypes.c_int32, _ip, ctypes.POINTER(Box), _ip, _dp, _dp, _dp, _dp,
+                                    ctypes.POINTER(ctypes.c_uint8)]
+    L.alens_get_num_systems.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
+    L.alens_get_system_reports.argtypes = [vp, ctypes.POINTER(Report)]
     L.alens_set_step.argtypes = [vp, ctypes.c_int64]
     L.alens_get_step.argtypes = [vp]
     L.alens_get_step.restype = ctypes.c_int64
@@ -214,6 +223,35 @@ class Context:
                                          _p(t, ctypes.c_double), _p(L, ctypes.c_double), _p(r, ctypes.c_double),
                                          _p(imm, ctypes.c_uint8)))
         self.n = n
+
+    def set_systems(self, systems_list):
+        """systems_list: [(rods, box_cfg)] with box_cfg holding box_lo, box_hi, pbc, monolayer."""
+        parts = [r for r, _ in systems_list]
+        off = np.zeros(len(parts) + 1, np.int32)
+        off[1:] = np.cumsum([len(r["length"]) for r in parts])
+        cat = lambda k, dt: np.ascontiguousarray(np.concatenate([np.asarray(r[k], dt) for r in parts]), dt)
+        gid, L, rad = cat("gid", np.int32), cat("length", np.float64), cat("radius", np.float64)
+        c = np.ascontiguousarray(np.concatenate([np.asarray(r["center"]).reshape(-1, 3) for r in parts]))
+        t = np.ascontiguousarray(np.concatenate([np.asarray(r["dir"]).reshape(-1, 3) for r in parts]))
+        boxes = (Box * len(parts))()
+        for b, (_, bc) in zip(boxes, systems_list):
+            b.box_lo[:] = [float(v) for v in bc["box_lo"]]
+            b.box_hi[:] = [float(v) for v in bc["box_hi"]]
+            b.pbc[:] = [int(v) for v in bc["pbc"]]
+            b.monolayer = int(bc.get("monolayer", 0))
+        self._check(lib().alens_set_systems(self._h, len(parts), _p(off, ctypes.c_int32), boxes,
+                                            _p(gid, ctypes.c_int32), _p(c, ctypes.c_double), _p(t, ctypes.c_double),
+                                            _p(L, ctypes.c_double), _p(rad, ctypes.c_double), None))
+        self.n = int(off[-1])
+        self.rod_offset = off
+        return off
+
+    def system_reports(self):
+        ns = ctypes.c_int32()
+        self._check(lib().alens_get_num_systems(self._h, ctypes.byref(ns)))
+        arr = (Report * max(ns.value, 1))()
+        self._check(lib().alens_get_system_reports(self._h, arr))
+        return [arr[i].as_dict() for i in range(ns.value)]
 
     def get_rods(self):
         c = np.zeros((self.n, 3))

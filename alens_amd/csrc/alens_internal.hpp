@@ -61,8 +61,20 @@ struct Grid {
     int keybits;
 };
 
+// One independent rod system of a context (a context holds 1 or many): its box and cell grid.
+struct SysDev {
+    double box_lo[3], len[3], h[3];
+    int32_t pbc[3], nc[3];
+    int32_t mono;
+    uint32_t cell_off;  // first global cell id of this system
+    int32_t rod_off, rod_end;
+};
+constexpr int kMaxSys = 256;
+
 struct RodsDev {
     int32_t n;
+    const int32_t *rsys;  // system of each rod
+    const SysDev *sys;
     const int32_t *gid;
     const double *center;  // 3n
     const double *dir;     // 3n
@@ -85,27 +97,25 @@ struct ScanScratch {
 };
 
 // collide kernels (k_collide.hip)
-HipErr launch_mob_unc(hipStream_t st, int32_t n, const int32_t *gid, const double *dir,
-                      const double *length, const double *radius, const uint8_t *immovable,
-                      const double *fext, const CfgDev &c, double *mob, double *unc);
-HipErr launch_aabb(hipStream_t st, const RodsDev &r, const CfgDev &c, const Grid &g, int32_t *clo,
-                   int32_t *chi, double *blo, double *bhi, uint32_t *ecnt);
+HipErr launch_mob_unc(hipStream_t st, const RodsDev &r, const double *length, const double *radius,
+                      const uint8_t *immovable, const double *fext, const CfgDev &c, double *mob, double *unc);
+HipErr launch_aabb(hipStream_t st, const RodsDev &r, const CfgDev &c, int32_t *clo, int32_t *chi, double *blo,
+                   double *bhi, uint32_t *ecnt);
 HipErr exclusive_scan_u32(hipStream_t st, const uint32_t *in, uint32_t *out, int64_t n,
                           ScanScratch &s);
-HipErr launch_fill_entries(hipStream_t st, int32_t n, const int32_t *clo, const int32_t *chi,
-                           const uint32_t *eoff, const Grid &g, uint32_t *keys, uint32_t *vals);
+HipErr launch_fill_entries(hipStream_t st, const RodsDev &r, const int32_t *clo, const int32_t *chi,
+                           const uint32_t *eoff, uint32_t *keys, uint32_t *vals);
 HipErr radix_sort_pairs(hipStream_t st, uint32_t *keys, uint32_t *vals, uint32_t *keys_alt,
                         uint32_t *vals_alt, int64_t n, int keybits, DBuf<uint32_t> &hist,
                         ScanScratch &s, bool &result_in_alt);
 HipErr launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, int64_t n);
 HipErr launch_cell_ranges(hipStream_t st, const uint32_t *keys, int64_t E, uint32_t *cstart,
                           uint32_t *cend);
-HipErr launch_sweep(hipStream_t st, const RodsDev &r, const CfgDev &c, const Grid &g, const uint32_t *keys,
-                    int64_t E, const uint32_t *cend, const uint32_t *vals, const int32_t *clo, const double *blo,
+HipErr launch_sweep(hipStream_t st, const RodsDev &r, const CfgDev &c, const uint32_t *keys, int64_t E, const uint32_t *cend, const uint32_t *vals, const int32_t *clo, const double *blo,
                     const double *bhi, uint32_t *pair_count, uint32_t cap, uint32_t *pI, uint32_t *pJ,
                     uint32_t *pImg, uint32_t *cntI, uint32_t *cntJ);
-HipErr launch_morton_keys(hipStream_t st, int32_t n, const double *center, const CfgDev &c, const Grid &g,
-                          uint32_t *keys, uint32_t *vals);
+HipErr launch_morton_keys(hipStream_t st, const RodsDev &r, int32_t i0, int32_t n, uint32_t *keys,
+                          uint32_t *vals);
 HipErr launch_permute_rods(hipStream_t st, int32_t n, const uint32_t *perm, const int32_t *gid,
                            const double *center, const double *dir, const double *length, const double *radius,
                            const double *hcol, const double *rcol, const uint8_t *immov, int32_t *gid2,
@@ -155,10 +165,15 @@ struct SolveArgsT {
     T *U;                        // 6n scratch
     double *part;                // kRedQ * nchunks partials
     uint32_t *bar;               // barrier words (zeroed before launch)
-    double *out;                 // report words
+    double *out;                 // per-system report words (8 per system)
+    const int32_t *rsys;         // system of each rod
+    const int32_t *smono;        // monolayer flag per system
+    const uint32_t *con_off;     // constraint range per system (nsys + 1)
+    const uint32_t *chunk_off;   // reduction-chunk range per system (nsys + 1)
+    const int32_t *chunk_sys;    // system of each chunk
     double dt, tol;
     int32_t max_ite;
-    int32_t mono;
+    int32_t nsys;
     int32_t warm;
     int32_t nchunks;
 };
@@ -168,12 +183,11 @@ HipErr launch_bbpgd32(hipStream_t st, SolveArgsT<float> a, int max_blocks, int *
 HipErr launch_to_f32(hipStream_t st, const double *src, float *dst, size_t count);
 HipErr launch_xg32_to_x64(hipStream_t st, const float2 *xg32, double *x64, uint32_t m);
 HipErr launch_init_xg32(hipStream_t st, float2 *xg, const double2 *xg64, uint32_t m);
-HipErr launch_writeback(hipStream_t st, int32_t n, const uint32_t *adjStart, const uint32_t *adjK,
-                        const double *adjW, const double2 *xg, const double *mob,
-                        const double *unc, int mono, double *F, double *U, double *gamma,
-                        uint32_t m);
-HipErr launch_euler(hipStream_t st, int32_t n, double *center, double *dir, const double *u,
-                    const CfgDev &c);
+HipErr launch_writeback(hipStream_t st, int32_t n, const int32_t *rsys, const SysDev *sys, const uint32_t *adjStart,
+                        const uint32_t *adjK, const double *adjW, const double2 *xg, const double *mob,
+                        const double *unc, double *F, double *U, double *gamma, uint32_t m);
+HipErr launch_euler(hipStream_t st, int32_t n, const int32_t *rsys, const SysDev *sys, double *center, double *dir,
+                    const double *u, const CfgDev &c);
 HipErr launch_init_xg(hipStream_t st, double2 *xg, const double *x0, uint32_t m);
 HipErr launch_warm_match(hipStream_t st, uint32_t m, const int32_t *cI, const int32_t *cJ,
                          const int32_t *gid, uint32_t mprev, const uint64_t *prev_key,

@@ -25,8 +25,18 @@ struct alens_ctx {
     int32_t n = 0;
     bool have_rods = false;
     bool have_step = false;
-    Grid grid{};
     int grid_used = 0;
+    // independent systems (1 for alens_set_rods): boxes, rod ranges, cell grids
+    int32_t nsys = 0;
+    std::vector<alens_box> boxes;
+    std::vector<int32_t> rod_off;
+    std::vector<SysDev> sysh;
+    int64_t ncell = 0;
+    int keybits = 1;
+    DBuf<SysDev> sysd;
+    DBuf<int32_t> rsys, smono, chunk_sys;
+    DBuf<uint32_t> con_off, chunk_off;
+    std::vector<alens_solve_report> sysrep;
 
     std::vector<int32_t> h_gid;
     DBuf<int32_t> gid;
@@ -66,6 +76,7 @@ struct alens_ctx {
     bool have_prev = false;
 
     int64_t m = 0, E = 0;
+    int32_t nchunks_cur = 0, nsys_cur = 1;
     hipEvent_t ev[6] = {};
 };
 
@@ -117,6 +128,8 @@ CfgDev make_cfgdev(const alens_ctx *ctx) {
 RodsDev make_rods(const alens_ctx *ctx) {
     RodsDev r{};
     r.n = ctx->n;
+    r.rsys = ctx->rsys.p;
+    r.sys = ctx->sysd.p;
     r.gid = ctx->gid.p;
     r.center = ctx->center.p;
     r.dir = ctx->dir.p;
@@ -202,16 +215,22 @@ void alens_destroy(alens_ctx *ctx) {
     rel(ctx->out); rel(ctx->bar); rel(ctx->pkey); rel(ctx->pgam);
     rel(ctx->geo32); rel(ctx->adjW32); rel(ctx->mob32); rel(ctx->b32); rel(ctx->U32); rel(ctx->xg32);
     rel(ctx->mlpP); rel(ctx->mlpNorm); rel(ctx->feat); rel(ctx->xprev);
+    rel(ctx->sysd); rel(ctx->rsys); rel(ctx->smono); rel(ctx->chunk_sys); rel(ctx->con_off); rel(ctx->chunk_off);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->st) (void)hipStreamDestroy(ctx->st);
     delete ctx;
 }
 
-alens_status alens_set_rods(alens_ctx *ctx, int32_t n, const int32_t *gid, const double *center,
-                            const double *dir, const double *length, const double *radius,
-                            const uint8_t *immovable) {
+alens_status alens_set_systems(alens_ctx *ctx, int32_t nsys, const int32_t *rod_offset, const alens_box *boxes,
+                               const int32_t *gid, const double *center, const double *dir, const double *length,
+                               const double *radius, const uint8_t *immovable) {
     if (!ctx) return ALENS_E_ARG;
+    if (nsys < 1 || nsys > kMaxSys || !rod_offset || !boxes || rod_offset[0] != 0)
+        return fail(ctx, ALENS_E_ARG, "set_systems: need 1..256 systems and rod_offset[0] = 0");
+    for (int32_t s = 0; s < nsys; ++s)
+        if (rod_offset[s + 1] < rod_offset[s]) return fail(ctx, ALENS_E_ARG, "set_systems: rod_offset decreases");
+    const int32_t n = rod_offset[nsys];
     if (n < 0 || (n > 0 && (!gid || !center || !dir || !length || !radius)))
         return fail(ctx, ALENS_E_ARG, "set_rods: null array or negative count");
     if (n > (1 << 30)) return fail(ctx, ALENS_E_ARG, "set_rods: too many rods");
@@ -220,51 +239,68 @@ alens_status alens_set_rods(alens_ctx *ctx, int32_t n, const int32_t *gid, const
         return fail(ctx, ALENS_E_NONFINITE, "set_rods: non-finite rod data");
     const alens_config &cf = ctx->cfg;
     std::vector<double> hc(n), rc(n);
-    double lmax = 0.0, rmax = 0.0, lsum = 0.0, imax = 0.0;
-    for (int32_t i = 0; i < n; ++i) {
-        if (!(length[i] >= 0.0) || !(radius[i] > 0.0))
-            return fail(ctx, ALENS_E_ARG, "set_rods: length must be >= 0 and radius > 0");
-        hc[i] = (length[i] < radius[i] * 2.0) ? 0.0 : 0.5 * (length[i] * cf.len_col_ratio);
-        rc[i] = radius[i] * cf.diam_col_ratio;
-        lmax = std::max(lmax, 2.0 * hc[i]);
-        rmax = std::max(rmax, rc[i]);
-        lsum += 2.0 * hc[i];
-        imax = std::max(imax, rc[i] + 0.5 * cf.col_buf + kAabbMargin);
-    }
-    for (int k = 0; k < 3; ++k) {
-        if (!cf.pbc[k]) continue;
-        const double len = cf.box_hi[k] - cf.box_lo[k];
-        if (!(len >= 2.0 * (lmax + 2.0 * rmax + cf.col_buf)))
-            return fail(ctx, ALENS_E_BOX, "periodic box shorter than twice the largest interaction range");
-    }
-    // cell grid: edge = configured size, or half the mean collision length (at least twice the
-    // largest AABB inflation); capped at 8 cells per rod
-    double h = cf.cell_size;
-    if (!(h > 0.0)) {
-        h = n ? 0.5 * lsum / n : 1.0;
-        if (h < 2.0 * imax) h = 2.0 * imax;
-        if (!(h > 0.0)) h = 1.0;
-    }
-    const double maxcells = std::max<double>(8.0 * n, 4096.0);
-    Grid g{};
-    for (;;) {
-        double tot = 1.0;
-        for (int k = 0; k < 3; ++k) {
-            g.len[k] = cf.box_hi[k] - cf.box_lo[k];
-            int c = (int)std::floor(g.len[k] / h);
-            if (c < 1) c = 1;
-            g.nc[k] = c;
-            g.h[k] = g.len[k] / c;
-            tot *= c;
+    std::vector<int32_t> rs(n);
+    std::vector<SysDev> sh(nsys);
+    std::vector<int32_t> mono(nsys);
+    int64_t ncell = 0;
+    for (int32_t s = 0; s < nsys; ++s) {
+        const alens_box &bx = boxes[s];
+        for (int k = 0; k < 3; ++k)
+            if (!(bx.box_hi[k] > bx.box_lo[k])) return fail(ctx, ALENS_E_ARG, "set_systems: empty box");
+        double lmax = 0.0, rmax = 0.0, lsum = 0.0, imax = 0.0;
+        const int32_t r0 = rod_offset[s], r1 = rod_offset[s + 1];
+        for (int32_t i = r0; i < r1; ++i) {
+            if (!(length[i] >= 0.0) || !(radius[i] > 0.0))
+                return fail(ctx, ALENS_E_ARG, "set_rods: length must be >= 0 and radius > 0");
+            hc[i] = (length[i] < radius[i] * 2.0) ? 0.0 : 0.5 * (length[i] * cf.len_col_ratio);
+            rc[i] = radius[i] * cf.diam_col_ratio;
+            rs[i] = s;
+            lmax = std::max(lmax, 2.0 * hc[i]);
+            rmax = std::max(rmax, rc[i]);
+            lsum += 2.0 * hc[i];
+            imax = std::max(imax, rc[i] + 0.5 * cf.col_buf + kAabbMargin);
         }
-        if (tot <= maxcells) break;
-        h *= 1.25;
+        for (int k = 0; k < 3; ++k) {
+            if (!bx.pbc[k]) continue;
+            if (!(bx.box_hi[k] - bx.box_lo[k] >= 2.0 * (lmax + 2.0 * rmax + cf.col_buf)))
+                return fail(ctx, ALENS_E_BOX, "periodic box shorter than twice the largest interaction range");
+        }
+        // cell grid: edge = configured size, or half the mean collision length (at least twice the
+        // largest AABB inflation); capped at 8 cells per rod
+        const int32_t ns = r1 - r0;
+        double h = cf.cell_size;
+        if (!(h > 0.0)) {
+            h = ns ? 0.5 * lsum / ns : 1.0;
+            if (h < 2.0 * imax) h = 2.0 * imax;
+            if (!(h > 0.0)) h = 1.0;
+        }
+        const double maxcells = std::max<double>(8.0 * ns, 4096.0);
+        SysDev &g = sh[s];
+        for (;;) {
+            double tot = 1.0;
+            for (int k = 0; k < 3; ++k) {
+                g.box_lo[k] = bx.box_lo[k];
+                g.len[k] = bx.box_hi[k] - bx.box_lo[k];
+                g.pbc[k] = bx.pbc[k] ? 1 : 0;
+                int c = (int)std::floor(g.len[k] / h);
+                if (c < 1) c = 1;
+                g.nc[k] = c;
+                g.h[k] = g.len[k] / c;
+                tot *= c;
+            }
+            if (tot <= maxcells) break;
+            h *= 1.25;
+        }
+        g.mono = bx.monolayer ? 1 : 0;
+        g.cell_off = (uint32_t)ncell;
+        g.rod_off = r0;
+        g.rod_end = r1;
+        mono[s] = g.mono;
+        ncell += (int64_t)g.nc[0] * g.nc[1] * g.nc[2];
+        if (ncell > (int64_t)0xFFFFFFFF) return fail(ctx, ALENS_E_ARG, "set_systems: too many cells");
     }
-    g.ncell = (int64_t)g.nc[0] * g.nc[1] * g.nc[2];
     int kb = 1;
-    while (((int64_t)1 << kb) < g.ncell) ++kb;
-    g.keybits = kb;
-    ctx->grid = g;
+    while (((int64_t)1 << kb) < ncell) ++kb;
 
     TRY_HIP(hipSetDevice(ctx->dev));
     const size_t N = (size_t)std::max(n, 1);
@@ -275,6 +311,7 @@ alens_status alens_set_rods(alens_ctx *ctx, int32_t n, const int32_t *gid, const
     TRY_HIP(ctx->radius.ensure(N));
     TRY_HIP(ctx->hcol.ensure(N));
     TRY_HIP(ctx->rcol.ensure(N));
+    TRY_HIP(ctx->rsys.ensure(N));
     for (auto *bf : {&ctx->mob, &ctx->unc, &ctx->U, &ctx->F, &ctx->Us, &ctx->fext}) TRY_HIP(bf->ensure(6 * N));
     TRY_HIP(ctx->clo.ensure(3 * N));
     TRY_HIP(ctx->chi.ensure(3 * N));
@@ -284,11 +321,15 @@ alens_status alens_set_rods(alens_ctx *ctx, int32_t n, const int32_t *gid, const
     TRY_HIP(ctx->eoff.ensure(N + 1));
     for (auto *bf : {&ctx->cntI, &ctx->cntJ, &ctx->rowStart, &ctx->jStart, &ctx->cur, &ctx->adjStart})
         TRY_HIP(bf->ensure(N + 1));
-    TRY_HIP(ctx->cstart.ensure((size_t)g.ncell));
-    TRY_HIP(ctx->cend.ensure((size_t)g.ncell));
+    TRY_HIP(ctx->cstart.ensure((size_t)ncell));
+    TRY_HIP(ctx->cend.ensure((size_t)ncell));
     TRY_HIP(ctx->counter.ensure(4));
     TRY_HIP(ctx->bar.ensure(4));
-    TRY_HIP(ctx->out.ensure(16));
+    TRY_HIP(ctx->out.ensure(8 * (size_t)nsys + 8));
+    TRY_HIP(ctx->sysd.ensure((size_t)nsys));
+    TRY_HIP(ctx->smono.ensure((size_t)nsys));
+    TRY_HIP(ctx->con_off.ensure((size_t)nsys + 1));
+    TRY_HIP(ctx->chunk_off.ensure((size_t)nsys + 1));
     if (n > 0) {
         TRY_HIP(hipMemcpyAsync(ctx->gid.p, gid, sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->st));
         TRY_HIP(hipMemcpyAsync(ctx->center.p, center, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->st));
@@ -297,7 +338,10 @@ alens_status alens_set_rods(alens_ctx *ctx, int32_t n, const int32_t *gid, const
         TRY_HIP(hipMemcpyAsync(ctx->radius.p, radius, sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
         TRY_HIP(hipMemcpyAsync(ctx->hcol.p, hc.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
         TRY_HIP(hipMemcpyAsync(ctx->rcol.p, rc.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
+        TRY_HIP(hipMemcpyAsync(ctx->rsys.p, rs.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->st));
     }
+    TRY_HIP(hipMemcpyAsync(ctx->sysd.p, sh.data(), sizeof(SysDev) * nsys, hipMemcpyHostToDevice, ctx->st));
+    TRY_HIP(hipMemcpyAsync(ctx->smono.p, mono.data(), sizeof(int32_t) * nsys, hipMemcpyHostToDevice, ctx->st));
     ctx->has_immov = immovable != nullptr;
     if (immovable && n > 0) {
         TRY_HIP(ctx->immov.ensure(N));
@@ -305,13 +349,36 @@ alens_status alens_set_rods(alens_ctx *ctx, int32_t n, const int32_t *gid, const
     }
     // the previous-step impulses stay valid only for the same rods in the same order
     std::vector<int32_t> ng(gid, gid + n);
-    if (ng != ctx->h_gid) ctx->have_prev = false;
+    if (ng != ctx->h_gid || nsys != ctx->nsys) ctx->have_prev = false;
     ctx->h_gid.swap(ng);
     TRY_HIP(hipStreamSynchronize(ctx->st));
     ctx->n = n;
+    ctx->nsys = nsys;
+    ctx->boxes.assign(boxes, boxes + nsys);
+    ctx->rod_off.assign(rod_offset, rod_offset + nsys + 1);
+    ctx->sysh = sh;
+    ctx->ncell = ncell;
+    ctx->keybits = kb;
+    ctx->sysrep.assign((size_t)nsys, alens_solve_report{});
     ctx->have_rods = true;
     ctx->have_step = false;
     return ALENS_OK;
+}
+
+alens_status alens_set_rods(alens_ctx *ctx, int32_t n, const int32_t *gid, const double *center,
+                            const double *dir, const double *length, const double *radius,
+                            const uint8_t *immovable) {
+    if (!ctx) return ALENS_E_ARG;
+    if (n < 0) return fail(ctx, ALENS_E_ARG, "set_rods: null array or negative count");
+    alens_box bx{};
+    for (int k = 0; k < 3; ++k) {
+        bx.box_lo[k] = ctx->cfg.box_lo[k];
+        bx.box_hi[k] = ctx->cfg.box_hi[k];
+        bx.pbc[k] = ctx->cfg.pbc[k];
+    }
+    bx.monolayer = ctx->cfg.monolayer;
+    const int32_t off[2] = {0, n};
+    return alens_set_systems(ctx, 1, off, &bx, gid, center, dir, length, radius, immovable);
 }
 
 alens_status alens_get_rods(alens_ctx *ctx, double *center, double *dir) {
@@ -341,11 +408,20 @@ alens_status alens_sort_rods(alens_ctx *ctx, int32_t *perm_out) {
     TRY_HIP(ctx->vals.ensure(N));
     TRY_HIP(ctx->keys2.ensure(N));
     TRY_HIP(ctx->vals2.ensure(N));
-    const CfgDev cd = make_cfgdev(ctx);
-    TRY_K(launch_morton_keys(st, n, ctx->center.p, cd, ctx->grid, ctx->keys.p, ctx->vals.p));
-    bool alt = false;
-    TRY_K(radix_sort_pairs(st, ctx->keys.p, ctx->vals.p, ctx->keys2.p, ctx->vals2.p, n, 30, ctx->hist, ctx->scan, alt));
-    const uint32_t *perm = alt ? ctx->vals2.p : ctx->vals.p;
+    const RodsDev rd = make_rods(ctx);
+    DBuf<uint32_t> permd;
+    TRY_HIP(permd.ensure(N));
+    for (int32_t sy = 0; sy < ctx->nsys; ++sy) {
+        const int32_t r0 = ctx->rod_off[sy], ns = ctx->rod_off[sy + 1] - r0;
+        if (ns == 0) continue;
+        TRY_K(launch_morton_keys(st, rd, r0, ns, ctx->keys.p, ctx->vals.p));
+        bool alt = false;
+        TRY_K(radix_sort_pairs(st, ctx->keys.p, ctx->vals.p, ctx->keys2.p, ctx->vals2.p, ns, 30, ctx->hist,
+                               ctx->scan, alt));
+        TRY_HIP(hipMemcpyAsync(permd.p + r0, alt ? ctx->vals2.p : ctx->vals.p, sizeof(uint32_t) * ns,
+                               hipMemcpyDeviceToDevice, st));
+    }
+    const uint32_t *perm = permd.p;
     DBuf<int32_t> gid2;
     DBuf<double> center2, dir2, length2, radius2, hcol2, rcol2;
     DBuf<uint8_t> immov2;
@@ -380,6 +456,7 @@ alens_status alens_sort_rods(alens_ctx *ctx, int32_t *perm_out) {
     hcol2.release();
     rcol2.release();
     immov2.release();
+    permd.release();
     std::vector<int32_t> ng(N);
     for (size_t i = 0; i < N; ++i) {
         ng[i] = ctx->h_gid[hp[i]];
@@ -401,19 +478,49 @@ int64_t alens_get_step(const alens_ctx *ctx) { return ctx ? ctx->step : -1; }
 }  // extern "C"
 
 // Launches one persistent BBPGD solve on the context's stream and reads its report words.
+// Per-system constraint offsets -> reduction-chunk offsets and the chunk -> system table.
+static alens_status set_chunk_table(alens_ctx *ctx, const std::vector<uint32_t> &coff) {
+    const int32_t ns = (int32_t)coff.size() - 1;
+    std::vector<uint32_t> choff((size_t)ns + 1, 0);
+    for (int32_t sy = 0; sy < ns; ++sy)
+        choff[sy + 1] = choff[sy] + (coff[sy + 1] - coff[sy] + kRedChunk - 1) / kRedChunk;
+    std::vector<int32_t> csys(std::max<uint32_t>(choff[ns], 1));
+    for (int32_t sy = 0; sy < ns; ++sy)
+        for (uint32_t c = choff[sy]; c < choff[sy + 1]; ++c) csys[c] = sy;
+    TRY_HIP(ctx->con_off.ensure((size_t)ns + 1));
+    TRY_HIP(ctx->chunk_off.ensure((size_t)ns + 1));
+    TRY_HIP(ctx->chunk_sys.ensure(csys.size()));
+    TRY_HIP(hipMemcpyAsync(ctx->con_off.p, coff.data(), sizeof(uint32_t) * (ns + 1), hipMemcpyHostToDevice, ctx->st));
+    TRY_HIP(hipMemcpyAsync(ctx->chunk_off.p, choff.data(), sizeof(uint32_t) * (ns + 1), hipMemcpyHostToDevice,
+                           ctx->st));
+    TRY_HIP(hipMemcpyAsync(ctx->chunk_sys.p, csys.data(), sizeof(int32_t) * csys.size(), hipMemcpyHostToDevice,
+                           ctx->st));
+    TRY_HIP(hipStreamSynchronize(ctx->st));
+    ctx->nchunks_cur = (int32_t)choff[ns];
+    ctx->nsys_cur = ns;
+    return ALENS_OK;
+}
+
+// Launches one persistent BBPGD solve on the context's stream and reads the per-system report words;
+// rep receives the aggregate (max iterations / residuals, all converged, any fallback, worst status).
 template <typename T>
-static alens_status run_bbpgd(alens_ctx *ctx, SolveArgsT<T> a, alens_solve_report *rep, float *ms_solve) {
-    const uint32_t nchunks = (a.m + kRedChunk - 1) / kRedChunk;
-    TRY_HIP(ctx->part.ensure((size_t)kRedQ * std::max<uint32_t>(nchunks, 1)));
+static alens_status run_bbpgd(alens_ctx *ctx, SolveArgsT<T> a, alens_solve_report *rep, float *ms_solve,
+                              std::vector<alens_solve_report> *per_sys) {
+    const int32_t ns = ctx->nsys_cur;
+    TRY_HIP(ctx->part.ensure((size_t)kRedQ * std::max<int32_t>(ctx->nchunks_cur, 1)));
+    TRY_HIP(ctx->out.ensure(8 * (size_t)ns + 8));
     TRY_HIP(hipMemsetAsync(ctx->bar.p, 0, 4 * sizeof(uint32_t), ctx->st));
     a.part = ctx->part.p;
     a.bar = ctx->bar.p;
     a.out = ctx->out.p;
+    a.con_off = ctx->con_off.p;
+    a.chunk_off = ctx->chunk_off.p;
+    a.chunk_sys = ctx->chunk_sys.p;
+    a.nsys = ns;
     a.dt = ctx->cfg.dt;
     a.tol = ctx->cfg.tol;
     a.max_ite = ctx->cfg.max_ite;
-    a.mono = ctx->cfg.monolayer ? 1 : 0;
-    a.nchunks = (int32_t)nchunks;
+    a.nchunks = ctx->nchunks_cur;
     // ALENS_SOLVER_MAX_BLOCKS caps the persistent solver's grid (tuning; default: all co-resident)
     const char *mb = getenv("ALENS_SOLVER_MAX_BLOCKS");
     const int max_blocks = mb ? atoi(mb) : 0;
@@ -423,26 +530,42 @@ static alens_status run_bbpgd(alens_ctx *ctx, SolveArgsT<T> a, alens_solve_repor
     else
         TRY_K(launch_bbpgd32(ctx->st, a, max_blocks, &ctx->grid_used));
     TRY_HIP(hipEventRecord(ctx->ev[3], ctx->st));
-    double o[8];
+    std::vector<double> o(8 * (size_t)ns);
     uint32_t barw[4];
-    TRY_HIP(hipMemcpyAsync(o, ctx->out.p, sizeof o, hipMemcpyDeviceToHost, ctx->st));
+    TRY_HIP(hipMemcpyAsync(o.data(), ctx->out.p, sizeof(double) * o.size(), hipMemcpyDeviceToHost, ctx->st));
     TRY_HIP(hipMemcpyAsync(barw, ctx->bar.p, sizeof barw, hipMemcpyDeviceToHost, ctx->st));
     TRY_HIP(hipStreamSynchronize(ctx->st));
     if (barw[2] != 0) return fail(ctx, ALENS_E_HIP, "BBPGD grid barrier timed out (blocks not co-resident)");
     TRY_HIP(hipEventElapsedTime(ms_solve, ctx->ev[2], ctx->ev[3]));
-    rep->iters = (int32_t)o[0];
-    rep->matvecs = (int32_t)o[1];
-    rep->converged = (int32_t)o[2];
-    rep->fallback_used = (int32_t)o[3];
-    rep->status = (int32_t)o[4];
-    rep->res0 = o[5];
-    rep->res = o[6];
+    *rep = alens_solve_report{};
+    rep->converged = 1;
+    if (per_sys) per_sys->assign((size_t)ns, alens_solve_report{});
+    for (int32_t sy = 0; sy < ns; ++sy) {
+        const double *w = o.data() + 8 * (size_t)sy;
+        alens_solve_report r{};
+        r.iters = (int32_t)w[0];
+        r.matvecs = (int32_t)w[1];
+        r.converged = (int32_t)w[2];
+        r.fallback_used = (int32_t)w[3];
+        r.status = (int32_t)w[4];
+        r.res0 = w[5];
+        r.res = w[6];
+        if (per_sys) (*per_sys)[sy] = r;
+        rep->iters = std::max(rep->iters, r.iters);
+        rep->matvecs = std::max(rep->matvecs, r.matvecs);
+        rep->converged &= r.converged;
+        rep->fallback_used |= r.fallback_used;
+        if (r.status == ALENS_E_NONFINITE || (r.status == ALENS_E_NOCONV && rep->status == 0)) rep->status = r.status;
+        rep->res0 = std::max(rep->res0, r.res0);
+        rep->res = std::max(rep->res, r.res);
+    }
     return ALENS_OK;
 }
 
 // FP64 solve on the step buffers; xg holds the start point
 static alens_status run_solver(alens_ctx *ctx, int32_t n, uint32_t m, const double *mob, bool warm,
-                               alens_solve_report *rep, float *ms_solve) {
+                               alens_solve_report *rep, float *ms_solve, const int32_t *rsys, const int32_t *smono,
+                               std::vector<alens_solve_report> *per_sys) {
     SolveArgsT<double> a{};
     a.n = n;
     a.m = m;
@@ -457,8 +580,10 @@ static alens_status run_solver(alens_ctx *ctx, int32_t n, uint32_t m, const doub
     a.lb = nullptr;
     a.xg = ctx->xg.p;
     a.U = ctx->Us.p;
+    a.rsys = rsys;
+    a.smono = smono;
     a.warm = warm ? 1 : 0;
-    return run_bbpgd<double>(ctx, a, rep, ms_solve);
+    return run_bbpgd<double>(ctx, a, rep, ms_solve, per_sys);
 }
 
 // Precision form: FP32 BBPGD from the start point in xg, then the FP64 solve started from the
@@ -492,16 +617,20 @@ static alens_status run_solver_mixed(alens_ctx *ctx, int32_t n, uint32_t m, bool
     a.lb = nullptr;
     a.xg = ctx->xg32.p;
     a.U = ctx->U32.p;
+    a.rsys = ctx->rsys.p;
+    a.smono = ctx->smono.p;
     a.warm = warm ? 1 : 0;
     alens_solve_report r32{};
+    std::vector<alens_solve_report> s32;
     float ms32 = 0.f, ms64 = 0.f;
-    alens_status s = run_bbpgd<float>(ctx, a, &r32, &ms32);
+    alens_status s = run_bbpgd<float>(ctx, a, &r32, &ms32, &s32);
     if (s != ALENS_OK) return s;
     TRY_K(launch_xg32_to_x64(st, ctx->xg32.p, ctx->gamma.p, m));
     TRY_K(launch_init_xg(st, ctx->xg.p, ctx->gamma.p, m));
-    s = run_solver(ctx, n, m, ctx->mob.p, true, rep, &ms64);
+    s = run_solver(ctx, n, m, ctx->mob.p, true, rep, &ms64, ctx->rsys.p, ctx->smono.p, &ctx->sysrep);
     if (s != ALENS_OK) return s;
     rep->iters_fp32 = r32.iters;
+    for (size_t q = 0; q < s32.size() && q < ctx->sysrep.size(); ++q) ctx->sysrep[q].iters_fp32 = s32[q].iters;
     *ms_solve = ms32 + ms64;
     return ALENS_OK;
 }
@@ -515,7 +644,6 @@ alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_s
     TRY_HIP(hipSetDevice(ctx->dev));
     hipStream_t st = ctx->st;
     const int32_t n = ctx->n;
-    const Grid &g = ctx->grid;
     const CfgDev cd = make_cfgdev(ctx);
     const RodsDev rd = make_rods(ctx);
     if (f_ext) {
@@ -523,11 +651,10 @@ alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_s
         if (n > 0) TRY_HIP(hipMemcpyAsync(ctx->fext.p, f_ext, sizeof(double) * 6 * n, hipMemcpyHostToDevice, st));
     }
     TRY_HIP(hipEventRecord(ctx->ev[0], st));
-    TRY_K(launch_mob_unc(st, n, ctx->gid.p, ctx->dir.p, ctx->length.p, ctx->radius.p,
-                         ctx->has_immov ? ctx->immov.p : nullptr, f_ext ? ctx->fext.p : nullptr, cd,
-                         ctx->mob.p, ctx->unc.p));
+    TRY_K(launch_mob_unc(st, rd, ctx->length.p, ctx->radius.p, ctx->has_immov ? ctx->immov.p : nullptr,
+                         f_ext ? ctx->fext.p : nullptr, cd, ctx->mob.p, ctx->unc.p));
     // cell list
-    TRY_K(launch_aabb(st, rd, cd, g, ctx->clo.p, ctx->chi.p, ctx->blo.p, ctx->bhi.p, ctx->ecnt.p));
+    TRY_K(launch_aabb(st, rd, cd, ctx->clo.p, ctx->chi.p, ctx->blo.p, ctx->bhi.p, ctx->ecnt.p));
     TRY_K(exclusive_scan_u32(st, ctx->ecnt.p, ctx->eoff.p, n, ctx->scan));
     uint32_t E = 0;
     TRY_HIP(hipMemcpyAsync(&E, ctx->eoff.p + n, sizeof E, hipMemcpyDeviceToHost, st));
@@ -538,14 +665,14 @@ alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_s
     TRY_HIP(ctx->vals.ensure(Ecap));
     TRY_HIP(ctx->keys2.ensure(Ecap));
     TRY_HIP(ctx->vals2.ensure(Ecap));
-    TRY_K(launch_fill_entries(st, n, ctx->clo.p, ctx->chi.p, ctx->eoff.p, g, ctx->keys.p, ctx->vals.p));
+    TRY_K(launch_fill_entries(st, rd, ctx->clo.p, ctx->chi.p, ctx->eoff.p, ctx->keys.p, ctx->vals.p));
     bool alt = false;
-    TRY_K(radix_sort_pairs(st, ctx->keys.p, ctx->vals.p, ctx->keys2.p, ctx->vals2.p, E, g.keybits,
+    TRY_K(radix_sort_pairs(st, ctx->keys.p, ctx->vals.p, ctx->keys2.p, ctx->vals2.p, E, ctx->keybits,
                            ctx->hist, ctx->scan, alt));
     const uint32_t *skeys = alt ? ctx->keys2.p : ctx->keys.p;
     const uint32_t *svals = alt ? ctx->vals2.p : ctx->vals.p;
-    TRY_HIP(hipMemsetAsync(ctx->cstart.p, 0, sizeof(uint32_t) * g.ncell, st));
-    TRY_HIP(hipMemsetAsync(ctx->cend.p, 0, sizeof(uint32_t) * g.ncell, st));
+    TRY_HIP(hipMemsetAsync(ctx->cstart.p, 0, sizeof(uint32_t) * ctx->ncell, st));
+    TRY_HIP(hipMemsetAsync(ctx->cend.p, 0, sizeof(uint32_t) * ctx->ncell, st));
     TRY_K(launch_cell_ranges(st, skeys, E, ctx->cstart.p, ctx->cend.p));
     // pair sweep (re-run once with a larger buffer if the pair buffer overflowed)
     if (ctx->pcap == 0) {
@@ -560,7 +687,7 @@ alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_s
         TRY_HIP(hipMemsetAsync(ctx->counter.p, 0, sizeof(uint32_t), st));
         TRY_HIP(hipMemsetAsync(ctx->cntI.p, 0, sizeof(uint32_t) * (n + 1), st));
         TRY_HIP(hipMemsetAsync(ctx->cntJ.p, 0, sizeof(uint32_t) * (n + 1), st));
-        TRY_K(launch_sweep(st, rd, cd, g, skeys, E, ctx->cend.p, svals, ctx->clo.p, ctx->blo.p, ctx->bhi.p,
+        TRY_K(launch_sweep(st, rd, cd, skeys, E, ctx->cend.p, svals, ctx->clo.p, ctx->blo.p, ctx->bhi.p,
                            ctx->counter.p, ctx->pcap, ctx->pI.p, ctx->pJ.p, ctx->pImg.p, ctx->cntI.p,
                            ctx->cntJ.p));
         TRY_HIP(hipMemcpyAsync(&M, ctx->counter.p, sizeof M, hipMemcpyDeviceToHost, st));
@@ -600,6 +727,16 @@ alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_s
                       ctx->posI.p, ctx->posJ.p, ctx->b.p));
     TRY_K(launch_adjacency(st, n, ctx->rowStart.p, ctx->jStart.p, ctx->jlist.p, ctx->geo.p, ctx->adjStart.p,
                            ctx->adjK.p, ctx->adjW.p));
+    {
+        // constraint range of each system = rowStart at its first rod (rows are grouped by rod I)
+        std::vector<uint32_t> coff((size_t)ctx->nsys + 1);
+        for (int32_t sy = 0; sy <= ctx->nsys; ++sy)
+            TRY_HIP(hipMemcpyAsync(&coff[sy], ctx->rowStart.p + ctx->rod_off[sy], sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, st));
+        TRY_HIP(hipStreamSynchronize(st));
+        alens_status cs = set_chunk_table(ctx, coff);
+        if (cs != ALENS_OK) return cs;
+    }
     TRY_HIP(hipEventRecord(ctx->ev[1], st));
     // solve: start point = 0 (cold), the previous step's gamma (warm_start 1) or the network's
     // prediction from this step's features and the previous gamma (warm_start 2)
@@ -628,14 +765,17 @@ alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_s
             TRY_K(launch_mlp_infer_xg(st, M, ctx->feat.p, ctx->mlpP.p, ctx->mlpNorm.p, ctx->xg.p));
             warm = true;
         }
-        alens_status s = ctx->cfg.precision == 1 ? run_solver_mixed(ctx, n, M, warm, &rep, &ms_solve)
-                                                 : run_solver(ctx, n, M, ctx->mob.p, warm, &rep, &ms_solve);
+        alens_status s = ctx->cfg.precision == 1
+                             ? run_solver_mixed(ctx, n, M, warm, &rep, &ms_solve)
+                             : run_solver(ctx, n, M, ctx->mob.p, warm, &rep, &ms_solve, ctx->rsys.p, ctx->smono.p,
+                                          &ctx->sysrep);
         if (s != ALENS_OK) return s;
     } else {
         rep.converged = 1;
+        for (auto &r : ctx->sysrep) r = alens_solve_report{}, r.converged = 1;
     }
-    TRY_K(launch_writeback(st, n, ctx->adjStart.p, ctx->adjK.p, ctx->adjW.p, ctx->xg.p, ctx->mob.p, ctx->unc.p,
-                           ctx->cfg.monolayer ? 1 : 0, ctx->F.p, ctx->U.p, ctx->gamma.p, M));
+    TRY_K(launch_writeback(st, n, ctx->rsys.p, ctx->sysd.p, ctx->adjStart.p, ctx->adjK.p, ctx->adjW.p, ctx->xg.p,
+                           ctx->mob.p, ctx->unc.p, ctx->F.p, ctx->U.p, ctx->gamma.p, M));
     if (ctx->cfg.warm_start >= 1) {
         TRY_HIP(ctx->pkey.ensure(Mc));
         TRY_HIP(ctx->pgam.ensure(Mc));
@@ -665,7 +805,7 @@ alens_status alens_step_euler(alens_ctx *ctx) {
     if (!ctx->have_step) return fail(ctx, ALENS_E_STATE, "step_euler before step_constraints");
     TRY_HIP(hipSetDevice(ctx->dev));
     const CfgDev cd = make_cfgdev(ctx);
-    TRY_K(launch_euler(ctx->st, ctx->n, ctx->center.p, ctx->dir.p, ctx->U.p, cd));
+    TRY_K(launch_euler(ctx->st, ctx->n, ctx->rsys.p, ctx->sysd.p, ctx->center.p, ctx->dir.p, ctx->U.p, cd));
     TRY_HIP(hipStreamSynchronize(ctx->st));
     ctx->step += 1;
     ctx->have_step = false;
@@ -850,11 +990,21 @@ alens_status alens_bcqp_solve(alens_ctx *ctx, int32_t n, int64_t m, const int32_
     TRY_HIP(hipMemcpyAsync(ctx->fext.p, mob, sizeof(double) * 6 * n, hipMemcpyHostToDevice, st));
     if (x0) TRY_HIP(hipMemcpyAsync(ctx->delta0.p, x0, sizeof(double) * M, hipMemcpyHostToDevice, st));
     TRY_K(launch_init_xg(st, ctx->xg.p, x0 ? ctx->delta0.p : nullptr, (uint32_t)M));
+    // one system: every rod in system 0, no monolayer projection
+    DBuf<int32_t> rs0, mono0;
+    TRY_HIP(rs0.ensure(N));
+    TRY_HIP(mono0.ensure(1));
+    TRY_HIP(hipMemsetAsync(rs0.p, 0, sizeof(int32_t) * N, st));
+    TRY_HIP(hipMemsetAsync(mono0.p, 0, sizeof(int32_t), st));
+    alens_status cs = set_chunk_table(ctx, std::vector<uint32_t>{0u, (uint32_t)M});
+    if (cs != ALENS_OK) return cs;
     float ms = 0.f;
-    alens_status s = run_solver(ctx, n, (uint32_t)M, ctx->fext.p, x0 != nullptr, &rep, &ms);
+    alens_status s = run_solver(ctx, n, (uint32_t)M, ctx->fext.p, x0 != nullptr, &rep, &ms, rs0.p, mono0.p, nullptr);
+    rs0.release();
+    mono0.release();
     if (s != ALENS_OK) return s;
-    TRY_K(launch_writeback(st, 0, nullptr, nullptr, nullptr, ctx->xg.p, nullptr, nullptr, 0, nullptr, nullptr,
-                           ctx->gamma.p, (uint32_t)M));
+    TRY_K(launch_writeback(st, 0, nullptr, nullptr, nullptr, nullptr, nullptr, ctx->xg.p, nullptr, nullptr, nullptr,
+                           nullptr, ctx->gamma.p, (uint32_t)M));
     TRY_HIP(hipMemcpyAsync(gamma_out, ctx->gamma.p, sizeof(double) * M, hipMemcpyDeviceToHost, st));
     TRY_HIP(hipStreamSynchronize(st));
     rep.n_con = m;
@@ -1075,6 +1225,19 @@ alens_status alens_mlp_train(alens_ctx *ctx, int64_t m, const float *X, const fl
     tb.release();
     for (auto *b : {&dX, &dT, &dXB, &dTB, &dG, &dM1, &dM2, &dL}) b->release();
     dIdx.release();
+    return ALENS_OK;
+}
+
+alens_status alens_get_num_systems(const alens_ctx *ctx, int32_t *nsys) {
+    if (!ctx || !nsys) return ALENS_E_ARG;
+    *nsys = ctx->nsys;
+    return ALENS_OK;
+}
+
+alens_status alens_get_system_reports(alens_ctx *ctx, alens_solve_report *reps) {
+    if (!ctx || !reps) return ALENS_E_ARG;
+    if (!ctx->have_step) return fail(ctx, ALENS_E_STATE, "no step has been run");
+    for (int32_t sy = 0; sy < ctx->nsys; ++sy) reps[sy] = ctx->sysrep[sy];
     return ALENS_OK;
 }
 

@@ -22,12 +22,14 @@ static inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / 
 // mobility + U_nc = M F_ext + U_B.  U_B = sqrt(2 kT/dt) M^{1/2} xi, xi from Philox4x32-10 keyed by
 // the seed, counter (gid, step_lo, step_hi, c), Box-Muller on 53-bit uniforms.
 // ------------------------------------------------------------------------------------------------
-__global__ void k_mob_unc(int32_t n, const int32_t *__restrict__ gid, const double *__restrict__ dir,
+__global__ void k_mob_unc(int32_t n, const int32_t *__restrict__ rsys, const SysDev *__restrict__ sys,
+                          const int32_t *__restrict__ gid, const double *__restrict__ dir,
                           const double *__restrict__ length, const double *__restrict__ radius,
                           const uint8_t *__restrict__ immovable, const double *__restrict__ fext,
                           CfgDev c, double *__restrict__ mob, double *__restrict__ unc) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const int mono = sys[rsys[i]].mono;
     double ipar, iperp, irot;
     drag_inv(length[i], radius[i], c.mu, ipar, iperp, irot);
     double mb[6];
@@ -78,14 +80,14 @@ __global__ void k_mob_unc(int32_t n, const int32_t *__restrict__ gid, const doub
         double fe[6], ue[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) fe[k] = fext[6 * i + k];
-        apply_M(mb, fe, c.mono, ue);
+        apply_M(mb, fe, mono, ue);
 #pragma unroll
         for (int k = 0; k < 6; ++k) u[k] = ue[k] + ub[k];
     } else {
 #pragma unroll
         for (int k = 0; k < 6; ++k) u[k] = ub[k];
     }
-    if (c.mono) {
+    if (mono) {
         u[2] = 0.0;
         u[3] = 0.0;
         u[4] = 0.0;
@@ -97,11 +99,11 @@ __global__ void k_mob_unc(int32_t n, const int32_t *__restrict__ gid, const doub
     }
 }
 
-HipErr launch_mob_unc(hipStream_t st, int32_t n, const int32_t *gid, const double *dir,
-                      const double *length, const double *radius, const uint8_t *immovable,
-                      const double *fext, const CfgDev &c, double *mob, double *unc) {
-    if (n == 0) return hip_ok();
-    k_mob_unc<<<nblk(n, 256), 256, 0, st>>>(n, gid, dir, length, radius, immovable, fext, c, mob, unc);
+HipErr launch_mob_unc(hipStream_t st, const RodsDev &r, const double *length, const double *radius,
+                      const uint8_t *immovable, const double *fext, const CfgDev &c, double *mob, double *unc) {
+    if (r.n == 0) return hip_ok();
+    k_mob_unc<<<nblk(r.n, 256), 256, 0, st>>>(r.n, r.rsys, r.sys, r.gid, r.dir, length, radius, immovable, fext, c,
+                                              mob, unc);
     LAUNCH_CHECK();
     return hip_ok();
 }
@@ -110,11 +112,11 @@ HipErr launch_mob_unc(hipStream_t st, int32_t n, const int32_t *gid, const doubl
 // AABB (inflated by collision radius + colBuf/2 + margin) and its cell range per axis.  Periodic
 // ranges are unwrapped integers; non-periodic ranges are clamped to the grid.
 // ------------------------------------------------------------------------------------------------
-__global__ void k_aabb(RodsDev r, CfgDev c, Grid g, int32_t *__restrict__ clo,
-                       int32_t *__restrict__ chi, double *__restrict__ blo,
-                       double *__restrict__ bhi, uint32_t *__restrict__ ecnt) {
+__global__ void k_aabb(RodsDev r, CfgDev c, int32_t *__restrict__ clo, int32_t *__restrict__ chi,
+                       double *__restrict__ blo, double *__restrict__ bhi, uint32_t *__restrict__ ecnt) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= r.n) return;
+    const SysDev &S = r.sys[r.rsys[i]];
     const double hh = r.hcol[i];
     const double infl = r.rcol[i] + 0.5 * c.col_buf + kAabbMargin;
     uint32_t cnt = 1;
@@ -123,11 +125,11 @@ __global__ void k_aabb(RodsDev r, CfgDev c, Grid g, int32_t *__restrict__ clo,
         const double hw = hh * fabs(r.dir[3 * i + k]);
         const double lo = (r.center[3 * i + k] - hw) - infl;
         const double hi = (r.center[3 * i + k] + hw) + infl;
-        int a = (int)floor((lo - c.box_lo[k]) / g.h[k]);
-        int b = (int)floor((hi - c.box_lo[k]) / g.h[k]);
-        if (!c.pbc[k]) {
-            a = a < 0 ? 0 : (a >= g.nc[k] ? g.nc[k] - 1 : a);
-            b = b < 0 ? 0 : (b >= g.nc[k] ? g.nc[k] - 1 : b);
+        int a = (int)floor((lo - S.box_lo[k]) / S.h[k]);
+        int b = (int)floor((hi - S.box_lo[k]) / S.h[k]);
+        if (!S.pbc[k]) {
+            a = a < 0 ? 0 : (a >= S.nc[k] ? S.nc[k] - 1 : a);
+            b = b < 0 ? 0 : (b >= S.nc[k] ? S.nc[k] - 1 : b);
         }
         clo[3 * i + k] = a;
         chi[3 * i + k] = b;
@@ -138,10 +140,10 @@ __global__ void k_aabb(RodsDev r, CfgDev c, Grid g, int32_t *__restrict__ clo,
     ecnt[i] = cnt;
 }
 
-HipErr launch_aabb(hipStream_t st, const RodsDev &r, const CfgDev &c, const Grid &g, int32_t *clo,
-                   int32_t *chi, double *blo, double *bhi, uint32_t *ecnt) {
+HipErr launch_aabb(hipStream_t st, const RodsDev &r, const CfgDev &c, int32_t *clo, int32_t *chi, double *blo,
+                   double *bhi, uint32_t *ecnt) {
     if (r.n == 0) return hip_ok();
-    k_aabb<<<nblk(r.n, 256), 256, 0, st>>>(r, c, g, clo, chi, blo, bhi, ecnt);
+    k_aabb<<<nblk(r.n, 256), 256, 0, st>>>(r, c, clo, chi, blo, bhi, ecnt);
     LAUNCH_CHECK();
     return hip_ok();
 }
@@ -255,11 +257,12 @@ __device__ inline int pmod_i(int a, int n) {
     return r < 0 ? r + n : r;
 }
 
-__global__ void k_fill_entries(int32_t n, const int32_t *__restrict__ clo,
-                               const int32_t *__restrict__ chi, const uint32_t *__restrict__ eoff,
-                               Grid g, uint32_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+__global__ void k_fill_entries(RodsDev r, const int32_t *__restrict__ clo, const int32_t *__restrict__ chi,
+                               const uint32_t *__restrict__ eoff, uint32_t *__restrict__ keys,
+                               uint32_t *__restrict__ vals) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= r.n) return;
+    const SysDev &g = r.sys[r.rsys[i]];
     uint32_t w = eoff[i];
     const int a0 = clo[3 * i], a1 = clo[3 * i + 1], a2 = clo[3 * i + 2];
     const int b0 = chi[3 * i], b1 = chi[3 * i + 1], b2 = chi[3 * i + 2];
@@ -269,7 +272,7 @@ __global__ void k_fill_entries(int32_t n, const int32_t *__restrict__ clo,
             const uint32_t cy = (uint32_t)pmod_i(y, g.nc[1]);
             for (int x = a0; x <= b0; ++x) {
                 const uint32_t cx = (uint32_t)pmod_i(x, g.nc[0]);
-                keys[w] = (cz * (uint32_t)g.nc[1] + cy) * (uint32_t)g.nc[0] + cx;
+                keys[w] = g.cell_off + (cz * (uint32_t)g.nc[1] + cy) * (uint32_t)g.nc[0] + cx;
                 vals[w] = (uint32_t)i;
                 ++w;
             }
@@ -277,10 +280,10 @@ __global__ void k_fill_entries(int32_t n, const int32_t *__restrict__ clo,
     }
 }
 
-HipErr launch_fill_entries(hipStream_t st, int32_t n, const int32_t *clo, const int32_t *chi,
-                           const uint32_t *eoff, const Grid &g, uint32_t *keys, uint32_t *vals) {
-    if (n == 0) return hip_ok();
-    k_fill_entries<<<nblk(n, 256), 256, 0, st>>>(n, clo, chi, eoff, g, keys, vals);
+HipErr launch_fill_entries(hipStream_t st, const RodsDev &r, const int32_t *clo, const int32_t *chi,
+                           const uint32_t *eoff, uint32_t *keys, uint32_t *vals) {
+    if (r.n == 0) return hip_ok();
+    k_fill_entries<<<nblk(r.n, 256), 256, 0, st>>>(r, clo, chi, eoff, keys, vals);
     LAUNCH_CHECK();
     return hip_ok();
 }
@@ -418,7 +421,7 @@ HipErr launch_cell_ranges(hipStream_t st, const uint32_t *keys, int64_t E, uint3
 // ------------------------------------------------------------------------------------------------
 // One thread per sorted entry e (rod i in cell c): pairs with the later entries of the same cell.
 // Lanes of a wave run the same trip count (the wave maximum) so emission stays wave-aggregated.
-__global__ __launch_bounds__(256) void k_sweep_entries(RodsDev r, CfgDev c, Grid g, const uint32_t *__restrict__ keys,
+__global__ __launch_bounds__(256) void k_sweep_entries(RodsDev r, CfgDev c, const uint32_t *__restrict__ keys,
                                                        int64_t E, const uint32_t *__restrict__ cend,
                                                        const uint32_t *__restrict__ vals,
                                                        const int32_t *__restrict__ clo,
@@ -436,19 +439,29 @@ __global__ __launch_bounds__(256) void k_sweep_entries(RodsDev r, CfgDev c, Grid
     int cc[3] = {0, 0, 0}, cli[3] = {0, 0, 0}, ui[3] = {0, 0, 0};
     double bli[3] = {0, 0, 0}, bhi_i[3] = {0, 0, 0}, ci[3] = {0, 0, 0}, ti[3] = {0, 0, 0};
     double hi_ = 0.0, ri = 0.0;
+    int pbc[3] = {0, 0, 0}, nc[3] = {1, 1, 1};
+    double len[3] = {0, 0, 0};
     if (e < E) {
         const uint32_t cell = keys[e];
         s1 = cend[cell];
         cnt = s1 - (uint32_t)e - 1u;
         i = vals[e];
-        cc[0] = (int)(cell % (uint32_t)g.nc[0]);
-        cc[1] = (int)((cell / (uint32_t)g.nc[0]) % (uint32_t)g.nc[1]);
-        cc[2] = (int)(cell / ((uint32_t)g.nc[0] * (uint32_t)g.nc[1]));
+        const SysDev &g = r.sys[r.rsys[i]];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            pbc[k] = g.pbc[k];
+            nc[k] = g.nc[k];
+            len[k] = g.len[k];
+        }
+        const uint32_t lc = cell - g.cell_off;
+        cc[0] = (int)(lc % (uint32_t)nc[0]);
+        cc[1] = (int)((lc / (uint32_t)nc[0]) % (uint32_t)nc[1]);
+        cc[2] = (int)(lc / ((uint32_t)nc[0] * (uint32_t)nc[1]));
         if (cnt) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 cli[k] = clo[3 * i + k];
-                ui[k] = c.pbc[k] ? cli[k] + pmod_i(cc[k] - cli[k], g.nc[k]) : cc[k];
+                ui[k] = pbc[k] ? cli[k] + pmod_i(cc[k] - cli[k], nc[k]) : cc[k];
                 bli[k] = blo[3 * i + k];
                 bhi_i[k] = bhi[3 * i + k];
                 ci[k] = r.center[3 * i + k];
@@ -474,17 +487,17 @@ __global__ __launch_bounds__(256) void k_sweep_entries(RodsDev r, CfgDev c, Grid
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 const int clj = clo[3 * j + k];
-                const int uj = c.pbc[k] ? clj + pmod_i(cc[k] - clj, g.nc[k]) : cc[k];
+                const int uj = pbc[k] ? clj + pmod_i(cc[k] - clj, nc[k]) : cc[k];
                 ok = ok && (ui[k] == cli[k] || uj == clj);
-                m[k] = c.pbc[k] ? (ui[k] - uj) / g.nc[k] : 0;
-                const double sh = c.pbc[k] ? (double)m[k] * c.len[k] : 0.0;
+                m[k] = pbc[k] ? (ui[k] - uj) / nc[k] : 0;
+                const double sh = pbc[k] ? (double)m[k] * len[k] : 0.0;
                 ok = ok && (blo[3 * j + k] + sh <= bhi_i[k]) && (bli[k] <= bhi[3 * j + k] + sh);
             }
             if (ok) {
                 double cj[3];
 #pragma unroll
                 for (int k = 0; k < 3; ++k)
-                    cj[k] = c.pbc[k] ? r.center[3 * j + k] + (double)m[k] * c.len[k] : r.center[3 * j + k];
+                    cj[k] = pbc[k] ? r.center[3 * j + k] + (double)m[k] * len[k] : r.center[3 * j + k];
                 const SegRes sg = segseg(ci[0], ci[1], ci[2], ti[0], ti[1], ti[2], hi_, cj[0], cj[1], cj[2],
                                          r.dir[3 * j], r.dir[3 * j + 1], r.dir[3 * j + 2], r.hcol[j]);
                 const double d0 = (sg.dist - ri) - r.rcol[j];
@@ -511,12 +524,12 @@ __global__ __launch_bounds__(256) void k_sweep_entries(RodsDev r, CfgDev c, Grid
     }
 }
 
-HipErr launch_sweep(hipStream_t st, const RodsDev &r, const CfgDev &c, const Grid &g, const uint32_t *keys,
-                    int64_t E, const uint32_t *cend, const uint32_t *vals, const int32_t *clo, const double *blo,
+HipErr launch_sweep(hipStream_t st, const RodsDev &r, const CfgDev &c, const uint32_t *keys, int64_t E,
+                    const uint32_t *cend, const uint32_t *vals, const int32_t *clo, const double *blo,
                     const double *bhi, uint32_t *pair_count, uint32_t cap, uint32_t *pI, uint32_t *pJ,
                     uint32_t *pImg, uint32_t *cntI, uint32_t *cntJ) {
     if (E == 0) return hip_ok();
-    k_sweep_entries<<<nblk(E, 256), 256, 0, st>>>(r, c, g, keys, E, cend, vals, clo, blo, bhi, pair_count, cap, pI,
+    k_sweep_entries<<<nblk(E, 256), 256, 0, st>>>(r, c, keys, E, cend, vals, clo, blo, bhi, pair_count, cap, pI,
                                                   pJ, pImg, cntI, cntJ);
     LAUNCH_CHECK();
     return hip_ok();
@@ -616,10 +629,11 @@ __global__ void k_rows(RodsDev r, CfgDev c, uint32_t m, const int32_t *__restric
     if (k >= m) return;
     const int32_t i = cI[k], j = cJ[k];
     const uint32_t im = cImg[k];
+    const SysDev &S = r.sys[r.rsys[i]];
     double cj[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
-        cj[a] = c.pbc[a] ? r.center[3 * j + a] + (double)img_get(im, a) * c.len[a] : r.center[3 * j + a];
+        cj[a] = S.pbc[a] ? r.center[3 * j + a] + (double)img_get(im, a) * S.len[a] : r.center[3 * j + a];
     const double tI0 = r.dir[3 * i], tI1 = r.dir[3 * i + 1], tI2 = r.dir[3 * i + 2];
     const double tJ0 = r.dir[3 * j], tJ1 = r.dir[3 * j + 1], tJ2 = r.dir[3 * j + 2];
     const SegRes sg = segseg(r.center[3 * i], r.center[3 * i + 1], r.center[3 * i + 2], tI0, tI1, tI2,
@@ -722,25 +736,28 @@ __device__ inline uint32_t spread3(uint32_t v) {
     return v;
 }
 
-__global__ void k_morton_keys(int32_t n, const double *__restrict__ center, CfgDev c, Grid g,
-                              uint32_t *__restrict__ keys, uint32_t *__restrict__ vals) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ void k_morton_keys(RodsDev r, int32_t i0, int32_t n, uint32_t *__restrict__ keys,
+                              uint32_t *__restrict__ vals) {
+    const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int32_t i = i0 + t;
+    const SysDev &g = r.sys[r.rsys[i]];
+    const double *center = r.center;
     uint32_t q[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        int a = (int)floor((center[3 * i + k] - c.box_lo[k]) / g.h[k]);
-        a = c.pbc[k] ? pmod_i(a, g.nc[k]) : (a < 0 ? 0 : (a >= g.nc[k] ? g.nc[k] - 1 : a));
+        int a = (int)floor((center[3 * i + k] - g.box_lo[k]) / g.h[k]);
+        a = g.pbc[k] ? pmod_i(a, g.nc[k]) : (a < 0 ? 0 : (a >= g.nc[k] ? g.nc[k] - 1 : a));
         q[k] = (uint32_t)a;
     }
-    keys[i] = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
-    vals[i] = (uint32_t)i;
+    keys[t] = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
+    vals[t] = (uint32_t)i;
 }
 
-HipErr launch_morton_keys(hipStream_t st, int32_t n, const double *center, const CfgDev &c, const Grid &g,
-                          uint32_t *keys, uint32_t *vals) {
+HipErr launch_morton_keys(hipStream_t st, const RodsDev &r, int32_t i0, int32_t n, uint32_t *keys,
+                          uint32_t *vals) {
     if (n == 0) return hip_ok();
-    k_morton_keys<<<nblk(n, 256), 256, 0, st>>>(n, center, c, g, keys, vals);
+    k_morton_keys<<<nblk(n, 256), 256, 0, st>>>(r, i0, n, keys, vals);
     LAUNCH_CHECK();
     return hip_ok();
 }

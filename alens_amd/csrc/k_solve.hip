@@ -105,44 +105,46 @@ __device__ inline void group_maxes(double (&v)[NQ], double *sh, double (&tot)[NQ
     __syncthreads();
 }
 
-// Totals over all chunk partials: sums of slots S[0..NS) in the fixed order (lane c % 256 of group 0
-// accumulates partials c, c + 256, ... left to right, then the group fold), maxes of slots X[0..NX).
-// Every block computes the same values; results land in tot_s / tot_x for every thread.
+// Per-system totals over chunk partials: for each system s (four at a time, one per 256-lane
+// group), lane c % 256 of the group accumulates the partials of the system's chunks c, c + 256, ...
+// left to right, then the group folds them (sums of slots S[], maxes of slots X[]).  Results land in
+// tot[q][s] (q < NS: sums, then the maxes) for every thread; systems not active are skipped.
+enum : int { S_ACTIVE = 0, S_CONVERGED = 1, S_NONFINITE = 2, S_NOCONV = 3 };
+
 template <int NS, int NX>
-__device__ inline void fold_partials(const double *part, int nchunks, const int (&S)[NS], const int (&X)[NX],
-                                     double *sh, double (&tot_s)[NS], double (&tot_x)[NX]) {
-    double vs[NS], vx[NX];
+__device__ inline void fold_systems(const double *part, int nchunks, const uint32_t *chunk_off, int nsys,
+                                    const int *state, const int (&S)[NS], const int (&X)[NX], double *sh,
+                                    double (*tot)[kMaxSys]) {
+    const int g = threadIdx.x >> 8, lane = threadIdx.x & 255;
+    for (int s0 = 0; s0 < nsys; s0 += kGrp) {
+        const int sy = s0 + g;
+        double vs[NS], vx[NX];
 #pragma unroll
-    for (int q = 0; q < NS; ++q) vs[q] = 0.0;
+        for (int q = 0; q < NS; ++q) vs[q] = 0.0;
 #pragma unroll
-    for (int q = 0; q < NX; ++q) vx[q] = 0.0;
-    if (threadIdx.x < kRedThreads) {
-        for (int c = threadIdx.x; c < nchunks; c += kRedThreads) {
+        for (int q = 0; q < NX; ++q) vx[q] = 0.0;
+        if (sy < nsys && state[sy] == S_ACTIVE) {
+            const int c1 = (int)chunk_off[sy + 1];
+            for (int c = (int)chunk_off[sy] + lane; c < c1; c += kRedThreads) {
 #pragma unroll
-            for (int q = 0; q < NS; ++q) vs[q] = vs[q] + part[(size_t)S[q] * nchunks + c];
+                for (int q = 0; q < NS; ++q) vs[q] = vs[q] + part[(size_t)S[q] * nchunks + c];
 #pragma unroll
-            for (int q = 0; q < NX; ++q) {
-                const double v = part[(size_t)X[q] * nchunks + c];
-                vx[q] = v > vx[q] ? v : vx[q];
+                for (int q = 0; q < NX; ++q) {
+                    const double v = part[(size_t)X[q] * nchunks + c];
+                    vx[q] = v > vx[q] ? v : vx[q];
+                }
             }
         }
+        double ts[NS], tx[NX];
+        group_sums<NS>(vs, sh, ts);
+        group_maxes<NX>(vx, sh, tx);
+        if (lane == 0 && sy < nsys) {
+#pragma unroll
+            for (int q = 0; q < NS; ++q) tot[q][sy] = ts[q];
+#pragma unroll
+            for (int q = 0; q < NX; ++q) tot[NS + q][sy] = tx[q];
+        }
     }
-    double ts[NS], tx[NX];
-    group_sums<NS>(vs, sh, ts);
-    group_maxes<NX>(vx, sh, tx);
-    // group 0 holds the totals; broadcast them through LDS
-    __shared__ double bc[16];
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NS; ++q) bc[q] = ts[q];
-#pragma unroll
-        for (int q = 0; q < NX; ++q) bc[8 + q] = tx[q];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NS; ++q) tot_s[q] = bc[q];
-#pragma unroll
-    for (int q = 0; q < NX; ++q) tot_x[q] = bc[8 + q];
     __syncthreads();
 }
 
@@ -167,11 +169,15 @@ __device__ inline T a_value(const SolveArgsT<T> &a, uint32_t k, T alpha) {
 }
 
 // phase A: per rod, F = sum over incident constraints (ascending k) of v_k * w_e; U = M F
+// (rods of systems that have stopped are skipped)
 template <int MODE, typename T>
-__device__ inline void phase_A(const SolveArgsT<T> &a, T alpha) {
+__device__ inline void phase_A(const SolveArgsT<T> &a, const double *s_alpha, const int *s_state) {
     using V2 = typename Vec2<T>::type;
     const int stride = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
+        const int sy = a.rsys[i];
+        if (s_state[sy] != S_ACTIVE) continue;
+        const T alpha = (T)s_alpha[sy];
         T f0 = T(0), f1 = T(0), f2 = T(0), f3 = T(0), f4 = T(0), f5 = T(0);
         uint32_t e = a.adjStart[i];
         const uint32_t e1 = a.adjStart[i + 1];
@@ -211,7 +217,7 @@ __device__ inline void phase_A(const SolveArgsT<T> &a, T alpha) {
         const T mb[6] = {m01.x, m01.y, m23.x, m23.y, m45.x, m45.y};
         const T f[6] = {f0, f1, f2, f3, f4, f5};
         T u[6];
-        apply_M(mb, f, a.mono, u);
+        apply_M(mb, f, a.smono[sy], u);
         V2 *U2 = reinterpret_cast<V2 *>(a.U + 6 * (size_t)i);
         U2[0] = mk2<T>(u[0], u[1]);
         U2[1] = mk2<T>(u[2], u[3]);
@@ -246,21 +252,26 @@ enum : int { B_INIT = 0, B_ALPHA0 = 1, B_ITER = 2 };
 // phase B: constraints in 512-element chunks, one chunk per 256-lane group (element e of the chunk
 // is lane e % 256, pass e / 256); chunk partials go to part[q * nchunks + c]
 template <int MODE, typename T>
-__device__ inline void phase_B(const SolveArgsT<T> &a, T alpha, double *sh) {
+__device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, const int *s_state, double *sh) {
     const int g = threadIdx.x >> 8, lane = threadIdx.x & 255;
     const int ngroups = gridDim.x * kGrp;
     const T dt = (T)a.dt;
     for (int base = blockIdx.x * kGrp; base < a.nchunks; base += ngroups) {
         const int c = base + g;
         double s0 = 0.0, s1 = 0.0, s2 = 0.0, rmax = 0.0, nf = 0.0, rc = 0.0, nfc = 0.0;
-        if (c < a.nchunks) {
+        const int sy = c < a.nchunks ? a.chunk_sys[c] : 0;
+        const bool act = c < a.nchunks && s_state[sy] == S_ACTIVE;
+        if (act) {
+            const T alpha = (T)s_alpha[sy];
+            const uint32_t k0 = a.con_off[sy] + (uint32_t)(c - (int)a.chunk_off[sy]) * kRedChunk;
+            const uint32_t kend = a.con_off[sy + 1];
             T xo[kRedPasses], go[kRedPasses], du[kRedPasses];
             uint32_t kk[kRedPasses];
             bool ok[kRedPasses];
 #pragma unroll
             for (int p = 0; p < kRedPasses; ++p) {
-                kk[p] = (uint32_t)c * kRedChunk + p * kRedThreads + lane;
-                ok[p] = kk[p] < a.m;
+                kk[p] = k0 + p * kRedThreads + lane;
+                ok[p] = kk[p] < kend;
                 if (ok[p]) {
                     const typename Vec2<T>::type xg = a.xg[kk[p]];
                     xo[p] = xg.x;
@@ -311,7 +322,7 @@ __device__ inline void phase_B(const SolveArgsT<T> &a, T alpha, double *sh) {
         group_maxes<1>(vx, sh, tx);
         double *pc = a.part + c;
         const int nc = a.nchunks;
-        if (c < nc && lane == 0) {
+        if (act && lane == 0) {
             if (MODE != B_INIT) {
                 pc[(size_t)Q_SS * nc] = ts[0];
                 pc[(size_t)Q_SY * nc] = ts[1];
@@ -326,7 +337,7 @@ __device__ inline void phase_B(const SolveArgsT<T> &a, T alpha, double *sh) {
             double v2[1] = {nfc}, t2[1], v3[1] = {rc}, t3[1];
             group_sums<1>(v2, sh, t2);
             group_maxes<1>(v3, sh, t3);
-            if (c < nc && lane == 0) {
+            if (act && lane == 0) {
                 pc[(size_t)Q_RC * nc] = t3[0];
                 pc[(size_t)Q_NFC * nc] = t2[0];
             }
@@ -334,99 +345,145 @@ __device__ inline void phase_B(const SolveArgsT<T> &a, T alpha, double *sh) {
     }
 }
 
-// out: [0] iters, [1] matvecs, [2] converged, [3] fallback, [4] status, [5] res0, [6] res,
-//      [7] alpha at exit
+// Per-system report words in sysout[8 s ..]: iters, matvecs, converged, fallback, status, res0, res,
+// alpha at exit.  All blocks hold identical per-system state, so control flow stays uniform.
 template <typename T>
 __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     __shared__ double sh[kGrp * 8 * 4];
     __shared__ int s_flag;
+    __shared__ double s_alpha[kMaxSys], s_res0[kMaxSys], s_res[kMaxSys];
+    __shared__ int s_state[kMaxSys], s_iters[kMaxSys], s_mv[kMaxSys], s_fb[kMaxSys];
+    __shared__ double tot[5][kMaxSys];
+    __shared__ int s_any;
+    const int ns = a.nsys;
+    for (int q = threadIdx.x; q < ns; q += blockDim.x) {
+        s_alpha[q] = 0.0;
+        s_state[q] = S_ACTIVE;
+        s_iters[q] = 0;
+        s_mv[q] = 1;
+        s_fb[q] = 0;
+        s_res0[q] = 0.0;
+        s_res[q] = 0.0;
+    }
+    __syncthreads();
     uint32_t epoch = 0;
-    double alpha = 0.0;
-    int iters = 0, matvecs = 1, converged = 0, fallback = 0, status = 0;
-    double res0 = 0.0, res = 0.0;
 
     // start point: x = P(x0), g = A x + b
-    phase_A<A_PROJ, T>(a, T(0));
+    phase_A<A_PROJ, T>(a, s_alpha, s_state);
     if (!grid_barrier(a.bar, epoch, &s_flag)) return;
-    phase_B<B_INIT, T>(a, T(0), sh);
+    phase_B<B_INIT, T>(a, s_alpha, s_state, sh);
     if (!grid_barrier(a.bar, epoch, &s_flag)) return;
-    double R0, NF0;
     {
         const int S[1] = {Q_NF}, X[1] = {Q_RMAX};
-        double ts[1], tx[1];
-        fold_partials<1, 1>(a.part, a.nchunks, S, X, sh, ts, tx);
-        R0 = tx[0];
-        NF0 = ts[0];
+        fold_systems<1, 1>(a.part, a.nchunks, a.chunk_off, ns, s_state, S, X, sh, tot);
     }
     if (a.warm) {
+        // tot[0] = NF0, tot[1] = R0 -> keep them in tot[2], tot[3]
+        for (int q = threadIdx.x; q < ns; q += blockDim.x) {
+            tot[2][q] = tot[0][q];
+            tot[3][q] = tot[1][q];
+        }
+        __syncthreads();
         const int S[1] = {Q_NFC}, X[1] = {Q_RC};
-        double ts[1], tx[1];
-        fold_partials<1, 1>(a.part, a.nchunks, S, X, sh, ts, tx);
-        const double Rc = tx[0], NFc = ts[0];
-        if (NF0 > 0.0 || R0 > Rc) {
-            // warm start rejected: x = 0, g = 0 + b
+        fold_systems<1, 1>(a.part, a.nchunks, a.chunk_off, ns, s_state, S, X, sh, tot);
+        // tot[0] = NFc, tot[1] = Rc
+        if (threadIdx.x == 0) s_any = 0;
+        __syncthreads();
+        for (int q = threadIdx.x; q < ns; q += blockDim.x) {
+            if (tot[2][q] > 0.0 || tot[3][q] > tot[1][q]) {
+                s_fb[q] = 1;
+                tot[2][q] = tot[0][q];
+                tot[3][q] = tot[1][q];
+                s_any = 1;
+            }
+        }
+        __syncthreads();
+        if (s_any) {
+            // warm start rejected for those systems: x = 0, g = 0 + b
             for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < a.m; k += gridDim.x * blockDim.x)
-                a.xg[k] = mk2<T>(T(0), T(0) + a.b[k]);
-            R0 = Rc;
-            NF0 = NFc;
-            fallback = 1;
+                if (s_fb[a.rsys[a.cI[k]]]) a.xg[k] = mk2<T>(T(0), T(0) + a.b[k]);
             if (!grid_barrier(a.bar, epoch, &s_flag)) return;
         }
+        for (int q = threadIdx.x; q < ns; q += blockDim.x) {
+            tot[0][q] = tot[2][q];
+            tot[1][q] = tot[3][q];
+        }
+        __syncthreads();
     }
-    res0 = R0;
-    res = R0;
-    if (NF0 > 0.0) {
-        status = ALENS_E_NONFINITE;
-    } else if (R0 <= a.tol) {
-        converged = 1;
-    } else {
+    if (threadIdx.x == 0) s_any = 0;
+    __syncthreads();
+    for (int q = threadIdx.x; q < ns; q += blockDim.x) {
+        s_res0[q] = tot[1][q];
+        s_res[q] = tot[1][q];
+        if (tot[0][q] > 0.0)
+            s_state[q] = S_NONFINITE;
+        else if (tot[1][q] <= a.tol)
+            s_state[q] = S_CONVERGED;
+        else
+            s_any = 1;
+    }
+    __syncthreads();
+    if (s_any) {
         // alpha0 = g^T g / g^T A g
-        phase_A<A_GRAD, T>(a, T(0));
+        phase_A<A_GRAD, T>(a, s_alpha, s_state);
         if (!grid_barrier(a.bar, epoch, &s_flag)) return;
-        phase_B<B_ALPHA0, T>(a, T(0), sh);
+        phase_B<B_ALPHA0, T>(a, s_alpha, s_state, sh);
         if (!grid_barrier(a.bar, epoch, &s_flag)) return;
-        ++matvecs;
         {
             const int S[2] = {Q_SS, Q_SY}, X[1] = {Q_RMAX};
-            double ts[2], tx[1];
-            fold_partials<2, 1>(a.part, a.nchunks, S, X, sh, ts, tx);
-            alpha = ts[0] / ts[1];
+            fold_systems<2, 1>(a.part, a.nchunks, a.chunk_off, ns, s_state, S, X, sh, tot);
         }
-        if (!(alpha > 0.0 && alpha < INFINITY)) alpha = 1.0;
-        status = ALENS_E_NOCONV;
+        for (int q = threadIdx.x; q < ns; q += blockDim.x) {
+            if (s_state[q] != S_ACTIVE) continue;
+            ++s_mv[q];
+            double al = tot[0][q] / tot[1][q];
+            if (!(al > 0.0 && al < INFINITY)) al = 1.0;
+            s_alpha[q] = al;
+        }
+        __syncthreads();
         for (int it = 1; it <= a.max_ite; ++it) {
-            phase_A<A_PROJ, T>(a, (T)alpha);
+            phase_A<A_PROJ, T>(a, s_alpha, s_state);
             if (!grid_barrier(a.bar, epoch, &s_flag)) return;
-            phase_B<B_ITER, T>(a, (T)alpha, sh);
+            phase_B<B_ITER, T>(a, s_alpha, s_state, sh);
             if (!grid_barrier(a.bar, epoch, &s_flag)) return;
-            ++matvecs;
-            const int S[4] = {Q_SS, Q_SY, Q_YY, Q_NF}, X[1] = {Q_RMAX};
-            double ts[4], tx[1];
-            fold_partials<4, 1>(a.part, a.nchunks, S, X, sh, ts, tx);
-            iters = it;
-            res = tx[0];
-            if (ts[3] > 0.0) {
-                status = ALENS_E_NONFINITE;
-                break;
+            {
+                const int S[4] = {Q_SS, Q_SY, Q_YY, Q_NF}, X[1] = {Q_RMAX};
+                fold_systems<4, 1>(a.part, a.nchunks, a.chunk_off, ns, s_state, S, X, sh, tot);
             }
-            if (tx[0] <= a.tol) {
-                converged = 1;
-                status = 0;
-                break;
+            if (threadIdx.x == 0) s_any = 0;
+            __syncthreads();
+            for (int q = threadIdx.x; q < ns; q += blockDim.x) {
+                if (s_state[q] != S_ACTIVE) continue;
+                ++s_mv[q];
+                s_iters[q] = it;
+                s_res[q] = tot[4][q];
+                if (tot[3][q] > 0.0) {
+                    s_state[q] = S_NONFINITE;
+                } else if (tot[4][q] <= a.tol) {
+                    s_state[q] = S_CONVERGED;
+                } else {
+                    const double an = (it & 1) ? tot[0][q] / tot[1][q] : tot[1][q] / tot[2][q];
+                    if (an > 0.0 && an < INFINITY) s_alpha[q] = an;
+                    s_any = 1;
+                }
             }
-            const double an = (it & 1) ? ts[0] / ts[1] : ts[1] / ts[2];
-            if (an > 0.0 && an < INFINITY) alpha = an;
+            __syncthreads();
+            if (!s_any) break;
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.out[0] = (double)iters;
-        a.out[1] = (double)matvecs;
-        a.out[2] = (double)converged;
-        a.out[3] = (double)fallback;
-        a.out[4] = (double)status;
-        a.out[5] = res0;
-        a.out[6] = res;
-        a.out[7] = alpha;
+    if (blockIdx.x == 0) {
+        for (int q = threadIdx.x; q < ns; q += blockDim.x) {
+            const int st = s_state[q];
+            double *o = a.out + 8 * (size_t)q;
+            o[0] = (double)s_iters[q];
+            o[1] = (double)s_mv[q];
+            o[2] = st == S_CONVERGED ? 1.0 : 0.0;
+            o[3] = (double)s_fb[q];
+            o[4] = st == S_CONVERGED ? 0.0 : (st == S_NONFINITE ? (double)ALENS_E_NONFINITE : (double)ALENS_E_NOCONV);
+            o[5] = s_res0[q];
+            o[6] = s_res[q];
+            o[7] = s_alpha[q];
+        }
     }
 }
 
@@ -492,13 +549,14 @@ HipErr launch_init_xg32(hipStream_t st, float2 *xg, const double2 *xg64, uint32_
 // ------------------------------------------------------------------------------------------------
 // write-back: F_con = D gamma (gamma = solver x), U = U_nc + M F_con; gamma copied out
 // ------------------------------------------------------------------------------------------------
-__global__ void k_writeback(int32_t n, const uint32_t *__restrict__ adjStart,
-                            const uint32_t *__restrict__ adjK, const double *__restrict__ adjW,
-                            const double2 *__restrict__ xg, const double *__restrict__ mob,
-                            const double *__restrict__ unc, int mono, double *__restrict__ F,
-                            double *__restrict__ U) {
+__global__ void k_writeback(int32_t n, const int32_t *__restrict__ rsys, const SysDev *__restrict__ sys,
+                            const uint32_t *__restrict__ adjStart, const uint32_t *__restrict__ adjK,
+                            const double *__restrict__ adjW, const double2 *__restrict__ xg,
+                            const double *__restrict__ mob, const double *__restrict__ unc,
+                            double *__restrict__ F, double *__restrict__ U) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const int mono = sys[rsys[i]].mono;
     double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (uint32_t e = adjStart[i]; e < adjStart[i + 1]; ++e) {
         const uint32_t k = adjK[e] & 0x7fffffffu;
@@ -523,12 +581,11 @@ __global__ void k_copy_gamma(const double2 *__restrict__ xg, double *__restrict_
     if (k < m) gamma[k] = xg[k].x;
 }
 
-HipErr launch_writeback(hipStream_t st, int32_t n, const uint32_t *adjStart, const uint32_t *adjK,
-                        const double *adjW, const double2 *xg, const double *mob,
-                        const double *unc, int mono, double *F, double *U, double *gamma,
-                        uint32_t m) {
+HipErr launch_writeback(hipStream_t st, int32_t n, const int32_t *rsys, const SysDev *sys, const uint32_t *adjStart,
+                        const uint32_t *adjK, const double *adjW, const double2 *xg, const double *mob,
+                        const double *unc, double *F, double *U, double *gamma, uint32_t m) {
     if (n > 0) {
-        k_writeback<<<nblk(n, 256), 256, 0, st>>>(n, adjStart, adjK, adjW, xg, mob, unc, mono, F, U);
+        k_writeback<<<nblk(n, 256), 256, 0, st>>>(n, rsys, sys, adjStart, adjK, adjW, xg, mob, unc, F, U);
         LAUNCH_CHECK();
     }
     if (m > 0) {
@@ -542,10 +599,12 @@ HipErr launch_writeback(hipStream_t st, int32_t n, const uint32_t *adjStart, con
 // Euler step: x += v dt; direction rotated by normalise(1, w dt / 2) and renormalised; periodic
 // coordinates wrapped into [lo, hi).
 // ------------------------------------------------------------------------------------------------
-__global__ void k_euler(int32_t n, double *__restrict__ center, double *__restrict__ dir,
-                        const double *__restrict__ u, CfgDev c) {
+__global__ void k_euler(int32_t n, const int32_t *__restrict__ rsys, const SysDev *__restrict__ sys,
+                        double *__restrict__ center, double *__restrict__ dir, const double *__restrict__ u,
+                        CfgDev c) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const SysDev &S = sys[rsys[i]];
     const double dt = c.dt;
     const double hdt = 0.5 * dt;
     const double *ui = u + 6 * (size_t)i;
@@ -575,15 +634,15 @@ __global__ void k_euler(int32_t n, double *__restrict__ center, double *__restri
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         double v = x[k];
-        if (c.pbc[k]) v = v - floor((v - c.box_lo[k]) / c.len[k]) * c.len[k];
+        if (S.pbc[k]) v = v - floor((v - S.box_lo[k]) / S.len[k]) * S.len[k];
         cc[k] = v;
     }
 }
 
-HipErr launch_euler(hipStream_t st, int32_t n, double *center, double *dir, const double *u,
-                    const CfgDev &c) {
+HipErr launch_euler(hipStream_t st, int32_t n, const int32_t *rsys, const SysDev *sys, double *center, double *dir,
+                    const double *u, const CfgDev &c) {
     if (n == 0) return hip_ok();
-    k_euler<<<nblk(n, 256), 256, 0, st>>>(n, center, dir, u, c);
+    k_euler<<<nblk(n, 256), 256, 0, st>>>(n, rsys, sys, center, dir, u, c);
     LAUNCH_CHECK();
     return hip_ok();
 }

@@ -106,3 +106,25 @@ def write_rod_file(path, rods, header_time=0.0):
             t = "S" if (imm is not None and imm[k]) else "C"
             fh.write(f"{t} {int(rods['gid'][k])} {rods['radius'][k]:.17g} " +
                      " ".join(f"{x:.17g}" for x in (*mi[k], *pl[k])) + "\n")
+
+
+def replicas(nsys=64, nmin=2000, nmax=50000, seed=1234):
+    """Independent replica systems with rod counts log-uniform in [nmin, nmax]; even-numbered replicas
+    are periodic 3-D boxes at 64 rods/um^3 (lognormal lengths, median 1 um, sigma 0.3, resampled above
+    1.5 um so that the smallest boxes satisfy the periodic-image condition), odd-numbered ones are
+    quasi-2D slabs at 20 rods/um^2 (L = 1 um, nematic kappa = 4).  gids are unique across replicas so
+    every replica draws its own Brownian noise.  Returns ([(rods, box_cfg)], common cfg)."""
+    rng = np.random.default_rng(seed)
+    ns = np.round(np.exp(rng.uniform(np.log(nmin), np.log(nmax), nsys))).astype(int)
+    out = []
+    off = 0
+    for s, n in enumerate(ns):
+        if s % 2 == 0:
+            rods, cfg = lognormal_3d(n=int(n), box=(n / 64.0) ** (1 / 3), lmax=1.5, seed=seed + 7 * s + 1)
+        else:
+            rods, cfg = monolayer_slab(n=int(n), side=float(np.sqrt(n / 20.0)), seed=seed + 7 * s + 2)
+        rods = dict(rods, gid=rods["gid"] + off)
+        off += int(n)
+        out.append((rods, dict(box_lo=cfg["box_lo"], box_hi=cfg["box_hi"], pbc=cfg["pbc"],
+                               monolayer=cfg["monolayer"])))
+    return out, dict(COMMON, box_lo=(0, 0, 0), box_hi=(1, 1, 1), pbc=(0, 0, 0), seed=seed)

@@ -46,12 +46,16 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="monolayer_slab",
-                    choices=["monolayer_slab", "lognormal_3d", "small_3d"])
+                    choices=["monolayer_slab", "lognormal_3d", "small_3d", "replicas"])
+    ap.add_argument("--precision", type=int, default=0, help="1: FP32 BBPGD + FP64 refinement for the timed pass")
     ap.add_argument("--n", type=int, default=0, help="rod count (default: the workload's own)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-form", action="store_true", help="skip the chosen-form (FP32 + warm start) pass")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--progress", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="before the run, one device-to-device copy of this many MiB (a kernel of known "
+                         "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -60,11 +64,22 @@ def main():
 
     from alens_amd import capi, systems
 
+    if args.calib_copy_mib:
+        import torch
+
+        nb = args.calib_copy_mib << 20
+        src = torch.empty(nb, dtype=torch.uint8, device="cuda").fill_(1)
+        dst = torch.empty_like(src)
+        torch.add(src, 1, out=dst)  # elementwise: reads nb bytes, writes nb bytes
+        torch.cuda.synchronize()
+        del src, dst
+    if args.workload == "replicas":
+        return bench_replicas(args, capi, systems)
     gen = {"monolayer_slab": systems.monolayer_slab, "lognormal_3d": systems.lognormal_3d,
            "small_3d": systems.small_3d}[args.workload]
     rods, cfg = gen(n=args.n) if args.n else gen()
     n = len(rods["length"])
-    ctx = capi.Context(**cfg)
+    ctx = capi.Context(**dict(cfg, precision=args.precision))
     ctx.set_rods(rods)
     # spatial (Morton cell) order, as the reference's FDPS containers keep it; later arrays follow it
     perm = ctx.sort_rods()
@@ -186,6 +201,43 @@ def main():
                       f"{res['report']['iters']} BBPGD iterations, {tc:.2f} s; CPU {cpu_model()}",
         }
         out["gpu_over_cpu"] = value / (n / tc)
+    print(json.dumps(out), flush=True)
+
+
+def bench_replicas(args, capi, systems):
+    """64 independent replicas (rod counts log-uniform in [2k, 50k], 3-D boxes and slabs) stepped
+    together: one collection pass and one persistent solver launch per step for all of them."""
+    parts, cfg = systems.replicas()
+    ctx = capi.Context(**dict(cfg, precision=args.precision, warm_start=1))
+    offs = ctx.set_systems(parts)
+    ctx.sort_rods()
+    n = int(offs[-1])
+    for w in range(args.warmup):
+        ctx.step()
+        ctx.euler()
+    reps, sreps = [], []
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        reps.append(ctx.step())
+        sreps.append(ctx.system_reports())
+        ctx.euler()
+    el = time.perf_counter() - t0
+    K = args.steps
+    it = np.array([[r["iters"] + r["iters_fp32"] for r in sr] for sr in sreps])  # steps x replicas
+    hist, edges = np.histogram(it.mean(0), bins=[0, 25, 50, 100, 200, 400, 800, 1600, 1e9])
+    out = {
+        "metric": "rod-steps/sec at BBPGD residual 1e-5 (64 replicas in one launch per step)",
+        "value": n * K / el, "unit": "rod-steps/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if args.precision else "f64", "data": "synthetic",
+        "config": {"workload": f"replicas: {len(parts)} systems, {n} rods, warm start from previous gamma",
+                   "rods": n, "replicas": len(parts)},
+        "constraints_per_step": float(np.mean([r["n_con"] for r in reps])),
+        "solver_iters_max_per_step": float(np.mean([r["iters"] for r in reps])),
+        "replica_iters_histogram": {"edges": [float(e) for e in edges[:-1]], "counts": hist.tolist()},
+        "t_solve_ms_mean": float(np.mean([r["t_solve_ms"] for r in reps])),
+        "t_collect_ms_mean": float(np.mean([r["t_collect_ms"] for r in reps])),
+    }
     print(json.dumps(out), flush=True)
 
 

@@ -97,12 +97,29 @@ alens_status alens_set_rods(alens_ctx *ctx, int32_t n, const int32_t *gid, const
                             const uint8_t *immovable);
 alens_status alens_get_rods(alens_ctx *ctx, double *center, double *dir);
 
+/* Many independent rod systems in one context (the batched form): system s owns rods
+ * [rod_offset[s], rod_offset[s+1]) of the concatenated arrays and its own box / periodicity /
+ * monolayer flag; all other settings come from the context's alens_config.  One step collects and
+ * solves every system together (one persistent solver launch with per-system step lengths and
+ * convergence); constraints never couple systems.  alens_set_rods is the one-system case. */
+typedef struct {
+    double box_lo[3], box_hi[3];
+    int32_t pbc[3];
+    int32_t monolayer;
+} alens_box;
+alens_status alens_set_systems(alens_ctx *ctx, int32_t nsys, const int32_t *rod_offset, const alens_box *boxes,
+                               const int32_t *gid, const double *center, const double *dir, const double *length,
+                               const double *radius, const uint8_t *immovable);
+alens_status alens_get_num_systems(const alens_ctx *ctx, int32_t *nsys);
+/* per-system solve reports of the last step (nsys entries; iters_fp32 filled for precision = 1) */
+alens_status alens_get_system_reports(alens_ctx *ctx, alens_solve_report *reps);
+
 /* Reorder the rods on the device by the Morton code of their centre's cell so that neighbours sit
  * close in memory (the reference keeps its particle arrays in FDPS tree order for the same reason;
  * repartitioning happens in prepareStep, SRC/TubuleSystem.cpp:120-125).  perm_out (n, may be NULL)
  * receives, for each new index i, the rod's previous index.  Rod and constraint indices refer to the
  * new order afterwards; the previous-step impulse cache is cleared. */
-alens_status alens_sort_rods(alens_ctx *ctx, int32_t *perm_out);
+alens_status alens_sort_rods(alens_ctx *ctx, int32_t *perm_out);  /* within each system */
 
 /* Step counter keying the Brownian stream (SylinderSystem's stepCount, SRC/TubuleSystem.cpp:114). */
 alens_status alens_set_step(alens_ctx *ctx, int64_t step);
