@@ -23,7 +23,8 @@ EXPORTS = [
     "alens_sort_rods", "alens_set_systems", "alens_get_num_systems", "alens_get_system_reports",
     "alens_set_step", "alens_get_step", "alens_step_constraints", "alens_step_euler",
     "alens_get_num_constraints", "alens_get_constraints", "alens_get_constraint_arrays", "alens_get_gamma",
-    "alens_get_velocity", "alens_bcqp_solve",
+    "alens_get_velocity", "alens_bcqp_solve", "alens_add_constraints", "alens_clear_constraints",
+    "alens_get_con_stress",
     "alens_mlp_set", "alens_mlp_get", "alens_get_features", "alens_mlp_predict", "alens_mlp_gradient",
     "alens_mlp_train",
 ]
@@ -308,6 +309,44 @@ class Context:
         mm = ctypes.c_int64()
         self._check(lib().alens_get_constraints(self._h, m, arr, ctypes.byref(mm)))
         return arr[:m]
+
+    def add_constraints(self, ext):
+        """Queue caller constraint blocks for the next step (the reference's constraint pool).  ext is a
+        dict of arrays: gid_i, gid_j, delta0, gamma0, nrm (m x 3, = norm_i; norm_j = -nrm), posI, posJ
+        (m x 3), bilateral, kappa, gamma_lb, and optionally labI, labJ (m x 3), stress (m x 9)."""
+        m = len(ext["gid_i"])
+        arr = (ConBlock * max(m, 1))()
+        f3 = lambda key: np.asarray(ext[key], np.float64).reshape(m, 3) if key in ext else np.zeros((m, 3))
+        nrm, pI, pJ, lI, lJ = f3("nrm"), f3("posI"), f3("posJ"), f3("labI"), f3("labJ")
+        st = np.asarray(ext["stress"], np.float64).reshape(m, 9) if "stress" in ext else np.zeros((m, 9))
+        for e in range(m):
+            b = arr[e]
+            b.delta0 = float(ext["delta0"][e])
+            b.gamma = float(ext["gamma0"][e])
+            b.kappa = float(ext["kappa"][e])
+            b.gamma_lb = float(ext["gamma_lb"][e])
+            b.gid_i = int(ext["gid_i"][e])
+            b.gid_j = int(ext["gid_j"][e])
+            b.bilateral = int(bool(ext["bilateral"][e]))
+            for a in range(3):
+                b.norm_i[a] = nrm[e, a]
+                b.norm_j[a] = -nrm[e, a]
+                b.pos_i[a] = pI[e, a]
+                b.pos_j[a] = pJ[e, a]
+                b.lab_i[a] = lI[e, a]
+                b.lab_j[a] = lJ[e, a]
+            for q in range(9):
+                b.stress[q] = st[e, q]
+        self._check(lib().alens_add_constraints(self._h, m, arr))
+
+    def clear_constraints(self):
+        self._check(lib().alens_clear_constraints(self._h))
+
+    def con_stress(self):
+        """(collision, caller) sums of gamma * per-unit stress over the last step's rows, 3 x 3 each."""
+        c, e = np.zeros(9), np.zeros(9)
+        self._check(lib().alens_get_con_stress(self._h, _p(c, ctypes.c_double), _p(e, ctypes.c_double)))
+        return c.reshape(3, 3), e.reshape(3, 3)
 
     def gamma(self):
         m = self.num_constraints()

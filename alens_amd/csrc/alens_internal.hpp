@@ -126,12 +126,30 @@ HipErr launch_order_I(hipStream_t st, int32_t n, uint32_t m, const uint32_t *pI,
                       uint32_t *cur, int32_t *cI, int32_t *cJ, uint32_t *cImg);
 HipErr launch_order_J(hipStream_t st, int32_t n, uint32_t m, const int32_t *cJ,
                       const uint32_t *jStart, uint32_t *cur, uint32_t *jlist);
-HipErr launch_rows(hipStream_t st, const RodsDev &r, const CfgDev &c, uint32_t m, const int32_t *cI,
+// constraint geometry is stored component-major: geo[q * gs + k], q = 0..8 = {n, rI, rJ}, gs = the
+// step's total row count (collision rows first, then the caller's rows)
+HipErr launch_rows(hipStream_t st, const RodsDev &r, const CfgDev &c, uint32_t m, uint32_t gs, const int32_t *cI,
                    const int32_t *cJ, const uint32_t *cImg, const double *unc, double *delta0,
                    double *geo, double *posI, double *posJ, double *b);
-HipErr launch_adjacency(hipStream_t st, int32_t n, const uint32_t *rowStart,
-                        const uint32_t *jStart, const uint32_t *jlist, const double *geo,
-                        uint32_t *adjStart, uint32_t *adjK, double *adjW);
+HipErr launch_rows_ext(hipStream_t st, uint32_t me, uint32_t k0, uint32_t gs, const int32_t *cI, const int32_t *cJ,
+                       const double *nrm, const double *posI, const double *posJ, const double *delta0,
+                       const double *unc, double dt, double *geo, double *b);
+
+// Rod-major adjacency (column blocks of D) in sliced, wave-interleaved form: rods are taken in
+// slices of kSlice consecutive indices; entry j of rod i (ascending constraint index, J-side entries
+// first) sits at sellStart[i / kSlice] + j * kSlice + i % kSlice, so the kSlice lanes of a wave that
+// walk their rods' entries in step read consecutive words.  sellK holds the constraint index (top
+// bit set on J-side entries); sellW holds the six weights component-major, sellW[c * epad + idx]:
+// (n, rI) on the I side, (-n, -rJ) on the J side.  adjStart keeps the CSR row starts (degrees).
+// Caller rows (k = mc + e) follow a rod's collision entries: extStart (n + 1, or NULL) / extEnt
+// list, per rod in ascending e, e with the top bit set on the J side.
+constexpr int kSlice = 64;
+HipErr launch_slice_sizes(hipStream_t st, int32_t n, const uint32_t *rowStart, const uint32_t *jStart,
+                          const uint32_t *extStart, uint32_t *slice_len);
+HipErr launch_adjacency(hipStream_t st, int32_t n, uint32_t m, const uint32_t *rowStart,
+                        const uint32_t *jStart, const uint32_t *jlist, const uint32_t *extStart,
+                        const uint32_t *extEnt, uint32_t mc, const double *geo, const uint32_t *sellStart,
+                        uint64_t epad, uint32_t *adjStart, uint32_t *sellK, double *sellW);
 
 // solve kernels (k_solve.hip)
 template <typename T>
@@ -152,15 +170,18 @@ template <typename T>
 struct SolveArgsT {
     int32_t n;
     uint32_t m;
-    const uint32_t *adjStart;
-    const uint32_t *adjK;
-    const T *adjW;
+    const uint32_t *adjStart;    // CSR row starts (rod degrees)
+    const uint32_t *sellStart;   // sliced adjacency (see launch_adjacency)
+    const uint32_t *sellK;
+    const T *sellW;
+    uint64_t epad;               // component stride of sellW
     const T *mob;
     const int32_t *cI;
     const int32_t *cJ;
-    const T *geo;
+    const T *geo;                // component-major, stride m
     const T *b;
     const T *lb;                 // NULL = all zero
+    const T *invk;               // NULL = no compliance; else A += diag(invk)
     typename Vec2<T>::type *xg;  // (x, g) per constraint; x holds the start point on entry
     T *U;                        // 6n scratch
     double *part;                // kRedQ * nchunks partials
@@ -184,11 +205,13 @@ HipErr launch_to_f32(hipStream_t st, const double *src, float *dst, size_t count
 HipErr launch_xg32_to_x64(hipStream_t st, const float2 *xg32, double *x64, uint32_t m);
 HipErr launch_init_xg32(hipStream_t st, float2 *xg, const double2 *xg64, uint32_t m);
 HipErr launch_writeback(hipStream_t st, int32_t n, const int32_t *rsys, const SysDev *sys, const uint32_t *adjStart,
-                        const uint32_t *adjK, const double *adjW, const double2 *xg, const double *mob,
-                        const double *unc, double *F, double *U, double *gamma, uint32_t m);
+                        const uint32_t *sellStart, const uint32_t *sellK, const double *sellW, uint64_t epad,
+                        const double2 *xg, const double *mob, const double *unc, double *F, double *U,
+                        double *gamma, uint32_t m);
 HipErr launch_euler(hipStream_t st, int32_t n, const int32_t *rsys, const SysDev *sys, double *center, double *dir,
                     const double *u, const CfgDev &c);
 HipErr launch_init_xg(hipStream_t st, double2 *xg, const double *x0, uint32_t m);
+HipErr launch_set_x(hipStream_t st, double2 *xg, const double *x0, uint32_t me);
 HipErr launch_warm_match(hipStream_t st, uint32_t m, const int32_t *cI, const int32_t *cJ,
                          const int32_t *gid, uint32_t mprev, const uint64_t *prev_key,
                          const double *prev_gamma, double2 *xg);
@@ -206,7 +229,7 @@ constexpr int kMlpNorm = 2 * kMlpIn + 1;  // mean[12], 1/std[12], gamma scale
 struct MlpScratch {
     float *XT, *Z1, *H1, *Z2, *H2, *Z3, *H3, *dO, *dZ1, *dZ2, *dZ3, *loss, *lossv;
 };
-HipErr launch_features(hipStream_t st, uint32_t m, const int32_t *cI, const int32_t *cJ, const double *delta0,
+HipErr launch_features(hipStream_t st, uint32_t m, uint32_t gs, const int32_t *cI, const int32_t *cJ, const double *delta0,
                        const double *geo, const double *posI, const double *posJ, const double *dir,
                        const double *length, const double *unc, const double2 *xprev, double dt, float *X);
 HipErr launch_mlp_infer_f32(hipStream_t st, uint32_t m, const float *X, const float *P, const float *norm,

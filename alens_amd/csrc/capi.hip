@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "alens_dev.hpp"
@@ -39,6 +40,8 @@ struct alens_ctx {
     std::vector<alens_solve_report> sysrep;
 
     std::vector<int32_t> h_gid;
+    std::unordered_map<int32_t, int32_t> gid_index;  // gid -> rod index (built on demand)
+    bool gid_index_ok = false;
     DBuf<int32_t> gid;
     DBuf<double> center, dir, length, radius, hcol, rcol;
     DBuf<uint8_t> immov;
@@ -55,13 +58,20 @@ struct alens_ctx {
     DBuf<int32_t> cI, cJ;
     DBuf<uint32_t> cImg;
     DBuf<double> delta0, geo, posI, posJ, b;
-    DBuf<uint32_t> adjStart, adjK;
-    DBuf<double> adjW;
+    DBuf<uint32_t> adjStart, sellStart, sellK, slen;
+    DBuf<double> sellW;
+    uint64_t epad = 0;  // padded entry count of the sliced adjacency (component stride of sellW)
 
     DBuf<double2> xg;
     DBuf<double> gamma, part, out;
+    // caller rows (the constraint pool): pending for the next step, and those of the last step
+    std::vector<alens_con_block> ext_pending, ext_step;
+    DBuf<uint32_t> extStart, extEnt;
+    DBuf<double> extNrm, extX0, lb, invk;
+    DBuf<float> lb32, invk32;
+    uint32_t mcol = 0, mext = 0;  // collision rows, caller rows of the last step
     // precision form: FP32 copies of the operator data and the FP32 iterate
-    DBuf<float> geo32, adjW32, mob32, b32, U32;
+    DBuf<float> geo32, sellW32, mob32, b32, U32;
     DBuf<float2> xg32;
     // learned warm start: FP32 master parameters, normalisation, features of the step
     DBuf<float> mlpP, mlpNorm, feat;
@@ -211,9 +221,10 @@ void alens_destroy(alens_ctx *ctx) {
     rel(ctx->counter); rel(ctx->pI); rel(ctx->pJ); rel(ctx->pImg); rel(ctx->cntI); rel(ctx->cntJ);
     rel(ctx->rowStart); rel(ctx->jStart); rel(ctx->cur); rel(ctx->jlist); rel(ctx->cI); rel(ctx->cJ);
     rel(ctx->cImg); rel(ctx->delta0); rel(ctx->geo); rel(ctx->posI); rel(ctx->posJ); rel(ctx->b);
-    rel(ctx->adjStart); rel(ctx->adjK); rel(ctx->adjW); rel(ctx->xg); rel(ctx->gamma); rel(ctx->part);
+    rel(ctx->adjStart); rel(ctx->sellStart); rel(ctx->sellK); rel(ctx->slen); rel(ctx->sellW); rel(ctx->xg); rel(ctx->gamma); rel(ctx->part);
     rel(ctx->out); rel(ctx->bar); rel(ctx->pkey); rel(ctx->pgam);
-    rel(ctx->geo32); rel(ctx->adjW32); rel(ctx->mob32); rel(ctx->b32); rel(ctx->U32); rel(ctx->xg32);
+    rel(ctx->extStart); rel(ctx->extEnt); rel(ctx->extNrm); rel(ctx->extX0); rel(ctx->lb); rel(ctx->invk);
+    rel(ctx->lb32); rel(ctx->invk32); rel(ctx->geo32); rel(ctx->sellW32); rel(ctx->mob32); rel(ctx->b32); rel(ctx->U32); rel(ctx->xg32);
     rel(ctx->mlpP); rel(ctx->mlpNorm); rel(ctx->feat); rel(ctx->xprev);
     rel(ctx->sysd); rel(ctx->rsys); rel(ctx->smono); rel(ctx->chunk_sys); rel(ctx->con_off); rel(ctx->chunk_off);
     for (auto &e : ctx->ev)
@@ -351,6 +362,7 @@ alens_status alens_set_systems(alens_ctx *ctx, int32_t nsys, const int32_t *rod_
     std::vector<int32_t> ng(gid, gid + n);
     if (ng != ctx->h_gid || nsys != ctx->nsys) ctx->have_prev = false;
     ctx->h_gid.swap(ng);
+    ctx->gid_index_ok = false;
     TRY_HIP(hipStreamSynchronize(ctx->st));
     ctx->n = n;
     ctx->nsys = nsys;
@@ -463,6 +475,7 @@ alens_status alens_sort_rods(alens_ctx *ctx, int32_t *perm_out) {
         if (perm_out) perm_out[i] = (int32_t)hp[i];
     }
     ctx->h_gid.swap(ng);
+    ctx->gid_index_ok = false;
     ctx->have_prev = false;
     ctx->have_step = false;
     return ALENS_OK;
@@ -570,14 +583,17 @@ static alens_status run_solver(alens_ctx *ctx, int32_t n, uint32_t m, const doub
     a.n = n;
     a.m = m;
     a.adjStart = ctx->adjStart.p;
-    a.adjK = ctx->adjK.p;
-    a.adjW = ctx->adjW.p;
+    a.sellStart = ctx->sellStart.p;
+    a.sellK = ctx->sellK.p;
+    a.sellW = ctx->sellW.p;
+    a.epad = ctx->epad;
     a.mob = mob;
     a.cI = ctx->cI.p;
     a.cJ = ctx->cJ.p;
     a.geo = ctx->geo.p;
     a.b = ctx->b.p;
-    a.lb = nullptr;
+    a.lb = ctx->mext ? ctx->lb.p : nullptr;
+    a.invk = ctx->mext ? ctx->invk.p : nullptr;
     a.xg = ctx->xg.p;
     a.U = ctx->Us.p;
     a.rsys = rsys;
@@ -593,28 +609,37 @@ static alens_status run_solver_mixed(alens_ctx *ctx, int32_t n, uint32_t m, bool
     hipStream_t st = ctx->st;
     const size_t M = std::max<uint32_t>(m, 1), N = (size_t)std::max(n, 1);
     TRY_HIP(ctx->geo32.ensure(9 * M));
-    TRY_HIP(ctx->adjW32.ensure(12 * M));
+    TRY_HIP(ctx->sellW32.ensure(6 * std::max<uint64_t>(ctx->epad, 1)));
     TRY_HIP(ctx->b32.ensure(M));
     TRY_HIP(ctx->xg32.ensure(M));
     TRY_HIP(ctx->mob32.ensure(6 * N));
     TRY_HIP(ctx->U32.ensure(6 * N));
     TRY_K(launch_to_f32(st, ctx->geo.p, ctx->geo32.p, 9 * (size_t)m));
-    TRY_K(launch_to_f32(st, ctx->adjW.p, ctx->adjW32.p, 12 * (size_t)m));
+    TRY_K(launch_to_f32(st, ctx->sellW.p, ctx->sellW32.p, 6 * (size_t)ctx->epad));
     TRY_K(launch_to_f32(st, ctx->b.p, ctx->b32.p, (size_t)m));
     TRY_K(launch_to_f32(st, ctx->mob.p, ctx->mob32.p, 6 * (size_t)n));
+    if (ctx->mext) {
+        TRY_HIP(ctx->lb32.ensure(M));
+        TRY_HIP(ctx->invk32.ensure(M));
+        TRY_K(launch_to_f32(st, ctx->lb.p, ctx->lb32.p, (size_t)m));
+        TRY_K(launch_to_f32(st, ctx->invk.p, ctx->invk32.p, (size_t)m));
+    }
     TRY_K(launch_init_xg32(st, ctx->xg32.p, ctx->xg.p, m));
     SolveArgsT<float> a{};
     a.n = n;
     a.m = m;
     a.adjStart = ctx->adjStart.p;
-    a.adjK = ctx->adjK.p;
-    a.adjW = ctx->adjW32.p;
+    a.sellStart = ctx->sellStart.p;
+    a.sellK = ctx->sellK.p;
+    a.sellW = ctx->sellW32.p;
+    a.epad = ctx->epad;
     a.mob = ctx->mob32.p;
     a.cI = ctx->cI.p;
     a.cJ = ctx->cJ.p;
     a.geo = ctx->geo32.p;
     a.b = ctx->b32.p;
-    a.lb = nullptr;
+    a.lb = ctx->mext ? ctx->lb32.p : nullptr;
+    a.invk = ctx->mext ? ctx->invk32.p : nullptr;
     a.xg = ctx->xg32.p;
     a.U = ctx->U32.p;
     a.rsys = ctx->rsys.p;
@@ -646,6 +671,52 @@ alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_s
     const int32_t n = ctx->n;
     const CfgDev cd = make_cfgdev(ctx);
     const RodsDev rd = make_rods(ctx);
+    // caller rows: rods by gid, per-rod entry lists in ascending row order
+    const uint32_t Me = (uint32_t)ctx->ext_pending.size();
+    std::vector<int32_t> eI(Me), eJ(Me);
+    std::vector<double> eD0(Me), eNrm(3 * (size_t)Me), ePI(3 * (size_t)Me), ePJ(3 * (size_t)Me), eX0(Me), eLb(Me),
+        eIk(Me);
+    std::vector<uint32_t> eStart, eEnt;
+    if (Me > 0) {
+        if (ctx->nsys > 1) return fail(ctx, ALENS_E_STATE, "caller constraint blocks need a one-system context");
+        if (!ctx->gid_index_ok) {
+            ctx->gid_index.clear();
+            ctx->gid_index.reserve((size_t)n * 2);
+            for (int32_t i = 0; i < n; ++i) ctx->gid_index[ctx->h_gid[i]] = i;
+            ctx->gid_index_ok = true;
+        }
+        eStart.assign((size_t)n + 1, 0);
+        for (uint32_t e = 0; e < Me; ++e) {
+            const alens_con_block &cb = ctx->ext_pending[e];
+            auto fi = ctx->gid_index.find(cb.gid_i), fj = ctx->gid_index.find(cb.gid_j);
+            if (fi == ctx->gid_index.end() || fj == ctx->gid_index.end() || fi->second == fj->second)
+                return fail(ctx, ALENS_E_ARG, "constraint block: unknown gid or gid_i == gid_j");
+            eI[e] = fi->second;
+            eJ[e] = fj->second;
+            bool ok = std::isfinite(cb.delta0) && std::isfinite(cb.gamma) && (cb.bilateral || std::isfinite(cb.gamma_lb));
+            for (int a = 0; a < 3; ++a) {
+                ok = ok && cb.norm_j[a] == -cb.norm_i[a] && std::isfinite(cb.norm_i[a]) && std::isfinite(cb.pos_i[a]) &&
+                     std::isfinite(cb.pos_j[a]);
+                eNrm[3 * (size_t)e + a] = cb.norm_i[a];
+                ePI[3 * (size_t)e + a] = cb.pos_i[a];
+                ePJ[3 * (size_t)e + a] = cb.pos_j[a];
+            }
+            if (!ok) return fail(ctx, ALENS_E_ARG, "constraint block: non-finite field or norm_j != -norm_i");
+            eD0[e] = cb.delta0;
+            eX0[e] = cb.gamma;
+            eLb[e] = cb.bilateral ? -INFINITY : cb.gamma_lb;
+            eIk[e] = (cb.bilateral && cb.kappa > 0.0) ? 1.0 / cb.kappa : 0.0;
+            eStart[(size_t)eI[e] + 1]++;
+            eStart[(size_t)eJ[e] + 1]++;
+        }
+        for (int32_t i = 0; i < n; ++i) eStart[(size_t)i + 1] += eStart[i];
+        eEnt.resize(2 * (size_t)Me);
+        std::vector<uint32_t> cur(eStart.begin(), eStart.end() - 1);
+        for (uint32_t e = 0; e < Me; ++e) {
+            eEnt[cur[eI[e]]++] = e;
+            eEnt[cur[eJ[e]]++] = e | 0x80000000u;
+        }
+    }
     if (f_ext) {
         if (!finite_all(f_ext, 6 * (size_t)n)) return fail(ctx, ALENS_E_NONFINITE, "non-finite f_ext");
         if (n > 0) TRY_HIP(hipMemcpyAsync(ctx->fext.p, f_ext, sizeof(double) * 6 * n, hipMemcpyHostToDevice, st));
@@ -700,7 +771,11 @@ alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_s
         ctx->pcap = (uint32_t)std::min(ctx->pI.cap, std::min(ctx->pJ.cap, ctx->pImg.cap));
     }
     if (M > ctx->pcap) return fail(ctx, ALENS_E_HIP, "pair buffer overflow after resize");
+    const uint32_t Mcol = M;
+    M = Mcol + Me;  // rows of the step: collision rows, then the caller's
     ctx->m = M;
+    ctx->mcol = Mcol;
+    ctx->mext = Me;
     const size_t Mc = std::max<size_t>(M, 1);
     TRY_HIP(ctx->cI.ensure(Mc));
     TRY_HIP(ctx->cJ.ensure(Mc));
@@ -711,29 +786,69 @@ alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_s
     TRY_HIP(ctx->posI.ensure(3 * Mc));
     TRY_HIP(ctx->posJ.ensure(3 * Mc));
     TRY_HIP(ctx->b.ensure(Mc));
-    TRY_HIP(ctx->adjK.ensure(2 * Mc));
-    TRY_HIP(ctx->adjW.ensure(12 * Mc));
     TRY_HIP(ctx->xg.ensure(Mc));
     TRY_HIP(ctx->gamma.ensure(Mc));
     // canonical order + rows + adjacency
     TRY_K(exclusive_scan_u32(st, ctx->cntI.p, ctx->rowStart.p, n, ctx->scan));
     TRY_K(exclusive_scan_u32(st, ctx->cntJ.p, ctx->jStart.p, n, ctx->scan));
     TRY_HIP(hipMemsetAsync(ctx->cur.p, 0, sizeof(uint32_t) * (n + 1), st));
-    TRY_K(launch_order_I(st, n, M, ctx->pI.p, ctx->pJ.p, ctx->pImg.p, ctx->rowStart.p, ctx->cur.p, ctx->cI.p,
+    TRY_K(launch_order_I(st, n, Mcol, ctx->pI.p, ctx->pJ.p, ctx->pImg.p, ctx->rowStart.p, ctx->cur.p, ctx->cI.p,
                          ctx->cJ.p, ctx->cImg.p));
     TRY_HIP(hipMemsetAsync(ctx->cur.p, 0, sizeof(uint32_t) * (n + 1), st));
-    TRY_K(launch_order_J(st, n, M, ctx->cJ.p, ctx->jStart.p, ctx->cur.p, ctx->jlist.p));
-    TRY_K(launch_rows(st, rd, cd, M, ctx->cI.p, ctx->cJ.p, ctx->cImg.p, ctx->unc.p, ctx->delta0.p, ctx->geo.p,
+    TRY_K(launch_order_J(st, n, Mcol, ctx->cJ.p, ctx->jStart.p, ctx->cur.p, ctx->jlist.p));
+    TRY_K(launch_rows(st, rd, cd, Mcol, M, ctx->cI.p, ctx->cJ.p, ctx->cImg.p, ctx->unc.p, ctx->delta0.p, ctx->geo.p,
                       ctx->posI.p, ctx->posJ.p, ctx->b.p));
-    TRY_K(launch_adjacency(st, n, ctx->rowStart.p, ctx->jStart.p, ctx->jlist.p, ctx->geo.p, ctx->adjStart.p,
-                           ctx->adjK.p, ctx->adjW.p));
+    if (Me > 0) {
+        TRY_HIP(ctx->extStart.ensure((size_t)n + 1));
+        TRY_HIP(ctx->extEnt.ensure(2 * (size_t)Me));
+        TRY_HIP(ctx->extNrm.ensure(3 * (size_t)Me));
+        TRY_HIP(ctx->extX0.ensure(Me));
+        TRY_HIP(ctx->lb.ensure(Mc));
+        TRY_HIP(ctx->invk.ensure(Mc));
+        TRY_HIP(hipMemcpyAsync(ctx->cI.p + Mcol, eI.data(), sizeof(int32_t) * Me, hipMemcpyHostToDevice, st));
+        TRY_HIP(hipMemcpyAsync(ctx->cJ.p + Mcol, eJ.data(), sizeof(int32_t) * Me, hipMemcpyHostToDevice, st));
+        TRY_HIP(hipMemsetAsync(ctx->cImg.p + Mcol, 0, sizeof(uint32_t) * Me, st));
+        TRY_HIP(hipMemcpyAsync(ctx->delta0.p + Mcol, eD0.data(), sizeof(double) * Me, hipMemcpyHostToDevice, st));
+        TRY_HIP(hipMemcpyAsync(ctx->posI.p + 3 * (size_t)Mcol, ePI.data(), sizeof(double) * 3 * Me,
+                               hipMemcpyHostToDevice, st));
+        TRY_HIP(hipMemcpyAsync(ctx->posJ.p + 3 * (size_t)Mcol, ePJ.data(), sizeof(double) * 3 * Me,
+                               hipMemcpyHostToDevice, st));
+        TRY_HIP(hipMemcpyAsync(ctx->extNrm.p, eNrm.data(), sizeof(double) * 3 * Me, hipMemcpyHostToDevice, st));
+        TRY_HIP(hipMemcpyAsync(ctx->extX0.p, eX0.data(), sizeof(double) * Me, hipMemcpyHostToDevice, st));
+        TRY_HIP(hipMemcpyAsync(ctx->extStart.p, eStart.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice, st));
+        TRY_HIP(hipMemcpyAsync(ctx->extEnt.p, eEnt.data(), sizeof(uint32_t) * 2 * Me, hipMemcpyHostToDevice, st));
+        if (Mcol > 0) {
+            TRY_HIP(hipMemsetAsync(ctx->lb.p, 0, sizeof(double) * Mcol, st));  // 0.0 = all-zero bytes
+            TRY_HIP(hipMemsetAsync(ctx->invk.p, 0, sizeof(double) * Mcol, st));
+        }
+        TRY_HIP(hipMemcpyAsync(ctx->lb.p + Mcol, eLb.data(), sizeof(double) * Me, hipMemcpyHostToDevice, st));
+        TRY_HIP(hipMemcpyAsync(ctx->invk.p + Mcol, eIk.data(), sizeof(double) * Me, hipMemcpyHostToDevice, st));
+        TRY_K(launch_rows_ext(st, Me, Mcol, M, ctx->cI.p, ctx->cJ.p, ctx->extNrm.p, ctx->posI.p, ctx->posJ.p,
+                              ctx->delta0.p, ctx->unc.p, ctx->cfg.dt, ctx->geo.p, ctx->b.p));
+    }
+    const uint32_t *xs = Me ? ctx->extStart.p : nullptr;
     {
-        // constraint range of each system = rowStart at its first rod (rows are grouped by rod I)
+        // sliced adjacency sizes; constraint range of each system = rowStart at its first rod (rows are
+        // grouped by rod I; caller rows exist only with one system and close its range)
+        const int64_t nsl = ((int64_t)n + kSlice - 1) / kSlice;
+        TRY_HIP(ctx->slen.ensure((size_t)nsl + 1));
+        TRY_HIP(ctx->sellStart.ensure((size_t)nsl + 1));
+        TRY_K(launch_slice_sizes(st, n, ctx->rowStart.p, ctx->jStart.p, xs, ctx->slen.p));
+        TRY_K(exclusive_scan_u32(st, ctx->slen.p, ctx->sellStart.p, nsl, ctx->scan));
+        uint32_t ep = 0;
+        TRY_HIP(hipMemcpyAsync(&ep, ctx->sellStart.p + nsl, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         std::vector<uint32_t> coff((size_t)ctx->nsys + 1);
         for (int32_t sy = 0; sy <= ctx->nsys; ++sy)
             TRY_HIP(hipMemcpyAsync(&coff[sy], ctx->rowStart.p + ctx->rod_off[sy], sizeof(uint32_t),
                                    hipMemcpyDeviceToHost, st));
         TRY_HIP(hipStreamSynchronize(st));
+        coff[ctx->nsys] += Me;
+        ctx->epad = ep;
+        TRY_HIP(ctx->sellK.ensure(std::max<size_t>(ep, 1)));
+        TRY_HIP(ctx->sellW.ensure(6 * std::max<size_t>(ep, 1)));
+        TRY_K(launch_adjacency(st, n, M, ctx->rowStart.p, ctx->jStart.p, ctx->jlist.p, xs, Me ? ctx->extEnt.p : nullptr,
+                               Mcol, ctx->geo.p, ctx->sellStart.p, ctx->epad, ctx->adjStart.p, ctx->sellK.p,
+                               ctx->sellW.p));
         alens_status cs = set_chunk_table(ctx, coff);
         if (cs != ALENS_OK) return cs;
     }
@@ -746,23 +861,28 @@ alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_s
     bool warm = false;
     ctx->have_xprev = false;
     if (M > 0) {
-        if (ws >= 1 && ctx->have_prev) {
-            TRY_K(launch_warm_match(st, M, ctx->cI.p, ctx->cJ.p, ctx->gid.p, ctx->mprev, ctx->pkey.p, ctx->pgam.p,
+        // start point of the collision rows (caller rows start from their blocks' gamma)
+        if (ws >= 1 && ctx->have_prev && Mcol > 0) {
+            TRY_K(launch_warm_match(st, Mcol, ctx->cI.p, ctx->cJ.p, ctx->gid.p, ctx->mprev, ctx->pkey.p, ctx->pgam.p,
                                     ctx->xg.p));
             TRY_HIP(ctx->xprev.ensure(Mc));
-            TRY_HIP(hipMemcpyAsync(ctx->xprev.p, ctx->xg.p, sizeof(double2) * M, hipMemcpyDeviceToDevice, st));
+            TRY_HIP(hipMemcpyAsync(ctx->xprev.p, ctx->xg.p, sizeof(double2) * Mcol, hipMemcpyDeviceToDevice, st));
             ctx->have_xprev = true;
             warm = true;
         } else {
-            TRY_K(launch_init_xg(st, ctx->xg.p, nullptr, M));
+            TRY_K(launch_init_xg(st, ctx->xg.p, nullptr, Mcol));
         }
-        if (ws == 2) {
+        if (ws == 2 && Mcol > 0) {
             TRY_HIP(ctx->feat.ensure(16 * Mc));
-            TRY_K(launch_features(st, M, ctx->cI.p, ctx->cJ.p, ctx->delta0.p, ctx->geo.p, ctx->posI.p, ctx->posJ.p,
-                                  ctx->dir.p, ctx->length.p, ctx->unc.p, ctx->have_xprev ? ctx->xprev.p : nullptr,
-                                  ctx->cfg.dt, ctx->feat.p));
-            TRY_K(launch_init_xg(st, ctx->xg.p, nullptr, M));
-            TRY_K(launch_mlp_infer_xg(st, M, ctx->feat.p, ctx->mlpP.p, ctx->mlpNorm.p, ctx->xg.p));
+            TRY_K(launch_features(st, Mcol, M, ctx->cI.p, ctx->cJ.p, ctx->delta0.p, ctx->geo.p, ctx->posI.p,
+                                  ctx->posJ.p, ctx->dir.p, ctx->length.p, ctx->unc.p,
+                                  ctx->have_xprev ? ctx->xprev.p : nullptr, ctx->cfg.dt, ctx->feat.p));
+            TRY_K(launch_init_xg(st, ctx->xg.p, nullptr, Mcol));
+            TRY_K(launch_mlp_infer_xg(st, Mcol, ctx->feat.p, ctx->mlpP.p, ctx->mlpNorm.p, ctx->xg.p));
+            warm = true;
+        }
+        if (Me > 0) {
+            TRY_K(launch_set_x(st, ctx->xg.p + Mcol, ctx->extX0.p, Me));
             warm = true;
         }
         alens_status s = ctx->cfg.precision == 1
@@ -774,13 +894,14 @@ alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_s
         rep.converged = 1;
         for (auto &r : ctx->sysrep) r = alens_solve_report{}, r.converged = 1;
     }
-    TRY_K(launch_writeback(st, n, ctx->rsys.p, ctx->sysd.p, ctx->adjStart.p, ctx->adjK.p, ctx->adjW.p, ctx->xg.p,
-                           ctx->mob.p, ctx->unc.p, ctx->F.p, ctx->U.p, ctx->gamma.p, M));
+    TRY_K(launch_writeback(st, n, ctx->rsys.p, ctx->sysd.p, ctx->adjStart.p, ctx->sellStart.p, ctx->sellK.p,
+                           ctx->sellW.p, ctx->epad, ctx->xg.p, ctx->mob.p, ctx->unc.p, ctx->F.p, ctx->U.p,
+                           ctx->gamma.p, M));
     if (ctx->cfg.warm_start >= 1) {
         TRY_HIP(ctx->pkey.ensure(Mc));
         TRY_HIP(ctx->pgam.ensure(Mc));
-        TRY_K(launch_make_keys(st, M, ctx->cI.p, ctx->cJ.p, ctx->gid.p, ctx->xg.p, ctx->pkey.p, ctx->pgam.p));
-        ctx->mprev = M;
+        TRY_K(launch_make_keys(st, Mcol, ctx->cI.p, ctx->cJ.p, ctx->gid.p, ctx->xg.p, ctx->pkey.p, ctx->pgam.p));
+        ctx->mprev = Mcol;
         ctx->have_prev = true;
     }
     TRY_HIP(hipEventRecord(ctx->ev[4], st));
@@ -794,6 +915,8 @@ alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_s
     rep.t_solve_ms = ms_solve;
     rep.t_step_ms = ms_step;
     ctx->have_step = true;
+    ctx->ext_step.swap(ctx->ext_pending);
+    ctx->ext_pending.clear();
     if (rep_out) *rep_out = rep;
     if (rep.status == ALENS_E_NONFINITE) return fail(ctx, ALENS_E_NONFINITE, "non-finite BBPGD residual");
     if (rep.status == ALENS_E_NOCONV) return fail(ctx, ALENS_E_NOCONV, "BBPGD did not reach tol within max_ite");
@@ -847,9 +970,9 @@ alens_status alens_get_constraint_arrays(alens_ctx *ctx, int32_t *I, int32_t *J,
         if (img)
             for (int a = 0; a < 3; ++a) img[3 * k + a] = img_get(im[k], a);
         for (int a = 0; a < 3; ++a) {
-            if (nrm) nrm[3 * k + a] = geo[9 * k + a];
-            if (r_i) r_i[3 * k + a] = geo[9 * k + 3 + a];
-            if (r_j) r_j[3 * k + a] = geo[9 * k + 6 + a];
+            if (nrm) nrm[3 * k + a] = geo[a * m + k];
+            if (r_i) r_i[3 * k + a] = geo[(3 + a) * m + k];
+            if (r_j) r_j[3 * k + a] = geo[(6 + a) * m + k];
         }
     }
     return ALENS_OK;
@@ -892,7 +1015,20 @@ alens_status alens_get_constraints(alens_ctx *ctx, int64_t cap, alens_con_block 
     if (s != ALENS_OK) return s;
     s = alens_get_rods(ctx, cen.data(), nullptr);
     if (s != ALENS_OK) return s;
+    size_t sy = 0;  // rows are grouped by rod I, and rods by system, so the system index only grows
+    const size_t mc = ctx->mcol;
     for (size_t k = 0; k < m; ++k) {
+        if (k >= mc) {
+            // caller row: the block as given, with this step's gamma and rod indices
+            alens_con_block &cb = out[k];
+            cb = ctx->ext_step[k - mc];
+            cb.gamma = gam[k];
+            cb.gidx_i = I[k];
+            cb.gidx_j = J[k];
+            continue;
+        }
+        while (sy + 1 < ctx->boxes.size() && I[k] >= ctx->rod_off[sy + 1]) ++sy;
+        const alens_box &bx = ctx->boxes[sy];
         alens_con_block &cb = out[k];
         memset(&cb, 0, sizeof cb);
         cb.delta0 = d0[k];
@@ -904,8 +1040,8 @@ alens_status alens_get_constraints(alens_ctx *ctx, int64_t cap, alens_con_block 
         cb.gidx_i = I[k];
         cb.gidx_j = J[k];
         for (int a = 0; a < 3; ++a) {
-            const double len = ctx->cfg.box_hi[a] - ctx->cfg.box_lo[a];
-            const double cj = ctx->cfg.pbc[a] ? cen[3 * J[k] + a] + (double)img[3 * k + a] * len : cen[3 * J[k] + a];
+            const double len = bx.box_hi[a] - bx.box_lo[a];
+            const double cj = bx.pbc[a] ? cen[3 * J[k] + a] + (double)img[3 * k + a] * len : cen[3 * J[k] + a];
             cb.norm_i[a] = nrm[3 * k + a];
             cb.norm_j[a] = -nrm[3 * k + a];
             cb.pos_i[a] = pI[3 * k + a];
@@ -913,9 +1049,42 @@ alens_status alens_get_constraints(alens_ctx *ctx, int64_t cap, alens_con_block 
             cb.lab_i[a] = cen[3 * I[k] + a] + pI[3 * k + a];
             cb.lab_j[a] = cj + pJ[3 * k + a];
         }
+        // per-unit-impulse stress: virial of the force pair +n at lab_i, -n at lab_j
+        for (int a = 0; a < 3; ++a)
+            for (int c = 0; c < 3; ++c) cb.stress[3 * a + c] = (cb.lab_i[a] - cb.lab_j[a]) * cb.norm_i[c];
         cb.one_side = 0;
         cb.bilateral = 0;
     }
+    return ALENS_OK;
+}
+
+alens_status alens_get_con_stress(alens_ctx *ctx, double collision[9], double external[9]) {
+    if (!ctx) return ALENS_E_ARG;
+    if (!ctx->have_step) return fail(ctx, ALENS_E_STATE, "no step has been run");
+    int64_t m = ctx->m;
+    std::vector<alens_con_block> blk((size_t)m);
+    alens_status s = alens_get_constraints(ctx, m, blk.data(), &m);
+    if (s != ALENS_OK) return s;
+    double col[9] = {0}, ext[9] = {0};
+    for (size_t k = 0; k < (size_t)m; ++k) {
+        double *t = k < ctx->mcol ? col : ext;
+        for (int q = 0; q < 9; ++q) t[q] = t[q] + blk[k].gamma * blk[k].stress[q];
+    }
+    if (collision) memcpy(collision, col, sizeof col);
+    if (external) memcpy(external, ext, sizeof ext);
+    return ALENS_OK;
+}
+
+alens_status alens_add_constraints(alens_ctx *ctx, int64_t m, const alens_con_block *blocks) {
+    if (!ctx || m < 0 || (m > 0 && !blocks)) return ALENS_E_ARG;
+    if (ctx->ext_pending.size() + (size_t)m > 0x3fffffffu) return fail(ctx, ALENS_E_ARG, "add_constraints: too many");
+    ctx->ext_pending.insert(ctx->ext_pending.end(), blocks, blocks + m);
+    return ALENS_OK;
+}
+
+alens_status alens_clear_constraints(alens_ctx *ctx) {
+    if (!ctx) return ALENS_E_ARG;
+    ctx->ext_pending.clear();
     return ALENS_OK;
 }
 
@@ -936,38 +1105,53 @@ alens_status alens_bcqp_solve(alens_ctx *ctx, int32_t n, int64_t m, const int32_
         if (I[k] < 0 || I[k] >= n || J[k] < 0 || J[k] >= n || I[k] == J[k])
             return fail(ctx, ALENS_E_ARG, "bcqp_solve: rod index out of range");
     TRY_HIP(hipSetDevice(ctx->dev));
-    // rod-major adjacency in ascending constraint index, built on the host for arbitrary rows
-    std::vector<uint32_t> start((size_t)n + 1, 0), adjK(2 * (size_t)m);
-    std::vector<double> adjW(12 * (size_t)m), geo(9 * (size_t)m);
+    ctx->have_step = false;  // the step buffers are about to hold this solve's rows
+    ctx->mext = 0;
+    ctx->mcol = 0;
+    // rod-major adjacency in ascending constraint index, built on the host for arbitrary rows, in the
+    // sliced layout the solver reads (see launch_adjacency); geometry component-major
+    const size_t M = (size_t)m, N = (size_t)std::max(n, 1);
+    std::vector<uint32_t> start((size_t)n + 1, 0);
+    std::vector<double> geo(9 * M);
     for (int64_t k = 0; k < m; ++k) {
         start[I[k] + 1]++;
         start[J[k] + 1]++;
         for (int a = 0; a < 3; ++a) {
-            geo[9 * k + a] = nrm[3 * k + a];
-            geo[9 * k + 3 + a] = r_i[3 * k + a];
-            geo[9 * k + 6 + a] = r_j[3 * k + a];
+            geo[a * M + k] = nrm[3 * k + a];
+            geo[(3 + a) * M + k] = r_i[3 * k + a];
+            geo[(6 + a) * M + k] = r_j[3 * k + a];
         }
     }
     for (int32_t i = 0; i < n; ++i) start[i + 1] += start[i];
-    std::vector<uint32_t> cur(start.begin(), start.end() - 1);
+    const size_t nsl = ((size_t)n + kSlice - 1) / kSlice;
+    std::vector<uint32_t> sstart(nsl + 1, 0);
+    for (size_t sl = 0; sl < nsl; ++sl) {
+        uint32_t dmax = 0;
+        for (size_t i = sl * kSlice; i < std::min((sl + 1) * kSlice, (size_t)n); ++i)
+            dmax = std::max(dmax, start[i + 1] - start[i]);
+        sstart[sl + 1] = sstart[sl] + dmax * (uint32_t)kSlice;
+    }
+    const uint64_t ep = sstart[nsl];
+    std::vector<uint32_t> sk(std::max<uint64_t>(ep, 1), 0), fill((size_t)n, 0);
+    std::vector<double> sw(6 * std::max<uint64_t>(ep, 1), 0.0);
+    // entries go in ascending k per rod; k runs in order, so each rod's slots fill left to right
     for (int64_t k = 0; k < m; ++k) {
-        uint32_t e = cur[I[k]]++;
-        adjK[e] = (uint32_t)k;
-        for (int a = 0; a < 3; ++a) {
-            adjW[6 * e + a] = nrm[3 * k + a];
-            adjW[6 * e + 3 + a] = r_i[3 * k + a];
-        }
-        e = cur[J[k]]++;
-        adjK[e] = (uint32_t)k | 0x80000000u;
-        for (int a = 0; a < 3; ++a) {
-            adjW[6 * e + a] = -nrm[3 * k + a];
-            adjW[6 * e + 3 + a] = -r_j[3 * k + a];
+        for (int side = 0; side < 2; ++side) {
+            const int32_t i = side == 0 ? I[k] : J[k];
+            const size_t idx = sstart[(size_t)i / kSlice] + (size_t)fill[i]++ * kSlice + (size_t)i % kSlice;
+            sk[idx] = side == 0 ? (uint32_t)k : ((uint32_t)k | 0x80000000u);
+            const double *r = side == 0 ? r_i : r_j;
+            for (int a = 0; a < 3; ++a) {
+                sw[a * ep + idx] = side == 0 ? nrm[3 * k + a] : -nrm[3 * k + a];
+                sw[(3 + a) * ep + idx] = side == 0 ? r[3 * k + a] : -r[3 * k + a];
+            }
         }
     }
-    const size_t M = (size_t)m, N = (size_t)std::max(n, 1);
+    ctx->epad = ep;
     TRY_HIP(ctx->adjStart.ensure(N + 1));
-    TRY_HIP(ctx->adjK.ensure(2 * M));
-    TRY_HIP(ctx->adjW.ensure(12 * M));
+    TRY_HIP(ctx->sellStart.ensure(nsl + 1));
+    TRY_HIP(ctx->sellK.ensure(sk.size()));
+    TRY_HIP(ctx->sellW.ensure(sw.size()));
     TRY_HIP(ctx->geo.ensure(9 * M));
     TRY_HIP(ctx->cI.ensure(M));
     TRY_HIP(ctx->cJ.ensure(M));
@@ -981,8 +1165,9 @@ alens_status alens_bcqp_solve(alens_ctx *ctx, int32_t n, int64_t m, const int32_
     TRY_HIP(ctx->delta0.ensure(M));
     hipStream_t st = ctx->st;
     TRY_HIP(hipMemcpyAsync(ctx->adjStart.p, start.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice, st));
-    TRY_HIP(hipMemcpyAsync(ctx->adjK.p, adjK.data(), sizeof(uint32_t) * 2 * M, hipMemcpyHostToDevice, st));
-    TRY_HIP(hipMemcpyAsync(ctx->adjW.p, adjW.data(), sizeof(double) * 12 * M, hipMemcpyHostToDevice, st));
+    TRY_HIP(hipMemcpyAsync(ctx->sellStart.p, sstart.data(), sizeof(uint32_t) * (nsl + 1), hipMemcpyHostToDevice, st));
+    TRY_HIP(hipMemcpyAsync(ctx->sellK.p, sk.data(), sizeof(uint32_t) * sk.size(), hipMemcpyHostToDevice, st));
+    TRY_HIP(hipMemcpyAsync(ctx->sellW.p, sw.data(), sizeof(double) * sw.size(), hipMemcpyHostToDevice, st));
     TRY_HIP(hipMemcpyAsync(ctx->geo.p, geo.data(), sizeof(double) * 9 * M, hipMemcpyHostToDevice, st));
     TRY_HIP(hipMemcpyAsync(ctx->cI.p, I, sizeof(int32_t) * M, hipMemcpyHostToDevice, st));
     TRY_HIP(hipMemcpyAsync(ctx->cJ.p, J, sizeof(int32_t) * M, hipMemcpyHostToDevice, st));
@@ -1003,8 +1188,8 @@ alens_status alens_bcqp_solve(alens_ctx *ctx, int32_t n, int64_t m, const int32_
     rs0.release();
     mono0.release();
     if (s != ALENS_OK) return s;
-    TRY_K(launch_writeback(st, 0, nullptr, nullptr, nullptr, nullptr, nullptr, ctx->xg.p, nullptr, nullptr, nullptr,
-                           nullptr, ctx->gamma.p, (uint32_t)M));
+    TRY_K(launch_writeback(st, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, ctx->xg.p, nullptr, nullptr,
+                           nullptr, nullptr, ctx->gamma.p, (uint32_t)M));
     TRY_HIP(hipMemcpyAsync(gamma_out, ctx->gamma.p, sizeof(double) * M, hipMemcpyDeviceToHost, st));
     TRY_HIP(hipStreamSynchronize(st));
     rep.n_con = m;
@@ -1050,9 +1235,14 @@ alens_status alens_get_features(alens_ctx *ctx, float *X) {
     const uint32_t M = (uint32_t)ctx->m;
     if (M == 0) return ALENS_OK;
     TRY_HIP(ctx->feat.ensure(16 * (size_t)M));
-    TRY_K(launch_features(ctx->st, M, ctx->cI.p, ctx->cJ.p, ctx->delta0.p, ctx->geo.p, ctx->posI.p, ctx->posJ.p,
+    // collision rows (with the previous gamma when the step had one), then caller rows (previous gamma 0)
+    const uint32_t mc = ctx->mcol, me = M - mc;
+    TRY_K(launch_features(ctx->st, mc, M, ctx->cI.p, ctx->cJ.p, ctx->delta0.p, ctx->geo.p, ctx->posI.p, ctx->posJ.p,
                           ctx->dir.p, ctx->length.p, ctx->unc.p, ctx->have_xprev ? ctx->xprev.p : nullptr,
                           ctx->cfg.dt, ctx->feat.p));
+    TRY_K(launch_features(ctx->st, me, M, ctx->cI.p + mc, ctx->cJ.p + mc, ctx->delta0.p + mc, ctx->geo.p + mc,
+                          ctx->posI.p + 3 * (size_t)mc, ctx->posJ.p + 3 * (size_t)mc, ctx->dir.p, ctx->length.p,
+                          ctx->unc.p, nullptr, ctx->cfg.dt, ctx->feat.p + 16 * (size_t)mc));
     TRY_HIP(hipMemcpyAsync(X, ctx->feat.p, sizeof(float) * 16 * M, hipMemcpyDeviceToHost, ctx->st));
     TRY_HIP(hipStreamSynchronize(ctx->st));
     return ALENS_OK;

@@ -618,9 +618,9 @@ HipErr launch_order_J(hipStream_t st, int32_t n, uint32_t m, const int32_t *cJ,
 
 // ------------------------------------------------------------------------------------------------
 // Constraint rows: delta0, n (J -> I), posI = s tI, posJ = u tJ, rI = posI x n, rJ = posJ x n,
-// b = delta0 + dt (D^T U_nc).  geo[k] = {n, rI, rJ}.
+// b = delta0 + dt (D^T U_nc).  geo component-major: geo[q * m + k], q = 0..8 = {n, rI, rJ}.
 // ------------------------------------------------------------------------------------------------
-__global__ void k_rows(RodsDev r, CfgDev c, uint32_t m, const int32_t *__restrict__ cI,
+__global__ void k_rows(RodsDev r, CfgDev c, uint32_t m, uint32_t gs, const int32_t *__restrict__ cI,
                        const int32_t *__restrict__ cJ, const uint32_t *__restrict__ cImg,
                        const double *__restrict__ unc, double *__restrict__ delta0,
                        double *__restrict__ geo, double *__restrict__ posI,
@@ -655,7 +655,7 @@ __global__ void k_rows(RodsDev r, CfgDev c, uint32_t m, const int32_t *__restric
     const double d0 = (sg.dist - r.rcol[i]) - r.rcol[j];
     delta0[k] = d0;
 #pragma unroll
-    for (int a = 0; a < 9; ++a) geo[9 * (size_t)k + a] = g9[a];
+    for (int a = 0; a < 9; ++a) geo[(size_t)a * gs + k] = g9[a];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         posI[3 * (size_t)k + a] = pI[a];
@@ -664,11 +664,47 @@ __global__ void k_rows(RodsDev r, CfgDev c, uint32_t m, const int32_t *__restric
     b[k] = d0 + c.dt * dt_row(g9, unc + 6 * (size_t)i, unc + 6 * (size_t)j);
 }
 
-HipErr launch_rows(hipStream_t st, const RodsDev &r, const CfgDev &c, uint32_t m, const int32_t *cI,
+HipErr launch_rows(hipStream_t st, const RodsDev &r, const CfgDev &c, uint32_t m, uint32_t gs, const int32_t *cI,
                    const int32_t *cJ, const uint32_t *cImg, const double *unc, double *delta0,
                    double *geo, double *posI, double *posJ, double *b) {
     if (m == 0) return hip_ok();
-    k_rows<<<nblk(m, 256), 256, 0, st>>>(r, c, m, cI, cJ, cImg, unc, delta0, geo, posI, posJ, b);
+    k_rows<<<nblk(m, 256), 256, 0, st>>>(r, c, m, gs, cI, cJ, cImg, unc, delta0, geo, posI, posJ, b);
+    LAUNCH_CHECK();
+    return hip_ok();
+}
+
+// Caller-supplied rows k = k0 + e (the constraint pool): normal n = normI (uploaded into nrm),
+// arms posI / posJ given; rI = posI x n, rJ = posJ x n, b = delta0 + dt (D^T U_nc).
+__global__ void k_rows_ext(uint32_t me, uint32_t k0, uint32_t gs, const int32_t *__restrict__ cI,
+                           const int32_t *__restrict__ cJ, const double *__restrict__ nrm,
+                           const double *__restrict__ posI, const double *__restrict__ posJ,
+                           const double *__restrict__ delta0, const double *__restrict__ unc, double dt,
+                           double *__restrict__ geo, double *__restrict__ b) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= me) return;
+    const uint32_t k = k0 + e;
+    const double *nk = nrm + 3 * (size_t)e;
+    const double *pI = posI + 3 * (size_t)k, *pJ = posJ + 3 * (size_t)k;
+    double g9[9];
+    g9[0] = nk[0];
+    g9[1] = nk[1];
+    g9[2] = nk[2];
+    g9[3] = pI[1] * nk[2] - pI[2] * nk[1];
+    g9[4] = pI[2] * nk[0] - pI[0] * nk[2];
+    g9[5] = pI[0] * nk[1] - pI[1] * nk[0];
+    g9[6] = pJ[1] * nk[2] - pJ[2] * nk[1];
+    g9[7] = pJ[2] * nk[0] - pJ[0] * nk[2];
+    g9[8] = pJ[0] * nk[1] - pJ[1] * nk[0];
+#pragma unroll
+    for (int a = 0; a < 9; ++a) geo[(size_t)a * gs + k] = g9[a];
+    b[k] = delta0[k] + dt * dt_row(g9, unc + 6 * (size_t)cI[k], unc + 6 * (size_t)cJ[k]);
+}
+
+HipErr launch_rows_ext(hipStream_t st, uint32_t me, uint32_t k0, uint32_t gs, const int32_t *cI, const int32_t *cJ,
+                       const double *nrm, const double *posI, const double *posJ, const double *delta0,
+                       const double *unc, double dt, double *geo, double *b) {
+    if (me == 0) return hip_ok();
+    k_rows_ext<<<nblk(me, 256), 256, 0, st>>>(me, k0, gs, cI, cJ, nrm, posI, posJ, delta0, unc, dt, geo, b);
     LAUNCH_CHECK();
     return hip_ok();
 }
@@ -676,49 +712,90 @@ HipErr launch_rows(hipStream_t st, const RodsDev &r, const CfgDev &c, uint32_t m
 // ------------------------------------------------------------------------------------------------
 // Rod-major adjacency in ascending constraint index: the J-side list (constraints with J = i all
 // have I < i, so they precede i's own rows) followed by i's I-side rows.  Each entry carries the
-// rod's column block of D: (n, rI) on the I side, (-n, -rJ) on the J side.
+// rod's column block of D: (n, rI) on the I side, (-n, -rJ) on the J side.  Stored sliced: slice s
+// (rods kSlice*s ..) holds kSlice * (largest degree in the slice) slots, entry j of rod i at
+// sellStart[s] + j * kSlice + i % kSlice.
 // ------------------------------------------------------------------------------------------------
-__global__ void k_adjacency(int32_t n, const uint32_t *__restrict__ rowStart,
+__global__ void k_slice_sizes(int32_t n, const uint32_t *__restrict__ rowStart, const uint32_t *__restrict__ jStart,
+                              const uint32_t *__restrict__ extStart, uint32_t *__restrict__ slice_len) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;  // one lane per rod, one wave per slice
+    uint32_t deg = 0;
+    if (i < n) {
+        deg = (rowStart[i + 1] - rowStart[i]) + (jStart[i + 1] - jStart[i]);
+        if (extStart) deg += extStart[i + 1] - extStart[i];
+    }
+#pragma unroll
+    for (int h = 32; h >= 1; h >>= 1) {
+        const uint32_t o = __shfl_xor(deg, h, 64);
+        deg = o > deg ? o : deg;
+    }
+    if ((threadIdx.x & 63) == 0 && i < n) slice_len[i >> 6] = deg * (uint32_t)kSlice;
+}
+
+HipErr launch_slice_sizes(hipStream_t st, int32_t n, const uint32_t *rowStart, const uint32_t *jStart,
+                          const uint32_t *extStart, uint32_t *slice_len) {
+    static_assert(kSlice == 64, "one wave per slice");
+    if (n == 0) return hip_ok();
+    k_slice_sizes<<<nblk(n, 256), 256, 0, st>>>(n, rowStart, jStart, extStart, slice_len);
+    LAUNCH_CHECK();
+    return hip_ok();
+}
+
+__global__ void k_adjacency(int32_t n, uint32_t m, const uint32_t *__restrict__ rowStart,
                             const uint32_t *__restrict__ jStart, const uint32_t *__restrict__ jlist,
-                            const double *__restrict__ geo, uint32_t *__restrict__ adjStart,
-                            uint32_t *__restrict__ adjK, double *__restrict__ adjW) {
+                            const uint32_t *__restrict__ extStart, const uint32_t *__restrict__ extEnt, uint32_t mc,
+                            const double *__restrict__ geo, const uint32_t *__restrict__ sellStart, uint64_t epad,
+                            uint32_t *__restrict__ adjStart, uint32_t *__restrict__ sellK,
+                            double *__restrict__ sellW) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i > n) return;
-    const uint32_t e0 = rowStart[i] + jStart[i];
-    adjStart[i] = e0;
+    adjStart[i] = rowStart[i] + jStart[i] + (extStart ? extStart[i] : 0u);
     if (i == n) return;
     const uint32_t j0 = jStart[i], j1 = jStart[i + 1];
-    uint32_t e = e0;
-    for (uint32_t t = j0; t < j1; ++t, ++e) {
+    uint64_t idx = sellStart[i >> 6] + (uint32_t)(i & (kSlice - 1));
+    for (uint32_t t = j0; t < j1; ++t, idx += kSlice) {
         const uint32_t k = jlist[t];
-        const double *gk = geo + 9 * (size_t)k;
-        adjK[e] = k | 0x80000000u;
-        double *w = adjW + 6 * (size_t)e;
-        w[0] = -gk[0];
-        w[1] = -gk[1];
-        w[2] = -gk[2];
-        w[3] = -gk[6];
-        w[4] = -gk[7];
-        w[5] = -gk[8];
+        sellK[idx] = k | 0x80000000u;
+        sellW[idx] = -geo[k];
+        sellW[epad + idx] = -geo[(size_t)m + k];
+        sellW[2 * epad + idx] = -geo[2 * (size_t)m + k];
+        sellW[3 * epad + idx] = -geo[6 * (size_t)m + k];
+        sellW[4 * epad + idx] = -geo[7 * (size_t)m + k];
+        sellW[5 * epad + idx] = -geo[8 * (size_t)m + k];
     }
-    for (uint32_t k = rowStart[i]; k < rowStart[i + 1]; ++k, ++e) {
-        const double *gk = geo + 9 * (size_t)k;
-        adjK[e] = k;
-        double *w = adjW + 6 * (size_t)e;
-        w[0] = gk[0];
-        w[1] = gk[1];
-        w[2] = gk[2];
-        w[3] = gk[3];
-        w[4] = gk[4];
-        w[5] = gk[5];
+    for (uint32_t k = rowStart[i]; k < rowStart[i + 1]; ++k, idx += kSlice) {
+        sellK[idx] = k;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) sellW[c * epad + idx] = geo[(size_t)c * m + k];
+    }
+    // caller rows (k = mc + e >= every collision row), ascending; top bit of the entry = J side
+    if (extStart) {
+        for (uint32_t t = extStart[i]; t < extStart[i + 1]; ++t, idx += kSlice) {
+            const uint32_t ent = extEnt[t];
+            const uint32_t k = mc + (ent & 0x7fffffffu);
+            if (ent & 0x80000000u) {
+                sellK[idx] = k | 0x80000000u;
+                sellW[idx] = -geo[k];
+                sellW[epad + idx] = -geo[(size_t)m + k];
+                sellW[2 * epad + idx] = -geo[2 * (size_t)m + k];
+                sellW[3 * epad + idx] = -geo[6 * (size_t)m + k];
+                sellW[4 * epad + idx] = -geo[7 * (size_t)m + k];
+                sellW[5 * epad + idx] = -geo[8 * (size_t)m + k];
+            } else {
+                sellK[idx] = k;
+#pragma unroll
+                for (int c = 0; c < 6; ++c) sellW[c * epad + idx] = geo[(size_t)c * m + k];
+            }
+        }
     }
 }
 
-HipErr launch_adjacency(hipStream_t st, int32_t n, const uint32_t *rowStart,
-                        const uint32_t *jStart, const uint32_t *jlist, const double *geo,
-                        uint32_t *adjStart, uint32_t *adjK, double *adjW) {
-    k_adjacency<<<nblk((int64_t)n + 1, 256), 256, 0, st>>>(n, rowStart, jStart, jlist, geo, adjStart,
-                                                          adjK, adjW);
+HipErr launch_adjacency(hipStream_t st, int32_t n, uint32_t m, const uint32_t *rowStart,
+                        const uint32_t *jStart, const uint32_t *jlist, const uint32_t *extStart,
+                        const uint32_t *extEnt, uint32_t mc, const double *geo, const uint32_t *sellStart,
+                        uint64_t epad, uint32_t *adjStart, uint32_t *sellK, double *sellW) {
+    k_adjacency<<<nblk((int64_t)n + 1, 256), 256, 0, st>>>(n, m, rowStart, jStart, jlist, extStart, extEnt, mc, geo,
+                                                          sellStart, epad, adjStart, sellK, sellW);
     LAUNCH_CHECK();
     return hip_ok();
 }

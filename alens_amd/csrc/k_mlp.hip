@@ -49,7 +49,7 @@ __device__ inline float sigmoidf(float o) { return 1.0f / (1.0f + __expf(-o)); }
 // ------------------------------------------------------------------------------------------------
 // raw features, 16 floats per constraint (12 used, 4 zero)
 // ------------------------------------------------------------------------------------------------
-__global__ void k_features(uint32_t m, const int32_t *__restrict__ cI, const int32_t *__restrict__ cJ,
+__global__ void k_features(uint32_t m, uint32_t gs, const int32_t *__restrict__ cI, const int32_t *__restrict__ cJ,
                            const double *__restrict__ delta0, const double *__restrict__ geo,
                            const double *__restrict__ posI, const double *__restrict__ posJ,
                            const double *__restrict__ dir, const double *__restrict__ length,
@@ -58,7 +58,7 @@ __global__ void k_features(uint32_t m, const int32_t *__restrict__ cI, const int
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= m) return;
     const int32_t i = cI[k], j = cJ[k];
-    const double *n = geo + 9 * (size_t)k;
+    const double n[3] = {geo[k], geo[(size_t)gs + k], geo[2 * (size_t)gs + k]};  // component-major, stride gs
     const double *pi = posI + 3 * (size_t)k, *pj = posJ + 3 * (size_t)k;
     const double *ti = dir + 3 * (size_t)i, *tj = dir + 3 * (size_t)j;
     const double Li = length[i], Lj = length[j];
@@ -89,11 +89,11 @@ __global__ void k_features(uint32_t m, const int32_t *__restrict__ cI, const int
     x[15] = 0.0f;
 }
 
-HipErr launch_features(hipStream_t st, uint32_t m, const int32_t *cI, const int32_t *cJ, const double *delta0,
+HipErr launch_features(hipStream_t st, uint32_t m, uint32_t gs, const int32_t *cI, const int32_t *cJ, const double *delta0,
                        const double *geo, const double *posI, const double *posJ, const double *dir,
                        const double *length, const double *unc, const double2 *xprev, double dt, float *X) {
     if (m == 0) return hip_ok();
-    k_features<<<nblk(m, 256), 256, 0, st>>>(m, cI, cJ, delta0, geo, posI, posJ, dir, length, unc, xprev, dt, X);
+    k_features<<<nblk(m, 256), 256, 0, st>>>(m, gs, cI, cJ, delta0, geo, posI, posJ, dir, length, unc, xprev, dt, X);
     LAUNCH_CHECK();
     return hip_ok();
 }

@@ -169,48 +169,52 @@ __device__ inline T a_value(const SolveArgsT<T> &a, uint32_t k, T alpha) {
 }
 
 // phase A: per rod, F = sum over incident constraints (ascending k) of v_k * w_e; U = M F
-// (rods of systems that have stopped are skipped)
+// (rods of systems that have stopped are skipped).  Lane i % 64 of a wave walks rod i's entries in
+// the sliced adjacency, so each step of the 64 lanes reads consecutive words.
 template <int MODE, typename T>
 __device__ inline void phase_A(const SolveArgsT<T> &a, const double *s_alpha, const int *s_state) {
     using V2 = typename Vec2<T>::type;
     const int stride = gridDim.x * blockDim.x;
+    const size_t ep = a.epad;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
         const int sy = a.rsys[i];
         if (s_state[sy] != S_ACTIVE) continue;
         const T alpha = (T)s_alpha[sy];
         T f0 = T(0), f1 = T(0), f2 = T(0), f3 = T(0), f4 = T(0), f5 = T(0);
-        uint32_t e = a.adjStart[i];
-        const uint32_t e1 = a.adjStart[i + 1];
-        for (; e + 2 <= e1; e += 2) {
-            const uint32_t k0 = a.adjK[e] & 0x7fffffffu, k1 = a.adjK[e + 1] & 0x7fffffffu;
-            const V2 *w2 = reinterpret_cast<const V2 *>(a.adjW + 6 * (size_t)e);
-            const V2 p0 = w2[0], p1 = w2[1], p2 = w2[2], q0 = w2[3], q1 = w2[4], q2 = w2[5];
+        const uint32_t deg = a.adjStart[i + 1] - a.adjStart[i];
+        size_t idx = (size_t)a.sellStart[i >> 6] + (uint32_t)(i & (kSlice - 1));
+        uint32_t j = 0;
+        for (; j + 2 <= deg; j += 2, idx += 2 * kSlice) {
+            const size_t i1 = idx + kSlice;
+            const uint32_t k0 = a.sellK[idx] & 0x7fffffffu, k1 = a.sellK[i1] & 0x7fffffffu;
+            const T p0 = a.sellW[idx], p1 = a.sellW[ep + idx], p2 = a.sellW[2 * ep + idx];
+            const T p3 = a.sellW[3 * ep + idx], p4 = a.sellW[4 * ep + idx], p5 = a.sellW[5 * ep + idx];
+            const T q0 = a.sellW[i1], q1 = a.sellW[ep + i1], q2 = a.sellW[2 * ep + i1];
+            const T q3 = a.sellW[3 * ep + i1], q4 = a.sellW[4 * ep + i1], q5 = a.sellW[5 * ep + i1];
             const T v0 = a_value<MODE, T>(a, k0, alpha);
             const T v1 = a_value<MODE, T>(a, k1, alpha);
-            f0 = f0 + v0 * p0.x;
-            f1 = f1 + v0 * p0.y;
-            f2 = f2 + v0 * p1.x;
-            f3 = f3 + v0 * p1.y;
-            f4 = f4 + v0 * p2.x;
-            f5 = f5 + v0 * p2.y;
-            f0 = f0 + v1 * q0.x;
-            f1 = f1 + v1 * q0.y;
-            f2 = f2 + v1 * q1.x;
-            f3 = f3 + v1 * q1.y;
-            f4 = f4 + v1 * q2.x;
-            f5 = f5 + v1 * q2.y;
+            f0 = f0 + v0 * p0;
+            f1 = f1 + v0 * p1;
+            f2 = f2 + v0 * p2;
+            f3 = f3 + v0 * p3;
+            f4 = f4 + v0 * p4;
+            f5 = f5 + v0 * p5;
+            f0 = f0 + v1 * q0;
+            f1 = f1 + v1 * q1;
+            f2 = f2 + v1 * q2;
+            f3 = f3 + v1 * q3;
+            f4 = f4 + v1 * q4;
+            f5 = f5 + v1 * q5;
         }
-        if (e < e1) {
-            const uint32_t k0 = a.adjK[e] & 0x7fffffffu;
-            const V2 *w2 = reinterpret_cast<const V2 *>(a.adjW + 6 * (size_t)e);
-            const V2 p0 = w2[0], p1 = w2[1], p2 = w2[2];
+        if (j < deg) {
+            const uint32_t k0 = a.sellK[idx] & 0x7fffffffu;
             const T v0 = a_value<MODE, T>(a, k0, alpha);
-            f0 = f0 + v0 * p0.x;
-            f1 = f1 + v0 * p0.y;
-            f2 = f2 + v0 * p1.x;
-            f3 = f3 + v0 * p1.y;
-            f4 = f4 + v0 * p2.x;
-            f5 = f5 + v0 * p2.y;
+            f0 = f0 + v0 * a.sellW[idx];
+            f1 = f1 + v0 * a.sellW[ep + idx];
+            f2 = f2 + v0 * a.sellW[2 * ep + idx];
+            f3 = f3 + v0 * a.sellW[3 * ep + idx];
+            f4 = f4 + v0 * a.sellW[4 * ep + idx];
+            f5 = f5 + v0 * a.sellW[5 * ep + idx];
         }
         const V2 *m2 = reinterpret_cast<const V2 *>(a.mob + 6 * (size_t)i);
         const V2 m01 = m2[0], m23 = m2[1], m45 = m2[2];
@@ -228,10 +232,9 @@ __device__ inline void phase_A(const SolveArgsT<T> &a, const double *s_alpha, co
 template <typename T>
 __device__ inline T du_row(const SolveArgsT<T> &a, uint32_t k) {
     using V2 = typename Vec2<T>::type;
-    const T *gp = a.geo + 9 * (size_t)k;
     T gk[9];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) gk[q] = gp[q];
+    for (int q = 0; q < 9; ++q) gk[q] = a.geo[(size_t)q * a.m + k];
     const size_t I = (size_t)a.cI[k], J = (size_t)a.cJ[k];
     const V2 *UI = reinterpret_cast<const V2 *>(a.U + 6 * I);
     const V2 *UJ = reinterpret_cast<const V2 *>(a.U + 6 * J);
@@ -285,13 +288,15 @@ __device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, co
                 const uint32_t k = kk[p];
                 const T l = a.lb ? a.lb[k] : T(0);
                 if (MODE == B_ALPHA0) {
-                    const T ag = dt * du[p];
+                    T ag = dt * du[p];
+                    if (a.invk) ag = ag + a.invk[k] * go[p];
                     s0 = s0 + (double)(go[p] * go[p]);
                     s1 = s1 + (double)(go[p] * ag);
                 } else {
                     const T xn = proj(xo[p] - alpha * go[p], l);
                     const T bk = a.b[k];
-                    const T gn = dt * du[p] + bk;
+                    T gn = dt * du[p] + bk;
+                    if (a.invk) gn = gn + a.invk[k] * xn;
                     const double res = (double)fabs(xn - proj(xn - gn, l));
                     if (!isfinite(res))
                         nf = nf + 1.0;
@@ -550,20 +555,22 @@ HipErr launch_init_xg32(hipStream_t st, float2 *xg, const double2 *xg64, uint32_
 // write-back: F_con = D gamma (gamma = solver x), U = U_nc + M F_con; gamma copied out
 // ------------------------------------------------------------------------------------------------
 __global__ void k_writeback(int32_t n, const int32_t *__restrict__ rsys, const SysDev *__restrict__ sys,
-                            const uint32_t *__restrict__ adjStart, const uint32_t *__restrict__ adjK,
-                            const double *__restrict__ adjW, const double2 *__restrict__ xg,
+                            const uint32_t *__restrict__ adjStart, const uint32_t *__restrict__ sellStart,
+                            const uint32_t *__restrict__ sellK, const double *__restrict__ sellW, uint64_t epad,
+                            const double2 *__restrict__ xg,
                             const double *__restrict__ mob, const double *__restrict__ unc,
                             double *__restrict__ F, double *__restrict__ U) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int mono = sys[rsys[i]].mono;
     double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (uint32_t e = adjStart[i]; e < adjStart[i + 1]; ++e) {
-        const uint32_t k = adjK[e] & 0x7fffffffu;
+    const uint32_t deg = adjStart[i + 1] - adjStart[i];
+    size_t idx = (size_t)sellStart[i >> 6] + (uint32_t)(i & (kSlice - 1));
+    for (uint32_t j = 0; j < deg; ++j, idx += kSlice) {
+        const uint32_t k = sellK[idx] & 0x7fffffffu;
         const double v = xg[k].x;
-        const double *w = adjW + 6 * (size_t)e;
 #pragma unroll
-        for (int q = 0; q < 6; ++q) f[q] = f[q] + v * w[q];
+        for (int q = 0; q < 6; ++q) f[q] = f[q] + v * sellW[q * epad + idx];
     }
     double mb[6], u[6];
 #pragma unroll
@@ -576,16 +583,30 @@ __global__ void k_writeback(int32_t n, const int32_t *__restrict__ rsys, const S
     }
 }
 
+// start point of the caller rows k0 .. k0 + me: x = the blocks' gamma, g = 0
+__global__ void k_set_x(double2 *__restrict__ xg, const double *__restrict__ x0, uint32_t me) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < me) xg[e] = make_double2(x0[e], 0.0);
+}
+HipErr launch_set_x(hipStream_t st, double2 *xg, const double *x0, uint32_t me) {
+    if (me == 0) return hip_ok();
+    k_set_x<<<nblk(me, 256), 256, 0, st>>>(xg, x0, me);
+    LAUNCH_CHECK();
+    return hip_ok();
+}
+
 __global__ void k_copy_gamma(const double2 *__restrict__ xg, double *__restrict__ gamma, uint32_t m) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < m) gamma[k] = xg[k].x;
 }
 
 HipErr launch_writeback(hipStream_t st, int32_t n, const int32_t *rsys, const SysDev *sys, const uint32_t *adjStart,
-                        const uint32_t *adjK, const double *adjW, const double2 *xg, const double *mob,
-                        const double *unc, double *F, double *U, double *gamma, uint32_t m) {
+                        const uint32_t *sellStart, const uint32_t *sellK, const double *sellW, uint64_t epad,
+                        const double2 *xg, const double *mob, const double *unc, double *F, double *U,
+                        double *gamma, uint32_t m) {
     if (n > 0) {
-        k_writeback<<<nblk(n, 256), 256, 0, st>>>(n, rsys, sys, adjStart, adjK, adjW, xg, mob, unc, F, U);
+        k_writeback<<<nblk(n, 256), 256, 0, st>>>(n, rsys, sys, adjStart, sellStart, sellK, sellW, epad, xg, mob, unc,
+                                                  F, U);
         LAUNCH_CHECK();
     }
     if (m > 0) {

@@ -53,6 +53,27 @@ struct ConstraintBlock {
     // row-major 3 x 3 (Emat3 in the reference, SRC/TubuleSystem.cpp:742)
     void setStress(const double s[9]) { std::memcpy(stress, s, sizeof stress); }
 
+    alens_con_block toC() const {
+        alens_con_block c{};
+        c.delta0 = delta0;
+        c.gamma = gamma;
+        c.kappa = kappa;
+        c.gamma_lb = gammaLB;
+        c.gid_i = gidI;
+        c.gid_j = gidJ;
+        c.gidx_i = globalIndexI;
+        c.gidx_j = globalIndexJ;
+        std::memcpy(c.norm_i, normI, sizeof normI);
+        std::memcpy(c.norm_j, normJ, sizeof normJ);
+        std::memcpy(c.pos_i, posI, sizeof posI);
+        std::memcpy(c.pos_j, posJ, sizeof posJ);
+        std::memcpy(c.lab_i, labI, sizeof labI);
+        std::memcpy(c.lab_j, labJ, sizeof labJ);
+        std::memcpy(c.stress, stress, sizeof stress);
+        c.one_side = oneSide ? 1 : 0;
+        c.bilateral = bilateral ? 1 : 0;
+        return c;
+    }
     static ConstraintBlock fromC(const alens_con_block &c) {
         ConstraintBlock b(c.delta0, c.gamma, c.gid_i, c.gid_j, c.gidx_i, c.gidx_j, c.norm_i, c.norm_j, c.pos_i,
                           c.pos_j, c.lab_i, c.lab_j, c.one_side != 0, c.bilateral != 0, c.kappa, c.gamma_lb);
@@ -93,12 +114,20 @@ class SylinderSystem {
     void prepareStep() {
         for (auto &q : pool_) q.clear();
     }
-    // runStep(): Brownian + non-constraint velocity, collision collection, BBPGD, write-back
+    // runStep(): Brownian + non-constraint velocity, collision collection, BBPGD over the collision
+    // rows and the pool's blocks (queues concatenated in thread order), write-back
     void runStep(const double *forceTorque = nullptr) {
+        check(ctx_, alens_clear_constraints(ctx_), "clear_constraints");
+        std::vector<alens_con_block> raw;
         for (const auto &q : pool_)
-            if (!q.empty()) throw Error(ALENS_E_ARG, "runStep: external constraint blocks are not supported yet");
+            for (const auto &b : q) raw.push_back(b.toC());
+        if (!raw.empty()) check(ctx_, alens_add_constraints(ctx_, (int64_t)raw.size(), raw.data()), "add_constraints");
         alens_status s = alens_step_constraints(ctx_, forceTorque, &report_);
         if (s != ALENS_OK && s != ALENS_E_NOCONV) check(ctx_, s, "step_constraints");
+    }
+    // calcConStress(): gamma-weighted stress sums of the last step (collision rows, pool rows)
+    void calcConStress(double collision[9], double pool[9]) {
+        check(ctx_, alens_get_con_stress(ctx_, collision, pool), "get_con_stress");
     }
     void stepEuler() { check(ctx_, alens_step_euler(ctx_), "step_euler"); }
 
@@ -107,7 +136,7 @@ class SylinderSystem {
     const alens_solve_report &getReport() const { return report_; }
     alens_ctx *handle() { return ctx_; }
 
-    // this step's collision constraints with their impulses, in canonical order
+    // this step's constraints with their impulses: collision rows in canonical order, then the pool's
     std::vector<ConstraintBlock> getCollisionBlocks() {
         int64_t m = 0;
         check(ctx_, alens_get_num_constraints(ctx_, &m), "get_num_constraints");

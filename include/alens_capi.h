@@ -125,6 +125,18 @@ alens_status alens_sort_rods(alens_ctx *ctx, int32_t *perm_out);  /* within each
 alens_status alens_set_step(alens_ctx *ctx, int64_t step);
 int64_t alens_get_step(const alens_ctx *ctx);
 
+/* Caller-supplied constraint blocks for the next step: the reference's constraint pool, filled
+ * through getConstraintPoolNonConst() by setProteinConstraints (SRC/TubuleSystem.cpp:694-745) and
+ * consumed by runStep().  Blocks accumulate until the next alens_step_constraints, which appends
+ * them after the collision rows in the order given and then clears them.  Rods are named by gid_i /
+ * gid_j (gidx_* are ignored on input); norm_j must equal -norm_i; pos_i / pos_j are the arms from
+ * the rod centres and lab_i / lab_j / stress are passed through.  Bilateral rows are unbounded and
+ * add 1 / kappa (kappa > 0) to the operator's diagonal (a spring: gamma = -kappa * separation);
+ * unilateral rows are bounded below by gamma_lb.  The solve starts from each block's gamma.
+ * One-system contexts only. */
+alens_status alens_add_constraints(alens_ctx *ctx, int64_t m, const alens_con_block *blocks);
+alens_status alens_clear_constraints(alens_ctx *ctx);
+
 /* One constraint step (rodSystem.runStep(), SRC/TubuleSystem.cpp:223): U_nc = M F_ext + U_B,
  * collision collection, BBPGD to tol, write-back F_con = D gamma, U = U_nc + M F_con.
  * f_ext is 6n (force, torque per rod) or NULL. */
@@ -134,15 +146,21 @@ alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_s
  * step counter. */
 alens_status alens_step_euler(alens_ctx *ctx);
 
-/* Constraints of the last step in canonical order (ascending rod index I, then J; I < J).
- * get_constraints fills blocks; get_constraint_arrays fills any non-NULL SoA array
- * (I, J: m; img: 3m periodic image of J; delta0, b: m; nrm, pos_i, pos_j, r_i, r_j: 3m). */
+/* Constraints of the last step: the collision rows in canonical order (ascending rod index I,
+ * then J; I < J), then the caller's rows in the order they were added.  get_constraints fills
+ * blocks (gamma and stress set); get_constraint_arrays fills any non-NULL SoA array (I, J: m; img:
+ * 3m periodic image of J, zero on caller rows; delta0, b: m; nrm, pos_i, pos_j, r_i, r_j: 3m). */
 alens_status alens_get_num_constraints(alens_ctx *ctx, int64_t *m);
 alens_status alens_get_constraints(alens_ctx *ctx, int64_t cap, alens_con_block *out, int64_t *m);
 alens_status alens_get_constraint_arrays(alens_ctx *ctx, int32_t *I, int32_t *J, int32_t *img,
                                          double *delta0, double *nrm, double *pos_i, double *pos_j,
                                          double *r_i, double *r_j, double *b);
 alens_status alens_get_gamma(alens_ctx *ctx, double *gamma);
+/* Constraint stress of the last step (the sums behind calcConStress, SRC/TubuleSystem.cpp:224):
+ * row-major 3 x 3 sums, in row order, of gamma_k * S_k over the collision rows (S_k = (lab_i -
+ * lab_j) (x) norm_i, the virial of the unit force pair) and over the caller's rows (S_k = the block's
+ * stress).  Not divided by a volume.  Either pointer may be NULL. */
+alens_status alens_get_con_stress(alens_ctx *ctx, double collision[9], double external[9]);
 /* u, u_nc, f_con: 6n (vx,vy,vz,wx,wy,wz per rod); any may be NULL. mob: 6n {t, 1/zperp,
  * 1/zpar - 1/zperp, 1/zrot} or NULL. */
 alens_status alens_get_velocity(alens_ctx *ctx, double *u, double *u_nc, double *f_con, double *mob);

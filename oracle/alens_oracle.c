@@ -770,6 +770,45 @@ void orc_rows(int32_t n, const double *center, const double *dir, const double *
     }
 }
 
+/* Rows supplied by the caller (the reference's constraint pool, SRC/TubuleSystem.cpp:694-745): the
+ * normal n = normI (normJ = -normI), arms posI / posJ relative to the rod centres, separation
+ * delta0.  rI = posI x n, rJ = posJ x n and b = delta0 + dt D^T U_nc as for collision rows. */
+void orc_rows_ext(int64_t m, const int32_t *I, const int32_t *J, const double *delta0,
+                  const double *nrm, const double *posI, const double *posJ, const double *unc,
+                  const orc_cfg *cfg, double *rI, double *rJ, double *b) {
+    for (int64_t k = 0; k < m; ++k) {
+        const double *nk = nrm + 3 * k, *pI = posI + 3 * k, *pJ = posJ + 3 * k;
+        const double ri[3] = {pI[1] * nk[2] - pI[2] * nk[1], pI[2] * nk[0] - pI[0] * nk[2],
+                              pI[0] * nk[1] - pI[1] * nk[0]};
+        const double rj[3] = {pJ[1] * nk[2] - pJ[2] * nk[1], pJ[2] * nk[0] - pJ[0] * nk[2],
+                              pJ[0] * nk[1] - pJ[1] * nk[0]};
+        for (int a = 0; a < 3; ++a) {
+            rI[3 * k + a] = ri[a];
+            rJ[3 * k + a] = rj[a];
+        }
+        b[k] = delta0[k] + cfg->dt * dt_row(nk, ri, rj, unc + 6 * (size_t)I[k], unc + 6 * (size_t)J[k]);
+    }
+}
+
+/* Sum over rows, in row order, of gamma_k * S_k (row-major 3 x 3), where S_k is the per-unit-impulse
+ * stress of row k: (labI - labJ) (x) normI for collision rows (the virial of the unit force pair
+ * +n at labI, -n at labJ), or the caller's matrix where one is given (stress != NULL and
+ * use_given[k] != 0). */
+void orc_stress_sum(int64_t m, const double *nrm, const double *labI, const double *labJ,
+                    const double *gamma, const double *stress, const uint8_t *use_given, double out[9]) {
+    for (int q = 0; q < 9; ++q) out[q] = 0.0;
+    for (int64_t k = 0; k < m; ++k) {
+        double S[9];
+        if (stress && use_given && use_given[k]) {
+            for (int q = 0; q < 9; ++q) S[q] = stress[9 * k + q];
+        } else {
+            for (int a = 0; a < 3; ++a)
+                for (int c = 0; c < 3; ++c) S[3 * a + c] = (labI[3 * k + a] - labJ[3 * k + a]) * nrm[3 * k + c];
+        }
+        for (int q = 0; q < 9; ++q) out[q] = out[q] + gamma[k] * S[q];
+    }
+}
+
 /* ------------------------------------------------------------------------------------------------
  * Operator pieces.  D is applied rod-major: rod i sums its incident constraints in ascending k.
  * ------------------------------------------------------------------------------------------------ */
@@ -913,8 +952,8 @@ static void reduce3(int64_t m, const double *a0, const double *b0, const double 
 
 int orc_bbpgd(int32_t n, int64_t m, const int32_t *I, const int32_t *J, const double *nrm,
               const double *rI, const double *rJ, const double *b, const double *lb,
-              const double *mob, const orc_cfg *cfg, const double *x0, double *gamma,
-              orc_report *rep) {
+              const double *invk, const double *mob, const orc_cfg *cfg, const double *x0,
+              double *gamma, orc_report *rep) {
     memset(rep, 0, sizeof(*rep));
     if (m == 0) {
         rep->converged = 1;
@@ -947,6 +986,7 @@ int orc_bbpgd(int32_t n, int64_t m, const int32_t *I, const int32_t *J, const do
         const double l = lb ? lb[k] : 0.0;
         x[k] = xn[k];
         g[k] = dt * du[k] + b[k];
+        if (invk) g[k] = g[k] + invk[k] * xn[k];
         res[k] = fabs(xn[k] - proj(xn[k] - g[k], l));
         if (rcold) rcold[k] = fabs(0.0 - proj(0.0 - b[k], l));
     }
@@ -985,6 +1025,7 @@ int orc_bbpgd(int32_t n, int64_t m, const int32_t *I, const int32_t *J, const do
 #pragma omp parallel for schedule(static) ORC_NT
     for (int64_t k = 0; k < m; ++k) {
         du[k] = dt * du[k];
+        if (invk) du[k] = du[k] + invk[k] * g[k];
         res[k] = 0.0;
     }
     reduce3(m, g, g, g, du, NULL, NULL, res, &rd);
@@ -1001,6 +1042,7 @@ int orc_bbpgd(int32_t n, int64_t m, const int32_t *I, const int32_t *J, const do
         for (int64_t k = 0; k < m; ++k) {
             const double l = lb ? lb[k] : 0.0;
             gn[k] = dt * du[k] + b[k];
+            if (invk) gn[k] = gn[k] + invk[k] * xn[k];
             res[k] = fabs(xn[k] - proj(xn[k] - gn[k], l));
             sv[k] = xn[k] - x[k];
             yv[k] = gn[k] - g[k];

@@ -85,8 +85,13 @@ void orc_apply_DT(int64_t m, const int32_t *I, const int32_t *J, const double *n
                   const double *rI, const double *rJ, const double *U, double *du);
 int  orc_bbpgd(int32_t n, int64_t m, const int32_t *I, const int32_t *J, const double *nrm,
                const double *rI, const double *rJ, const double *b, const double *lb,
-               const double *mob, const orc_cfg *cfg, const double *x0, double *gamma,
-               orc_report *rep);
+               const double *invk, const double *mob, const orc_cfg *cfg, const double *x0,
+               double *gamma, orc_report *rep);
+void orc_rows_ext(int64_t m, const int32_t *I, const int32_t *J, const double *delta0,
+                  const double *nrm, const double *posI, const double *posJ, const double *unc,
+                  const orc_cfg *cfg, double *rI, double *rJ, double *b);
+void orc_stress_sum(int64_t m, const double *nrm, const double *labI, const double *labJ,
+                    const double *gamma, const double *stress, const uint8_t *use_given, double out[9]);
 void orc_writeback(int32_t n, int64_t m, const int32_t *I, const int32_t *J, const double *nrm,
                    const double *rI, const double *rJ, const double *gamma, const double *mob,
                    const double *unc, const orc_cfg *cfg, double *fcon, double *u);

@@ -104,8 +104,10 @@ def lib():
     L.orc_apply_D_M.argtypes = [ctypes.c_int32, ctypes.c_int64, _ip, _ip, _dp, _dp, _dp, _dp, c_cfg, _dp,
                                 _OptDouble, _dp]
     L.orc_apply_DT.argtypes = [ctypes.c_int64, _ip, _ip, _dp, _dp, _dp, _dp, _dp]
-    L.orc_bbpgd.argtypes = [ctypes.c_int32, ctypes.c_int64, _ip, _ip, _dp, _dp, _dp, _dp, _OptDouble, _dp,
-                            c_cfg, _OptDouble, _dp, ctypes.POINTER(OrcReport)]
+    L.orc_bbpgd.argtypes = [ctypes.c_int32, ctypes.c_int64, _ip, _ip, _dp, _dp, _dp, _dp, _OptDouble, _OptDouble,
+                            _dp, c_cfg, _OptDouble, _dp, ctypes.POINTER(OrcReport)]
+    L.orc_rows_ext.argtypes = [ctypes.c_int64, _ip, _ip, _dp, _dp, _dp, _dp, _dp, c_cfg, _dp, _dp, _dp]
+    L.orc_stress_sum.argtypes = [ctypes.c_int64, _dp, _dp, _dp, _dp, _OptDouble, _OptU8, _dp]
     L.orc_bbpgd.restype = ctypes.c_int
     L.orc_writeback.argtypes = [ctypes.c_int32, ctypes.c_int64, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, c_cfg,
                                 _dp, _dp]
@@ -256,14 +258,49 @@ def apply_DT(I, J, rw, U):
     return du
 
 
-def bbpgd(n, I, J, rw, mob, cfg, x0=None, lb=None):
+def bbpgd(n, I, J, rw, mob, cfg, x0=None, lb=None, invk=None):
+    """BBPGD on A = dt D^T M D (+ diag(invk)), bounds x >= lb (lb None = 0)."""
     m = I.size
     gamma = np.zeros(m)
     rep = OrcReport()
-    lib().orc_bbpgd(n, m, I, J, rw["nrm"], rw["rI"], rw["rJ"], rw["b"],
-                    None if lb is None else np.ascontiguousarray(lb, np.float64), mob, ctypes.byref(cfg),
-                    None if x0 is None else np.ascontiguousarray(x0, np.float64), gamma, ctypes.byref(rep))
+    opt = lambda v: None if v is None else np.ascontiguousarray(v, np.float64)
+    lib().orc_bbpgd(n, m, np.ascontiguousarray(I, np.int32), np.ascontiguousarray(J, np.int32), rw["nrm"],
+                    rw["rI"], rw["rJ"], rw["b"], opt(lb), opt(invk), mob, ctypes.byref(cfg), opt(x0), gamma,
+                    ctypes.byref(rep))
     return gamma, rep.as_dict()
+
+
+def rows_ext(ext, unc_v, cfg):
+    """Rows of caller-supplied constraint blocks (dict of arrays: I, J, delta0, nrm, posI, posJ)."""
+    m = len(ext["I"])
+    out = {k: np.zeros(3 * m) for k in ("rI", "rJ")}
+    out["b"] = np.zeros(m)
+    f = lambda v: np.ascontiguousarray(v, np.float64).reshape(-1)
+    lib().orc_rows_ext(m, np.ascontiguousarray(ext["I"], np.int32), np.ascontiguousarray(ext["J"], np.int32),
+                       f(ext["delta0"]), f(ext["nrm"]), f(ext["posI"]), f(ext["posJ"]), unc_v, ctypes.byref(cfg),
+                       out["rI"], out["rJ"], out["b"])
+    return out
+
+
+def ext_bounds(ext):
+    """Lower bound and compliance per caller row: bilateral rows are unbounded and add 1/kappa
+    (kappa > 0) to the diagonal; unilateral rows are bounded below by gamma_lb."""
+    bil = np.asarray(ext["bilateral"]).astype(bool)
+    kap = np.asarray(ext["kappa"], np.float64)
+    lb = np.where(bil, -np.inf, np.asarray(ext["gamma_lb"], np.float64))
+    invk = np.zeros(len(kap))
+    sel = bil & (kap > 0)
+    invk[sel] = 1.0 / kap[sel]
+    return lb, invk
+
+
+def stress_sum(nrm, labI, labJ, gamma, stress=None, use_given=None):
+    m = np.asarray(gamma).size
+    out = np.zeros(9)
+    f = lambda v: np.ascontiguousarray(v, np.float64).reshape(-1)
+    lib().orc_stress_sum(m, f(nrm), f(labI), f(labJ), f(gamma), None if stress is None else f(stress),
+                         None if use_given is None else np.ascontiguousarray(use_given, np.uint8), out)
+    return out.reshape(3, 3)
 
 
 def writeback(n, I, J, rw, gamma, mob, unc_v, cfg):
@@ -282,13 +319,34 @@ def euler(rods, u, cfg):
     return c.reshape(-1, 3), t.reshape(-1, 3)
 
 
-def step(rods, cfg, f_ext=None, x0=None, method="cells"):
-    """One full constraint step (U_nc, pairs, rows, BBPGD, write-back). Returns a dict."""
+def step(rods, cfg, f_ext=None, x0=None, method="cells", ext=None):
+    """One full constraint step (U_nc, pairs, rows, BBPGD, write-back). Returns a dict.
+
+    ext: caller-supplied constraint blocks (dict of arrays I, J, delta0, nrm, posI, posJ, gamma0,
+    bilateral, kappa, gamma_lb), appended after the collision rows in the given order; the solve
+    starts from gamma0 on them (and from x0, or 0, on the collision rows)."""
     n = len(rods["length"])
     mob = mobility(rods, cfg)
     un = unc(rods, mob, cfg, f_ext)
     I, J, img = pairs(rods, cfg, method)
     rw = rows(rods, cfg, I, J, img, un)
-    gamma, rep = bbpgd(n, I, J, rw, mob, cfg, x0=x0)
+    lb = invk = None
+    mc = I.size
+    if ext is not None and len(ext["I"]) > 0:
+        re = rows_ext(ext, un, cfg)
+        lbe, ike = ext_bounds(ext)
+        I = np.concatenate([I, np.asarray(ext["I"], np.int32)])
+        J = np.concatenate([J, np.asarray(ext["J"], np.int32)])
+        rw = dict(rw)
+        for key in ("rI", "rJ", "b"):
+            rw[key] = np.concatenate([rw[key], re[key]])
+        for key in ("nrm", "posI", "posJ", "delta0"):
+            rw[key] = np.concatenate([rw[key], np.asarray(ext[key], np.float64).reshape(-1)])
+        lb = np.concatenate([np.zeros(mc), lbe])
+        invk = np.concatenate([np.zeros(mc), ike])
+        x0 = np.concatenate([np.zeros(mc) if x0 is None else np.asarray(x0, np.float64),
+                             np.asarray(ext["gamma0"], np.float64)])
+    gamma, rep = bbpgd(n, I, J, rw, mob, cfg, x0=x0, lb=lb, invk=invk)
     fcon, u = writeback(n, I, J, rw, gamma, mob, un, cfg)
-    return dict(mob=mob, unc=un, I=I, J=J, img=img, rows=rw, gamma=gamma, report=rep, fcon=fcon, u=u)
+    return dict(mob=mob, unc=un, I=I, J=J, img=img, rows=rw, gamma=gamma, report=rep, fcon=fcon, u=u,
+                n_collision=mc)

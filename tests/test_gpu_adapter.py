@@ -18,3 +18,6 @@ def test_cpp_adapter_runs_steps():
         tok = p.stdout.split()
         vals = dict(zip(tok[0::2], tok[1::2]))
         assert int(vals["M"]) > 50 and vals["converged"] == "1" and vals["steps"] == "3", p.stdout
+        # three stretched springs from the pool come back bilateral, pulling, with trace(stress sum)
+        # = 3 * sum of their gammas (unit matrices given)
+        assert vals["pooled"] == "3" and vals["pull"] == "1" and float(vals["trace"]) < 0, p.stdout

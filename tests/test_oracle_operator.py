@@ -241,3 +241,107 @@ def test_warm_start_fallback_and_failure_codes(sys1):
     bad["b"][5] = np.nan
     _, rn = O.bbpgd(700, I, J, bad, mob, oc)
     assert rn["status"] == 4
+
+
+def _two_rod_row(rng):
+    n = rng.normal(size=3)
+    n /= np.linalg.norm(n)
+    pI, pJ = rng.normal(size=(2, 3)) * 0.3
+    rI, rJ = np.cross(pI, n), np.cross(pJ, n)
+    tI, tJ = [v / np.linalg.norm(v) for v in rng.normal(size=(2, 3))]
+    mob = np.array([*tI, 38.0, 22.0, 450.0, *tJ, 30.0, 18.0, 300.0])
+    a = 1e-4 * (38.0 * (n @ n) + 22.0 * (tI @ n) ** 2 + 450.0 * (rI @ rI)
+                + 30.0 * (n @ n) + 18.0 * (tJ @ n) ** 2 + 300.0 * (rJ @ rJ))
+    return n, rI, rJ, mob, a
+
+
+def test_single_bilateral_spring_closed_form():
+    """A bilateral row is unbounded and adds 1/kappa to A: gamma = -b / (a + 1/kappa) of either sign;
+    kappa <= 0 (rigid) gives the equality constraint gamma = -b / a."""
+    rng = np.random.default_rng(21)
+    for trial in range(24):
+        n, rI, rJ, mob, a = _two_rod_row(rng)
+        b = rng.normal() * 0.01
+        kappa = [0.0, 1.0, 40.0, 1e4][trial % 4]
+        invk = 1.0 / kappa if kappa > 0 else 0.0
+        rw = dict(nrm=n, rI=rI, rJ=rJ, b=np.array([b]))
+        oc = O.make_cfg(dt=1e-4, tol=1e-14, max_ite=1000)
+        g, rep = O.bbpgd(2, np.array([0], np.int32), np.array([1], np.int32), rw, mob, oc,
+                         lb=np.array([-np.inf]), invk=np.array([invk]))
+        assert rep["converged"] == 1
+        assert np.isclose(g[0], -b / (a + invk), rtol=1e-10, atol=1e-16)
+    # the unilateral bound still clips when the row is not bilateral
+    n, rI, rJ, mob, a = _two_rod_row(rng)
+    rw = dict(nrm=n, rI=rI, rJ=rJ, b=np.array([0.01]))
+    g, _ = O.bbpgd(2, np.array([0], np.int32), np.array([1], np.int32), rw, mob, O.make_cfg(dt=1e-4, tol=1e-14),
+                   lb=np.array([0.0]), invk=np.array([0.0]))
+    assert g[0] == 0.0
+
+
+def test_mixed_rows_agree_with_projected_gauss_seidel(sys1):
+    """Collision rows plus caller rows (bilateral springs and rigid links between random rod pairs,
+    one unilateral row with a negative lower bound) against projected Gauss-Seidel on the dense
+    A + diag(1/kappa), bilateral rows unprojected."""
+    rods, cfg, oc, mob, I, J, img, rw = sys1
+    un = O.unc(rods, mob, oc)
+    rng = np.random.default_rng(5)
+    me = 12
+    eI = rng.choice(700, me, replace=False).astype(np.int32)
+    eJ = ((eI + 1 + rng.integers(0, 698, me)) % 700).astype(np.int32)
+    c, t, L = rods["center"], rods["dir"], rods["length"]
+    sI, sJ = (rng.uniform(-0.5, 0.5, me) * L[eI]), (rng.uniform(-0.5, 0.5, me) * L[eJ])
+    P, Q = c[eI] + sI[:, None] * t[eI], c[eJ] + sJ[:, None] * t[eJ]
+    d = np.linalg.norm(P - Q, axis=1)
+    ext = dict(I=eI, J=eJ, delta0=(d - 0.05) * 1e-3, nrm=(P - Q) / d[:, None], posI=sI[:, None] * t[eI],
+               posJ=sJ[:, None] * t[eJ], gamma0=np.zeros(me), bilateral=np.array([1] * (me - 1) + [0]),
+               kappa=np.array([0.0, 5.0, 50.0] * 4)[:me], gamma_lb=np.array([0.0] * (me - 1) + [-0.5]))
+    re = O.rows_ext(ext, un, oc)
+    lbe, ike = O.ext_bounds(ext)
+    II, JJ = np.concatenate([I, eI]), np.concatenate([J, eJ])
+    rw_all = dict(nrm=np.concatenate([rw["nrm"], ext["nrm"].reshape(-1)]),
+                  rI=np.concatenate([rw["rI"], re["rI"]]), rJ=np.concatenate([rw["rJ"], re["rJ"]]),
+                  b=np.concatenate([O.rows(rods, oc, I, J, img, un)["b"], re["b"]]))
+    lb = np.concatenate([np.zeros(I.size), lbe])
+    invk = np.concatenate([np.zeros(I.size), ike])
+    oct = O.cfg_from_dict(dict(cfg, tol=1e-12, max_ite=400000))
+    g, rep = O.bbpgd(700, II, JJ, rw_all, mob, oct, lb=lb, invk=invk)
+    assert rep["converged"] == 1
+    A = _dense_A(700, II, JJ, rw_all, mob, oc) + np.diag(invk)
+    b = rw_all["b"]
+    x = np.zeros(II.size)
+    for _ in range(40000):
+        xo = x.copy()
+        for k in range(II.size):
+            x[k] = max(lb[k], x[k] - (A[k] @ x + b[k]) / A[k, k])
+        if np.max(np.abs(x - xo)) < 1e-15:
+            break
+    assert np.allclose(g, x, rtol=1e-6, atol=1e-9)
+    gr = A @ g + b
+    bil = np.concatenate([np.zeros(I.size, bool), ext["bilateral"].astype(bool)])
+    assert np.max(np.abs(gr[bil])) < 1e-9
+    # springs: gamma = -kappa * (end-of-step separation), the separation from the write-back velocity
+    fcon, u = O.writeback(700, II, JJ, rw_all, g, mob, un, oc)
+    sep = rw_all["b"] - oc.dt * O.apply_DT(II, JJ, rw_all, un) + oc.dt * O.apply_DT(II, JJ, rw_all, u)
+    k0 = I.size
+    for e in range(me):
+        if ext["bilateral"][e] and ext["kappa"][e] > 0:
+            assert np.isclose(g[k0 + e], -ext["kappa"][e] * sep[k0 + e], rtol=1e-6, atol=1e-10)
+        elif ext["bilateral"][e]:
+            assert abs(sep[k0 + e]) < 1e-10
+
+
+def test_stress_sum_single_contact():
+    """Collision stress per unit impulse is (labI - labJ) (x) n = |labI - labJ| n n^T (labI - labJ is
+    parallel to n); the sum is in row order; given matrices replace it row by row."""
+    rng = np.random.default_rng(3)
+    n = rng.normal(size=3)
+    n /= np.linalg.norm(n)
+    labJ = rng.normal(size=3)
+    labI = labJ + 0.04 * n
+    S = O.stress_sum(n, labI, labJ, np.array([2.5]))
+    assert np.allclose(S, 2.5 * 0.04 * np.outer(n, n), rtol=1e-12, atol=1e-18)
+    assert np.allclose(S, S.T, atol=1e-18)
+    given = rng.normal(size=9)
+    S2 = O.stress_sum(np.tile(n, 2), np.tile(labI, 2), np.tile(labJ, 2), np.array([2.5, -1.0]),
+                      stress=np.concatenate([np.zeros(9), given]), use_given=np.array([0, 1], np.uint8))
+    assert np.allclose(S2, S - given.reshape(3, 3), rtol=1e-12, atol=1e-15)
