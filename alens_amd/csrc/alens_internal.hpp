@@ -197,6 +197,8 @@ struct SolveArgsT {
     int32_t nsys;
     int32_t warm;
     int32_t nchunks;
+    int32_t bar_mode;            // grid barrier variant (k_solve.hip)
+    int32_t xcd_map;             // 1: blocks of one XCD take consecutive work (k_solve.hip)
 };
 using SolveArgs = SolveArgsT<double>;
 HipErr launch_bbpgd64(hipStream_t st, SolveArgsT<double> a, int max_blocks, int *grid_used);
@@ -219,6 +221,10 @@ HipErr launch_make_keys(hipStream_t st, uint32_t m, const int32_t *cI, const int
                         const int32_t *gid, const double2 *xg, uint64_t *key, double *gam);
 
 constexpr int kRedQ = 8;  // partial-sum slots per chunk
+// grid-barrier words of the persistent solver (layout in k_solve.hip)
+constexpr int kBarWords = 32;
+constexpr int kBarGroups = 8;
+constexpr int kBarTotalWords = (2 + kBarGroups) * kBarWords;
 
 // learned warm start (k_mlp.hip): 12 -> 64 -> 64 -> 64 -> 1; W1 stored 64 x 16 (columns 12..15 zero)
 constexpr int kMlpIn = 12;

@@ -79,6 +79,7 @@ struct alens_ctx {
     DBuf<double2> xprev;  // previous-step gamma matched to this step's rows (x component)
     bool have_xprev = false;
     DBuf<uint32_t> bar;
+    int32_t bar_mode = 1, xcd_map = 0;
 
     DBuf<uint64_t> pkey;
     DBuf<double> pgam;
@@ -335,7 +336,7 @@ alens_status alens_set_systems(alens_ctx *ctx, int32_t nsys, const int32_t *rod_
     TRY_HIP(ctx->cstart.ensure((size_t)ncell));
     TRY_HIP(ctx->cend.ensure((size_t)ncell));
     TRY_HIP(ctx->counter.ensure(4));
-    TRY_HIP(ctx->bar.ensure(4));
+    TRY_HIP(ctx->bar.ensure(kBarTotalWords));
     TRY_HIP(ctx->out.ensure(8 * (size_t)nsys + 8));
     TRY_HIP(ctx->sysd.ensure((size_t)nsys));
     TRY_HIP(ctx->smono.ensure((size_t)nsys));
@@ -522,7 +523,7 @@ static alens_status run_bbpgd(alens_ctx *ctx, SolveArgsT<T> a, alens_solve_repor
     const int32_t ns = ctx->nsys_cur;
     TRY_HIP(ctx->part.ensure((size_t)kRedQ * std::max<int32_t>(ctx->nchunks_cur, 1)));
     TRY_HIP(ctx->out.ensure(8 * (size_t)ns + 8));
-    TRY_HIP(hipMemsetAsync(ctx->bar.p, 0, 4 * sizeof(uint32_t), ctx->st));
+    TRY_HIP(hipMemsetAsync(ctx->bar.p, 0, kBarTotalWords * sizeof(uint32_t), ctx->st));
     a.part = ctx->part.p;
     a.bar = ctx->bar.p;
     a.out = ctx->out.p;
@@ -537,6 +538,13 @@ static alens_status run_bbpgd(alens_ctx *ctx, SolveArgsT<T> a, alens_solve_repor
     // ALENS_SOLVER_MAX_BLOCKS caps the persistent solver's grid (tuning; default: all co-resident)
     const char *mb = getenv("ALENS_SOLVER_MAX_BLOCKS");
     const int max_blocks = mb ? atoi(mb) : 0;
+    // ALENS_BARRIER_MODE selects the grid-barrier variant (tuning)
+    const char *bmode = getenv("ALENS_BARRIER_MODE");
+    a.bar_mode = bmode ? atoi(bmode) : ctx->bar_mode;
+    if (a.bar_mode < 0 || a.bar_mode > 2) a.bar_mode = 0;
+    // ALENS_XCD_MAP (0/1): XCD-contiguous work mapping
+    const char *xm = getenv("ALENS_XCD_MAP");
+    a.xcd_map = xm ? atoi(xm) != 0 : ctx->xcd_map;
     TRY_HIP(hipEventRecord(ctx->ev[2], ctx->st));
     if constexpr (sizeof(T) == 8)
         TRY_K(launch_bbpgd64(ctx->st, a, max_blocks, &ctx->grid_used));
@@ -544,11 +552,11 @@ static alens_status run_bbpgd(alens_ctx *ctx, SolveArgsT<T> a, alens_solve_repor
         TRY_K(launch_bbpgd32(ctx->st, a, max_blocks, &ctx->grid_used));
     TRY_HIP(hipEventRecord(ctx->ev[3], ctx->st));
     std::vector<double> o(8 * (size_t)ns);
-    uint32_t barw[4];
+    uint32_t abort_word = 0;
     TRY_HIP(hipMemcpyAsync(o.data(), ctx->out.p, sizeof(double) * o.size(), hipMemcpyDeviceToHost, ctx->st));
-    TRY_HIP(hipMemcpyAsync(barw, ctx->bar.p, sizeof barw, hipMemcpyDeviceToHost, ctx->st));
+    TRY_HIP(hipMemcpyAsync(&abort_word, ctx->bar.p + kBarWords, sizeof abort_word, hipMemcpyDeviceToHost, ctx->st));
     TRY_HIP(hipStreamSynchronize(ctx->st));
-    if (barw[2] != 0) return fail(ctx, ALENS_E_HIP, "BBPGD grid barrier timed out (blocks not co-resident)");
+    if (abort_word != 0) return fail(ctx, ALENS_E_HIP, "BBPGD grid barrier timed out (blocks not co-resident)");
     TRY_HIP(hipEventElapsedTime(ms_solve, ctx->ev[2], ctx->ev[3]));
     *rep = alens_solve_report{};
     rep->converged = 1;
@@ -1160,7 +1168,7 @@ alens_status alens_bcqp_solve(alens_ctx *ctx, int32_t n, int64_t m, const int32_
     TRY_HIP(ctx->gamma.ensure(M));
     TRY_HIP(ctx->Us.ensure(6 * N));
     TRY_HIP(ctx->fext.ensure(6 * N));
-    TRY_HIP(ctx->bar.ensure(4));
+    TRY_HIP(ctx->bar.ensure(kBarTotalWords));
     TRY_HIP(ctx->out.ensure(16));
     TRY_HIP(ctx->delta0.ensure(M));
     hipStream_t st = ctx->st;

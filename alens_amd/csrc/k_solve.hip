@@ -28,29 +28,55 @@ constexpr int kGrp = kBlk / kRedThreads;
 
 enum : int { Q_SS = 0, Q_SY = 1, Q_YY = 2, Q_RMAX = 3, Q_NF = 4, Q_RC = 5, Q_NFC = 6 };
 
-// Grid barrier over co-resident blocks (cooperative launch).  bar[0] counts arrivals monotonically:
-// barrier number e completes when it reaches (e + 1) * gridDim.x.  bar[2] = abort.  Thread 0 of
-// each block publishes the block's writes with one agent-scope release fence, polls with relaxed
-// loads, and takes one agent-scope acquire fence after the wait.
-__device__ inline bool grid_barrier(uint32_t *bar, uint32_t &epoch, int *s_flag) {
+// Grid barrier over co-resident blocks (cooperative launch).  Words (kBarWords apart, one cache
+// line each): bar[0] = arrivals, bar[kBarWords] = abort, bar[(2 + q) * kBarWords] = arrivals of
+// block group q = blockIdx % kBarGroups.  Counters only grow: barrier number e completes when the
+// count reaches (e + 1) * (number of arrivers).  Thread 0 of each block publishes the block's writes
+// with one agent-scope release fence and, after the wait, takes one agent-scope acquire fence.
+//   mode 0: every block adds to bar[0] and polls it.
+//   mode 1: blocks add to their group's counter; the last of a group adds to bar[0]; everyone polls
+//           bar[0] (kBarGroups contenders on bar[0] instead of gridDim.x).
+//   mode 2: as 1, with a longer pause between polls.
+// The wait is bounded: past kSpinLimit polls the abort word is set and every block leaves.
+
+__device__ inline bool grid_barrier(uint32_t *bar, uint32_t &epoch, int *s_flag, int mode) {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        const uint32_t target = (epoch + 1u) * gridDim.x;
-        __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t target;
+        if (mode == 0) {
+            target = (epoch + 1u) * gridDim.x;
+            __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            const uint32_t q = blockIdx.x % kBarGroups;
+            const uint32_t ngroups = gridDim.x < (uint32_t)kBarGroups ? gridDim.x : (uint32_t)kBarGroups;
+            const uint32_t gsize = (gridDim.x - q + kBarGroups - 1) / kBarGroups;
+            target = (epoch + 1u) * ngroups;
+            const uint32_t old = __hip_atomic_fetch_add(&bar[(2 + q) * kBarWords], 1u, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+            if (old + 1u == (epoch + 1u) * gsize) {
+                // the group's last arriver passes the group's published writes on to bar[0]
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+                __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
         uint32_t spins = 0;
         while (__hip_atomic_load(&bar[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            __builtin_amdgcn_s_sleep(1);
+            if (mode == 2)
+                __builtin_amdgcn_s_sleep(8);
+            else
+                __builtin_amdgcn_s_sleep(1);
             if (++spins > kSpinLimit) {
-                __hip_atomic_store(&bar[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&bar[kBarWords], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
             }
-            if ((spins & 1023u) == 0 && __hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            if ((spins & 1023u) == 0 &&
+                __hip_atomic_load(&bar[kBarWords], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
                 break;
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        *s_flag = (int)__hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        *s_flag = (int)__hip_atomic_load(&bar[kBarWords], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     ++epoch;
     __syncthreads();
@@ -124,14 +150,26 @@ __device__ inline void fold_systems(const double *part, int nchunks, const uint3
 #pragma unroll
         for (int q = 0; q < NX; ++q) vx[q] = 0.0;
         if (sy < nsys && state[sy] == S_ACTIVE) {
+            // chunks c, c + 256, ... in order; the loads of kAhead of them are issued together
+            constexpr int kAhead = 4;
             const int c1 = (int)chunk_off[sy + 1];
-            for (int c = (int)chunk_off[sy] + lane; c < c1; c += kRedThreads) {
+            for (int c = (int)chunk_off[sy] + lane; c < c1; c += kAhead * kRedThreads) {
+                double ls[kAhead][NS], lx[kAhead][NX];
 #pragma unroll
-                for (int q = 0; q < NS; ++q) vs[q] = vs[q] + part[(size_t)S[q] * nchunks + c];
+                for (int u = 0; u < kAhead; ++u) {
+                    const int cc = c + u * kRedThreads;
 #pragma unroll
-                for (int q = 0; q < NX; ++q) {
-                    const double v = part[(size_t)X[q] * nchunks + c];
-                    vx[q] = v > vx[q] ? v : vx[q];
+                    for (int q = 0; q < NS; ++q) ls[u][q] = cc < c1 ? part[(size_t)S[q] * nchunks + cc] : 0.0;
+#pragma unroll
+                    for (int q = 0; q < NX; ++q) lx[u][q] = cc < c1 ? part[(size_t)X[q] * nchunks + cc] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < kAhead; ++u) {
+                    if (c + u * kRedThreads >= c1) break;
+#pragma unroll
+                    for (int q = 0; q < NS; ++q) vs[q] = vs[q] + ls[u][q];
+#pragma unroll
+                    for (int q = 0; q < NX; ++q) vx[q] = lx[u][q] > vx[q] ? lx[u][q] : vx[q];
                 }
             }
         }
@@ -161,38 +199,73 @@ __device__ inline float2 mk2<float>(float x, float y) {
     return make_float2(x, y);
 }
 
+// Per-rod data that stays fixed through a solve: system, degree and first slot in the sliced
+// adjacency.  Each thread keeps those of its first rod in registers.
+struct RodMeta {
+    uint32_t deg, idx;
+    int sy;
+};
+template <typename T>
+__device__ inline RodMeta rod_meta(const SolveArgsT<T> &a, int i) {
+    RodMeta r;
+    r.sy = a.rsys[i];
+    r.deg = a.adjStart[i + 1] - a.adjStart[i];
+    r.idx = a.sellStart[i >> 6] + (uint32_t)(i & (kSlice - 1));
+    return r;
+}
+
+// value of the iterate seen by one adjacency entry: x+ = P(x - alpha g) (A_PROJ) or g (A_GRAD),
+// from the constraint's (x, g) and lower bound
 template <int MODE, typename T>
-__device__ inline T a_value(const SolveArgsT<T> &a, uint32_t k, T alpha) {
-    const typename Vec2<T>::type xg = a.xg[k];
+__device__ inline T a_value(typename Vec2<T>::type xg, T l, T alpha) {
     if (MODE == A_GRAD) return xg.y;
-    return proj(xg.x - alpha * xg.y, a.lb ? a.lb[k] : T(0));
+    return proj(xg.x - alpha * xg.y, l);
+}
+
+// Work index of this block.  Workgroups are dealt to the XCDs round-robin (workgroup b runs on XCD
+// b % 8); with xcd_map set, the blocks of one XCD take consecutive work indices, so each XCD walks
+// a contiguous run of rods and constraints and its L2 serves the neighbour gathers.
+__device__ inline int work_block(int xcd_map) {
+    const int b = blockIdx.x, G = gridDim.x;
+    if (!xcd_map || G < kBarGroups) return b;
+    const int x = b % kBarGroups, q = b / kBarGroups;
+    return x * (G / kBarGroups) + (x < G % kBarGroups ? x : G % kBarGroups) + q;
 }
 
 // phase A: per rod, F = sum over incident constraints (ascending k) of v_k * w_e; U = M F
 // (rods of systems that have stopped are skipped).  Lane i % 64 of a wave walks rod i's entries in
 // the sliced adjacency, so each step of the 64 lanes reads consecutive words.
 template <int MODE, typename T>
-__device__ inline void phase_A(const SolveArgsT<T> &a, const double *s_alpha, const int *s_state) {
+__device__ inline void phase_A(const SolveArgsT<T> &a, const double *s_alpha, const int *s_state,
+                               const RodMeta &rm0) {
     using V2 = typename Vec2<T>::type;
     const int stride = gridDim.x * blockDim.x;
+    const int i0 = work_block(a.xcd_map) * blockDim.x + threadIdx.x;
     const size_t ep = a.epad;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
-        const int sy = a.rsys[i];
-        if (s_state[sy] != S_ACTIVE) continue;
-        const T alpha = (T)s_alpha[sy];
+    for (int i = i0; i < a.n; i += stride) {
+        const RodMeta rm = i == i0 ? rm0 : rod_meta(a, i);
+        if (s_state[rm.sy] != S_ACTIVE) continue;
+        const T alpha = (T)s_alpha[rm.sy];
         T f0 = T(0), f1 = T(0), f2 = T(0), f3 = T(0), f4 = T(0), f5 = T(0);
-        const uint32_t deg = a.adjStart[i + 1] - a.adjStart[i];
-        size_t idx = (size_t)a.sellStart[i >> 6] + (uint32_t)(i & (kSlice - 1));
+        // entries in pairs; the constraint indices of the next pair load with this pair's gathers
+        size_t idx = rm.idx;
+        const uint32_t deg = rm.deg;
+        constexpr uint32_t kMask = 0x7fffffffu;
+        uint32_t k0 = deg >= 1 ? a.sellK[idx] & kMask : 0u;
+        uint32_t k1 = deg >= 2 ? a.sellK[idx + kSlice] & kMask : 0u;
         uint32_t j = 0;
         for (; j + 2 <= deg; j += 2, idx += 2 * kSlice) {
             const size_t i1 = idx + kSlice;
-            const uint32_t k0 = a.sellK[idx] & 0x7fffffffu, k1 = a.sellK[i1] & 0x7fffffffu;
+            const V2 x0 = a.xg[k0], x1 = a.xg[k1];
+            const T l0 = a.lb ? a.lb[k0] : T(0), l1 = a.lb ? a.lb[k1] : T(0);
             const T p0 = a.sellW[idx], p1 = a.sellW[ep + idx], p2 = a.sellW[2 * ep + idx];
             const T p3 = a.sellW[3 * ep + idx], p4 = a.sellW[4 * ep + idx], p5 = a.sellW[5 * ep + idx];
             const T q0 = a.sellW[i1], q1 = a.sellW[ep + i1], q2 = a.sellW[2 * ep + i1];
             const T q3 = a.sellW[3 * ep + i1], q4 = a.sellW[4 * ep + i1], q5 = a.sellW[5 * ep + i1];
-            const T v0 = a_value<MODE, T>(a, k0, alpha);
-            const T v1 = a_value<MODE, T>(a, k1, alpha);
+            const uint32_t nk0 = j + 2 < deg ? a.sellK[idx + 2 * kSlice] & kMask : 0u;
+            const uint32_t nk1 = j + 3 < deg ? a.sellK[idx + 3 * kSlice] & kMask : 0u;
+            const T v0 = a_value<MODE, T>(x0, l0, alpha);
+            const T v1 = a_value<MODE, T>(x1, l1, alpha);
             f0 = f0 + v0 * p0;
             f1 = f1 + v0 * p1;
             f2 = f2 + v0 * p2;
@@ -205,10 +278,11 @@ __device__ inline void phase_A(const SolveArgsT<T> &a, const double *s_alpha, co
             f3 = f3 + v1 * q3;
             f4 = f4 + v1 * q4;
             f5 = f5 + v1 * q5;
+            k0 = nk0;
+            k1 = nk1;
         }
         if (j < deg) {
-            const uint32_t k0 = a.sellK[idx] & 0x7fffffffu;
-            const T v0 = a_value<MODE, T>(a, k0, alpha);
+            const T v0 = a_value<MODE, T>(a.xg[k0], a.lb ? a.lb[k0] : T(0), alpha);
             f0 = f0 + v0 * a.sellW[idx];
             f1 = f1 + v0 * a.sellW[ep + idx];
             f2 = f2 + v0 * a.sellW[2 * ep + idx];
@@ -221,13 +295,14 @@ __device__ inline void phase_A(const SolveArgsT<T> &a, const double *s_alpha, co
         const T mb[6] = {m01.x, m01.y, m23.x, m23.y, m45.x, m45.y};
         const T f[6] = {f0, f1, f2, f3, f4, f5};
         T u[6];
-        apply_M(mb, f, a.smono[sy], u);
+        apply_M(mb, f, a.smono[rm.sy], u);
         V2 *U2 = reinterpret_cast<V2 *>(a.U + 6 * (size_t)i);
         U2[0] = mk2<T>(u[0], u[1]);
         U2[1] = mk2<T>(u[2], u[3]);
         U2[2] = mk2<T>(u[4], u[5]);
     }
 }
+
 
 template <typename T>
 __device__ inline T du_row(const SolveArgsT<T> &a, uint32_t k) {
@@ -252,82 +327,164 @@ __device__ inline T du_row(const SolveArgsT<T> &a, uint32_t k) {
 
 enum : int { B_INIT = 0, B_ALPHA0 = 1, B_ITER = 2 };
 
-// phase B: constraints in 512-element chunks, one chunk per 256-lane group (element e of the chunk
-// is lane e % 256, pass e / 256); chunk partials go to part[q * nchunks + c]
+// Phase-B reduction groups.  A 512-element chunk is reduced by kGL lanes; in the fixed order element
+// e belongs to virtual lane e % 256 (pass e / 256), and lane l of a group holds the virtual lanes
+// l + j * kGL (j < kVPT).  Each physical wave folds its kVPT virtual waves separately by halving
+// (lane l += lane l + h, h = 32 .. 1) and the four virtual-wave totals are added left to right, so
+// the result is the one a 256-lane group gives.
+constexpr int kGL = 128;
+constexpr int kVPT = kRedThreads / kGL;
+constexpr int kGrpB = kBlk / kGL;
+
+template <int NQ, bool MAX>
+__device__ inline void vgroup_fold(const double (&v)[NQ][kVPT], double *sh /* kGrpB * NQ * 4 */,
+                                   double (&tot)[NQ]) {
+    const int l = threadIdx.x % kGL, g = threadIdx.x / kGL, lane = l & 63, pw = l >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+#pragma unroll
+        for (int j = 0; j < kVPT; ++j) {
+            double x = v[q][j];
+#pragma unroll
+            for (int h = 32; h >= 1; h >>= 1) {
+                const double o = __shfl_down(x, h, 64);
+                if (MAX)
+                    x = o > x ? o : x;
+                else
+                    x = x + o;
+            }
+            if (lane == 0) sh[(g * NQ + q) * 4 + pw + j * (kGL / 64)] = x;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const double *s4 = sh + (g * NQ + q) * 4;
+        if (MAX) {
+            double t = s4[0];
+            t = s4[1] > t ? s4[1] : t;
+            t = s4[2] > t ? s4[2] : t;
+            t = s4[3] > t ? s4[3] : t;
+            tot[q] = t;
+        } else {
+            tot[q] = ((s4[0] + s4[1]) + s4[2]) + s4[3];
+        }
+    }
+    __syncthreads();
+}
+
+// Chunk table of a block, fixed through a solve: for round r (chunk c = work_block * kGrpB + g +
+// r * gridDim * kGrpB) its system and constraint range.  The first kChunkRounds rounds are kept in
+// LDS; later rounds (very large systems) read the tables.
+constexpr int kChunkRounds = 16;
+struct ChunkMeta {
+    uint32_t k0, k1;
+    int sy;
+};
+template <typename T>
+__device__ inline ChunkMeta chunk_meta(const SolveArgsT<T> &a, int c) {
+    ChunkMeta m{0u, 0u, -1};
+    if (c < a.nchunks) {
+        m.sy = a.chunk_sys[c];
+        m.k0 = a.con_off[m.sy] + (uint32_t)(c - (int)a.chunk_off[m.sy]) * kRedChunk;
+        const uint32_t k1 = m.k0 + kRedChunk, kend = a.con_off[m.sy + 1];
+        m.k1 = k1 < kend ? k1 : kend;
+    }
+    return m;
+}
+
+// phase B: constraints in 512-element chunks, one chunk per kGL-lane group; chunk partials go to
+// part[q * nchunks + c]
 template <int MODE, typename T>
-__device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, const int *s_state, double *sh) {
-    const int g = threadIdx.x >> 8, lane = threadIdx.x & 255;
-    const int ngroups = gridDim.x * kGrp;
+__device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, const int *s_state, double *sh,
+                               const ChunkMeta (*s_cm)[kGrpB]) {
+    constexpr int NE = kVPT * kRedPasses;  // elements per lane
+    const int g = threadIdx.x / kGL, l = threadIdx.x % kGL;
+    const int ngroups = gridDim.x * kGrpB;
     const T dt = (T)a.dt;
-    for (int base = blockIdx.x * kGrp; base < a.nchunks; base += ngroups) {
+    int r = 0;
+    for (int base = work_block(a.xcd_map) * kGrpB; base < a.nchunks; base += ngroups, ++r) {
         const int c = base + g;
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, rmax = 0.0, nf = 0.0, rc = 0.0, nfc = 0.0;
-        const int sy = c < a.nchunks ? a.chunk_sys[c] : 0;
-        const bool act = c < a.nchunks && s_state[sy] == S_ACTIVE;
+        // per virtual lane: sums (s^T s, s^T y, y^T y, non-finite count), max residual, and for a warm
+        // start the cold-start residual and its non-finite count
+        double vs[4][kVPT], vx[1][kVPT], vrc[1][kVPT], vnfc[1][kVPT];
+#pragma unroll
+        for (int j = 0; j < kVPT; ++j) {
+            vs[0][j] = vs[1][j] = vs[2][j] = vs[3][j] = 0.0;
+            vx[0][j] = vrc[0][j] = vnfc[0][j] = 0.0;
+        }
+        const ChunkMeta cm = r < kChunkRounds ? s_cm[r][g] : chunk_meta(a, c);
+        const int sy = cm.sy < 0 ? 0 : cm.sy;
+        const bool act = cm.sy >= 0 && s_state[sy] == S_ACTIVE;
         if (act) {
             const T alpha = (T)s_alpha[sy];
-            const uint32_t k0 = a.con_off[sy] + (uint32_t)(c - (int)a.chunk_off[sy]) * kRedChunk;
-            const uint32_t kend = a.con_off[sy + 1];
-            T xo[kRedPasses], go[kRedPasses], du[kRedPasses];
-            uint32_t kk[kRedPasses];
-            bool ok[kRedPasses];
+            T xo[NE], go[NE], du[NE];
+            uint32_t kk[NE];
+            bool ok[NE];
 #pragma unroll
-            for (int p = 0; p < kRedPasses; ++p) {
-                kk[p] = k0 + p * kRedThreads + lane;
-                ok[p] = kk[p] < kend;
-                if (ok[p]) {
-                    const typename Vec2<T>::type xg = a.xg[kk[p]];
-                    xo[p] = xg.x;
-                    go[p] = xg.y;
-                    du[p] = du_row<T>(a, kk[p]);
+            for (int j = 0; j < kVPT; ++j) {
+#pragma unroll
+                for (int p = 0; p < kRedPasses; ++p) {
+                    const int q = j * kRedPasses + p;
+                    kk[q] = cm.k0 + (uint32_t)(l + j * kGL + p * kRedThreads);
+                    ok[q] = kk[q] < cm.k1;
+                    if (ok[q]) {
+                        const typename Vec2<T>::type xg = a.xg[kk[q]];
+                        xo[q] = xg.x;
+                        go[q] = xg.y;
+                        du[q] = du_row<T>(a, kk[q]);
+                    }
                 }
             }
 #pragma unroll
-            for (int p = 0; p < kRedPasses; ++p) {
-                if (!ok[p]) continue;
-                const uint32_t k = kk[p];
-                const T l = a.lb ? a.lb[k] : T(0);
-                if (MODE == B_ALPHA0) {
-                    T ag = dt * du[p];
-                    if (a.invk) ag = ag + a.invk[k] * go[p];
-                    s0 = s0 + (double)(go[p] * go[p]);
-                    s1 = s1 + (double)(go[p] * ag);
-                } else {
-                    const T xn = proj(xo[p] - alpha * go[p], l);
-                    const T bk = a.b[k];
-                    T gn = dt * du[p] + bk;
-                    if (a.invk) gn = gn + a.invk[k] * xn;
-                    const double res = (double)fabs(xn - proj(xn - gn, l));
-                    if (!isfinite(res))
-                        nf = nf + 1.0;
-                    else if (res > rmax)
-                        rmax = res;
-                    if (MODE == B_INIT) {
-                        if (a.warm) {
-                            const double r0 = (double)fabs(T(0) - proj(T(0) - bk, l));
-                            if (!isfinite(r0))
-                                nfc = nfc + 1.0;
-                            else if (r0 > rc)
-                                rc = r0;
-                        }
+            for (int j = 0; j < kVPT; ++j) {
+#pragma unroll
+                for (int p = 0; p < kRedPasses; ++p) {
+                    const int q = j * kRedPasses + p;
+                    if (!ok[q]) continue;
+                    const uint32_t k = kk[q];
+                    const T lo = a.lb ? a.lb[k] : T(0);
+                    if (MODE == B_ALPHA0) {
+                        T ag = dt * du[q];
+                        if (a.invk) ag = ag + a.invk[k] * go[q];
+                        vs[0][j] = vs[0][j] + (double)(go[q] * go[q]);
+                        vs[1][j] = vs[1][j] + (double)(go[q] * ag);
                     } else {
-                        const T s = xn - xo[p];
-                        const T y = gn - go[p];
-                        s0 = s0 + (double)(s * s);
-                        s1 = s1 + (double)(s * y);
-                        s2 = s2 + (double)(y * y);
+                        const T xn = proj(xo[q] - alpha * go[q], lo);
+                        const T bk = a.b[k];
+                        T gn = dt * du[q] + bk;
+                        if (a.invk) gn = gn + a.invk[k] * xn;
+                        const double res = (double)fabs(xn - proj(xn - gn, lo));
+                        if (!isfinite(res))
+                            vs[3][j] = vs[3][j] + 1.0;
+                        else if (res > vx[0][j])
+                            vx[0][j] = res;
+                        if (MODE == B_INIT) {
+                            if (a.warm) {
+                                const double r0 = (double)fabs(T(0) - proj(T(0) - bk, lo));
+                                if (!isfinite(r0))
+                                    vnfc[0][j] = vnfc[0][j] + 1.0;
+                                else if (r0 > vrc[0][j])
+                                    vrc[0][j] = r0;
+                            }
+                        } else {
+                            const T sv = xn - xo[q];
+                            const T yv = gn - go[q];
+                            vs[0][j] = vs[0][j] + (double)(sv * sv);
+                            vs[1][j] = vs[1][j] + (double)(sv * yv);
+                            vs[2][j] = vs[2][j] + (double)(yv * yv);
+                        }
+                        a.xg[k] = mk2<T>(xn, gn);
                     }
-                    a.xg[k] = mk2<T>(xn, gn);
                 }
             }
         }
-        double vs[4] = {s0, s1, s2, nf}, ts[4];
-        double vx[1] = {rmax}, tx[1];
-        group_sums<4>(vs, sh, ts);
-        group_maxes<1>(vx, sh, tx);
+        double ts[4], tx[1];
+        vgroup_fold<4, false>(vs, sh, ts);
+        vgroup_fold<1, true>(vx, sh, tx);
         double *pc = a.part + c;
         const int nc = a.nchunks;
-        if (act && lane == 0) {
+        if (act && l == 0) {
             if (MODE != B_INIT) {
                 pc[(size_t)Q_SS * nc] = ts[0];
                 pc[(size_t)Q_SY * nc] = ts[1];
@@ -339,10 +496,10 @@ __device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, co
             }
         }
         if (MODE == B_INIT && a.warm) {
-            double v2[1] = {nfc}, t2[1], v3[1] = {rc}, t3[1];
-            group_sums<1>(v2, sh, t2);
-            group_maxes<1>(v3, sh, t3);
-            if (act && lane == 0) {
+            double t2[1], t3[1];
+            vgroup_fold<1, false>(vnfc, sh, t2);
+            vgroup_fold<1, true>(vrc, sh, t3);
+            if (act && l == 0) {
                 pc[(size_t)Q_RC * nc] = t3[0];
                 pc[(size_t)Q_NFC * nc] = t2[0];
             }
@@ -354,13 +511,21 @@ __device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, co
 // alpha at exit.  All blocks hold identical per-system state, so control flow stays uniform.
 template <typename T>
 __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
-    __shared__ double sh[kGrp * 8 * 4];
+    __shared__ double sh[kGrpB * 8 * 4];
     __shared__ int s_flag;
     __shared__ double s_alpha[kMaxSys], s_res0[kMaxSys], s_res[kMaxSys];
     __shared__ int s_state[kMaxSys], s_iters[kMaxSys], s_mv[kMaxSys], s_fb[kMaxSys];
     __shared__ double tot[5][kMaxSys];
     __shared__ int s_any;
+    __shared__ ChunkMeta s_cm[kChunkRounds][kGrpB];
     const int ns = a.nsys;
+    const int bm = a.bar_mode;
+    if (threadIdx.x < kChunkRounds * kGrpB) {
+        const int r = threadIdx.x / kGrpB, g = threadIdx.x % kGrpB;
+        s_cm[r][g] = chunk_meta(a, work_block(a.xcd_map) * kGrpB + g + r * gridDim.x * kGrpB);
+    }
+    const int i0 = work_block(a.xcd_map) * blockDim.x + threadIdx.x;
+    const RodMeta rm0 = i0 < a.n ? rod_meta(a, i0) : RodMeta{0u, 0u, 0};
     for (int q = threadIdx.x; q < ns; q += blockDim.x) {
         s_alpha[q] = 0.0;
         s_state[q] = S_ACTIVE;
@@ -374,10 +539,10 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     uint32_t epoch = 0;
 
     // start point: x = P(x0), g = A x + b
-    phase_A<A_PROJ, T>(a, s_alpha, s_state);
-    if (!grid_barrier(a.bar, epoch, &s_flag)) return;
-    phase_B<B_INIT, T>(a, s_alpha, s_state, sh);
-    if (!grid_barrier(a.bar, epoch, &s_flag)) return;
+    phase_A<A_PROJ, T>(a, s_alpha, s_state, rm0);
+    if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
+    phase_B<B_INIT, T>(a, s_alpha, s_state, sh, s_cm);
+    if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
     {
         const int S[1] = {Q_NF}, X[1] = {Q_RMAX};
         fold_systems<1, 1>(a.part, a.nchunks, a.chunk_off, ns, s_state, S, X, sh, tot);
@@ -407,7 +572,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
             // warm start rejected for those systems: x = 0, g = 0 + b
             for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < a.m; k += gridDim.x * blockDim.x)
                 if (s_fb[a.rsys[a.cI[k]]]) a.xg[k] = mk2<T>(T(0), T(0) + a.b[k]);
-            if (!grid_barrier(a.bar, epoch, &s_flag)) return;
+            if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
         }
         for (int q = threadIdx.x; q < ns; q += blockDim.x) {
             tot[0][q] = tot[2][q];
@@ -430,10 +595,10 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     __syncthreads();
     if (s_any) {
         // alpha0 = g^T g / g^T A g
-        phase_A<A_GRAD, T>(a, s_alpha, s_state);
-        if (!grid_barrier(a.bar, epoch, &s_flag)) return;
-        phase_B<B_ALPHA0, T>(a, s_alpha, s_state, sh);
-        if (!grid_barrier(a.bar, epoch, &s_flag)) return;
+        phase_A<A_GRAD, T>(a, s_alpha, s_state, rm0);
+        if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
+        phase_B<B_ALPHA0, T>(a, s_alpha, s_state, sh, s_cm);
+        if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
         {
             const int S[2] = {Q_SS, Q_SY}, X[1] = {Q_RMAX};
             fold_systems<2, 1>(a.part, a.nchunks, a.chunk_off, ns, s_state, S, X, sh, tot);
@@ -447,10 +612,10 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
         }
         __syncthreads();
         for (int it = 1; it <= a.max_ite; ++it) {
-            phase_A<A_PROJ, T>(a, s_alpha, s_state);
-            if (!grid_barrier(a.bar, epoch, &s_flag)) return;
-            phase_B<B_ITER, T>(a, s_alpha, s_state, sh);
-            if (!grid_barrier(a.bar, epoch, &s_flag)) return;
+            phase_A<A_PROJ, T>(a, s_alpha, s_state, rm0);
+            if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
+            phase_B<B_ITER, T>(a, s_alpha, s_state, sh, s_cm);
+            if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
             {
                 const int S[4] = {Q_SS, Q_SY, Q_YY, Q_NF}, X[1] = {Q_RMAX};
                 fold_systems<4, 1>(a.part, a.nchunks, a.chunk_off, ns, s_state, S, X, sh, tot);
@@ -500,7 +665,7 @@ static HipErr launch_bbpgd(hipStream_t st, SolveArgsT<T> a, int max_blocks, int 
     ALENS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_bbpgd<T>, kBlk, 0));
     if (per_cu < 1) per_cu = 1;
     int64_t need = (int64_t)nblk(a.n, kBlk);
-    const int64_t needB = ((int64_t)a.nchunks + kGrp - 1) / kGrp;
+    const int64_t needB = ((int64_t)a.nchunks + kGrpB - 1) / kGrpB;
     if (needB > need) need = needB;
     if (need < 1) need = 1;
     int64_t lim = (int64_t)ncu * per_cu;

@@ -316,7 +316,9 @@ def test_batched_systems_match_single_system_oracle():
     sreps = ctx.system_reports()
     assert len(sreps) == 4
     for s, (r, cfg) in enumerate(parts):
-        o = O.step(r, O.cfg_from_dict(cfg))
+        # one context = one set of physical settings (seed, dt, ...); only the box is per system
+        o = O.step(r, O.cfg_from_dict(dict(base, box_lo=cfg["box_lo"], box_hi=cfg["box_hi"], pbc=cfg["pbc"],
+                                           monolayer=cfg["monolayer"])))
         lo, hi = offs[s], offs[s + 1]
         sel = (g["I"] >= lo) & (g["I"] < hi)
         assert np.array_equal(g["I"][sel] - lo, o["I"]) and np.array_equal(g["J"][sel] - lo, o["J"])
