@@ -324,7 +324,8 @@ alens_status alens_set_systems(alens_ctx *ctx, int32_t nsys, const int32_t *rod_
     TRY_HIP(ctx->hcol.ensure(N));
     TRY_HIP(ctx->rcol.ensure(N));
     TRY_HIP(ctx->rsys.ensure(N));
-    for (auto *bf : {&ctx->mob, &ctx->unc, &ctx->U, &ctx->F, &ctx->Us, &ctx->fext}) TRY_HIP(bf->ensure(6 * N));
+    for (auto *bf : {&ctx->mob, &ctx->unc, &ctx->U, &ctx->F, &ctx->fext}) TRY_HIP(bf->ensure(6 * N));
+    TRY_HIP(ctx->Us.ensure(kUStride * N));
     TRY_HIP(ctx->clo.ensure(3 * N));
     TRY_HIP(ctx->chi.ensure(3 * N));
     TRY_HIP(ctx->blo.ensure(3 * N));
@@ -621,7 +622,7 @@ static alens_status run_solver_mixed(alens_ctx *ctx, int32_t n, uint32_t m, bool
     TRY_HIP(ctx->b32.ensure(M));
     TRY_HIP(ctx->xg32.ensure(M));
     TRY_HIP(ctx->mob32.ensure(6 * N));
-    TRY_HIP(ctx->U32.ensure(6 * N));
+    TRY_HIP(ctx->U32.ensure(kUStride * N));
     TRY_K(launch_to_f32(st, ctx->geo.p, ctx->geo32.p, 9 * (size_t)m));
     TRY_K(launch_to_f32(st, ctx->sellW.p, ctx->sellW32.p, 6 * (size_t)ctx->epad));
     TRY_K(launch_to_f32(st, ctx->b.p, ctx->b32.p, (size_t)m));
@@ -1166,7 +1167,7 @@ alens_status alens_bcqp_solve(alens_ctx *ctx, int32_t n, int64_t m, const int32_
     TRY_HIP(ctx->b.ensure(M));
     TRY_HIP(ctx->xg.ensure(M));
     TRY_HIP(ctx->gamma.ensure(M));
-    TRY_HIP(ctx->Us.ensure(6 * N));
+    TRY_HIP(ctx->Us.ensure(kUStride * N));
     TRY_HIP(ctx->fext.ensure(6 * N));
     TRY_HIP(ctx->bar.ensure(kBarTotalWords));
     TRY_HIP(ctx->out.ensure(16));

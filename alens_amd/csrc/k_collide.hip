@@ -421,7 +421,11 @@ HipErr launch_cell_ranges(hipStream_t st, const uint32_t *keys, int64_t E, uint3
 // ------------------------------------------------------------------------------------------------
 // One thread per sorted entry e (rod i in cell c): pairs with the later entries of the same cell.
 // Lanes of a wave run the same trip count (the wave maximum) so emission stays wave-aggregated.
-__global__ __launch_bounds__(256) void k_sweep_entries(RodsDev r, CfgDev c, const uint32_t *__restrict__ keys,
+// Found pairs are staged in LDS (an LDS counter per block) and reserved in the global list with one
+// atomic per block at the end; a block that fills its stage reserves the rest one by one.
+constexpr int kSweepT = 256;
+constexpr int kStage = 1024;
+__global__ __launch_bounds__(kSweepT) void k_sweep_entries(RodsDev r, CfgDev c, const uint32_t *__restrict__ keys,
                                                        int64_t E, const uint32_t *__restrict__ cend,
                                                        const uint32_t *__restrict__ vals,
                                                        const int32_t *__restrict__ clo,
@@ -432,9 +436,13 @@ __global__ __launch_bounds__(256) void k_sweep_entries(RodsDev r, CfgDev c, cons
                                                        uint32_t *__restrict__ pImg,
                                                        uint32_t *__restrict__ cntI,
                                                        uint32_t *__restrict__ cntJ) {
+    __shared__ uint32_t s_n, s_base;
+    __shared__ uint32_t s_pi[kStage], s_pj[kStage], s_pm[kStage];
     const int lane = threadIdx.x & 63;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
     uint32_t cnt = 0, i = 0, s1 = 0;
     int cc[3] = {0, 0, 0}, cli[3] = {0, 0, 0}, ui[3] = {0, 0, 0};
     double bli[3] = {0, 0, 0}, bhi_i[3] = {0, 0, 0}, ci[3] = {0, 0, 0}, ti[3] = {0, 0, 0};
@@ -508,18 +516,38 @@ __global__ __launch_bounds__(256) void k_sweep_entries(RodsDev r, CfgDev c, cons
         if (mask) {
             const int leader = __ffsll((unsigned long long)mask) - 1;
             uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(pair_count, (uint32_t)__popcll(mask));
+            if (lane == leader) base = atomicAdd(&s_n, (uint32_t)__popcll(mask));
             base = __shfl(base, leader, 64);
             if (pass) {
-                const uint32_t slot = base + (uint32_t)__popcll(mask & lt);
-                if (slot < cap) {
-                    pI[slot] = i;
-                    pJ[slot] = j;
-                    pImg[slot] = img_pack(m[0], m[1], m[2]);
+                const uint32_t q = base + (uint32_t)__popcll(mask & lt);
+                const uint32_t im = img_pack(m[0], m[1], m[2]);
+                if (q < (uint32_t)kStage) {
+                    s_pi[q] = i;
+                    s_pj[q] = j;
+                    s_pm[q] = im;
+                } else {
+                    const uint32_t slot = atomicAdd(pair_count, 1u);
+                    if (slot < cap) {
+                        pI[slot] = i;
+                        pJ[slot] = j;
+                        pImg[slot] = im;
+                    }
                 }
                 atomicAdd(&cntI[i], 1u);
                 atomicAdd(&cntJ[j], 1u);
             }
+        }
+    }
+    __syncthreads();
+    const uint32_t ns = s_n < (uint32_t)kStage ? s_n : (uint32_t)kStage;
+    if (threadIdx.x == 0) s_base = ns ? atomicAdd(pair_count, ns) : 0u;
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < ns; q += kSweepT) {
+        const uint32_t slot = s_base + q;
+        if (slot < cap) {
+            pI[slot] = s_pi[q];
+            pJ[slot] = s_pj[q];
+            pImg[slot] = s_pm[q];
         }
     }
 }
@@ -529,7 +557,7 @@ HipErr launch_sweep(hipStream_t st, const RodsDev &r, const CfgDev &c, const uin
                     const double *bhi, uint32_t *pair_count, uint32_t cap, uint32_t *pI, uint32_t *pJ,
                     uint32_t *pImg, uint32_t *cntI, uint32_t *cntJ) {
     if (E == 0) return hip_ok();
-    k_sweep_entries<<<nblk(E, 256), 256, 0, st>>>(r, c, keys, E, cend, vals, clo, blo, bhi, pair_count, cap, pI,
+    k_sweep_entries<<<nblk(E, kSweepT), kSweepT, 0, st>>>(r, c, keys, E, cend, vals, clo, blo, bhi, pair_count, cap, pI,
                                                   pJ, pImg, cntI, cntJ);
     LAUNCH_CHECK();
     return hip_ok();

@@ -296,7 +296,7 @@ __device__ inline void phase_A(const SolveArgsT<T> &a, const double *s_alpha, co
         const T f[6] = {f0, f1, f2, f3, f4, f5};
         T u[6];
         apply_M(mb, f, a.smono[rm.sy], u);
-        V2 *U2 = reinterpret_cast<V2 *>(a.U + 6 * (size_t)i);
+        V2 *U2 = reinterpret_cast<V2 *>(a.U + kUStride * (size_t)i);
         U2[0] = mk2<T>(u[0], u[1]);
         U2[1] = mk2<T>(u[2], u[3]);
         U2[2] = mk2<T>(u[4], u[5]);
@@ -311,8 +311,8 @@ __device__ inline T du_row(const SolveArgsT<T> &a, uint32_t k) {
 #pragma unroll
     for (int q = 0; q < 9; ++q) gk[q] = a.geo[(size_t)q * a.m + k];
     const size_t I = (size_t)a.cI[k], J = (size_t)a.cJ[k];
-    const V2 *UI = reinterpret_cast<const V2 *>(a.U + 6 * I);
-    const V2 *UJ = reinterpret_cast<const V2 *>(a.U + 6 * J);
+    const V2 *UI = reinterpret_cast<const V2 *>(a.U + kUStride * I);
+    const V2 *UJ = reinterpret_cast<const V2 *>(a.U + kUStride * J);
     T uI[6], uJ[6];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {

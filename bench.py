@@ -56,7 +56,13 @@ def main():
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="before the run, one device-to-device copy of this many MiB (a kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--rocprof-summary", nargs=3, metavar=("TRACE_DIR", "FETCH_DIR", "WRITE_DIR"),
+                    help="summarise rocprofv3 output of a bench run made with the same --warmup / --steps "
+                         "(--no-form --no-cpu --calib-copy-mib): prints markdown, writes --traffic-out")
+    ap.add_argument("--traffic-out", default=os.path.join(ROOT, "profiles", "k_bbpgd_traffic.json"))
     args = ap.parse_args()
+    if args.rocprof_summary:
+        return rocprof_summary(args)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
@@ -135,7 +141,9 @@ def main():
         },
         "bbpgd_iters_per_step": float(iters.mean()),
         "bbpgd_iters_per_step_plain_fp64": float(iters.mean()),
+        "matvecs_per_step": [int(v) for v in mv],
         "constraints_per_step": float(ms.mean()),
+        "constraints_per_timed_step": [int(v) for v in ms],
         "t_collect_ms_mean": float(np.mean([r["t_collect_ms"] for r in reps])),
         "t_solve_ms_mean": float(1e3 * tsolve.mean()),
         "roofline": {
@@ -149,6 +157,16 @@ def main():
             "algorithmic_bytes_per_launch": float(alg / K),
         },
     }
+    # measured HBM traffic per launch: bytes per matvec from the committed rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE passes on this workload, times this run's matvecs per launch
+    tp = os.path.join(ROOT, "profiles", "k_bbpgd_traffic.json")
+    if os.path.exists(tp) and not args.n and args.precision == 0:
+        with open(tp) as fh:
+            tr = json.load(fh)
+        if tr.get("workload", "").split(":")[0] == args.workload:
+            out["roofline"]["traffic"] = tr["traffic_bytes_per_matvec"] * float(mv.mean())
+            out["roofline"]["traffic_source"] = ("profiles/k_bbpgd_traffic.json: rocprofv3 FETCH_SIZE + WRITE_SIZE "
+                                                 f"per matvec ({tr['traffic_over_algorithmic']:.2f} x algorithmic)")
 
     # chosen form on the same workload: FP32 iterations + FP64 residual/refinement, warm start from
     # the previous step's impulses
@@ -202,6 +220,73 @@ def main():
         }
         out["gpu_over_cpu"] = value / (n / tc)
     print(json.dumps(out), flush=True)
+
+
+def _csv(path):
+    import csv
+    with open(path) as fh:
+        return list(csv.DictReader(fh))
+
+
+def _timed_window(rows, warmup):
+    """Dispatches of the timed steps: from the first k_mob_unc after the warm-up steps' solves."""
+    rows = sorted(rows, key=lambda r: int(r["Start_Timestamp"]))
+    solves = [i for i, r in enumerate(rows) if "k_bbpgd" in r["Kernel_Name"]]
+    start = solves[warmup - 1] + 1 if warmup > 0 else 0
+    while start < len(rows) and "k_mob_unc" not in rows[start]["Kernel_Name"]:
+        start += 1
+    return rows[start:]
+
+
+def rocprof_summary(args):
+    """Markdown summary of a profiled bench run (kernel trace plus FETCH_SIZE / WRITE_SIZE passes), and
+    the solver's measured traffic per matvec as JSON for the roofline object."""
+    tdir, fdir, wdir = args.rocprof_summary
+    trace = _timed_window(_csv(os.path.join(tdir, "run_kernel_trace.csv")), args.warmup)
+    stats = {}
+    for r in trace:
+        nm = r["Kernel_Name"].split("(")[0].replace("void ", "")
+        d = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+        e = stats.setdefault(nm, [0, 0])
+        e[0] += 1
+        e[1] += d
+    tot = sum(v[1] for v in stats.values())
+    print("| kernel | calls | total ms | average us | share |")
+    print("|---|---:|---:|---:|---:|")
+    for nm, (c, d) in sorted(stats.items(), key=lambda kv: -kv[1][1])[:15]:
+        print(f"| `{nm}` | {c} | {d / 1e6:.3f} | {d / c / 1e3:.1f} | {100.0 * d / tot:.1f}% |")
+    out = {}
+    for key, d, cname in (("fetch", fdir, "FETCH_SIZE"), ("write", wdir, "WRITE_SIZE")):
+        rows = [r for r in _csv(os.path.join(d, "run_counter_collection.csv")) if r["Counter_Name"] == cname]
+        rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+        # calibration: the second elementwise kernel is the --calib-copy-mib add (reads N, writes N bytes)
+        calib = [r for r in rows if "elementwise" in r["Kernel_Name"]][1]
+        true_bytes = float(args.calib_copy_mib) * (1 << 20)
+        factor = true_bytes / (float(calib["Counter_Value"]) * 1024.0)
+        solves = [r for r in rows if "k_bbpgd" in r["Kernel_Name"]][args.warmup:]
+        out[key] = dict(factor=factor, counter_kib=[float(r["Counter_Value"]) for r in solves],
+                        bytes=[float(r["Counter_Value"]) * 1024.0 * factor for r in solves])
+    # matvecs per timed solve from the bench line of the FETCH pass (same seed and steps)
+    with open(os.path.join(fdir, "..", os.path.basename(fdir) + ".log")) as fh:
+        js = [json.loads(l) for l in fh if l.startswith("{")][-1]
+    mv = js["matvecs_per_step"]
+    alg = [bytes_per_matvec(int(m), int(js["config"]["rods"])) for m in js["constraints_per_timed_step"]]
+    per_mv = [(f + w) / v for f, w, v in zip(out["fetch"]["bytes"], out["write"]["bytes"], mv)]
+    res = dict(kernel="k_bbpgd<double>", workload=js["config"]["workload"],
+               fetch_factor=out["fetch"]["factor"], write_factor=out["write"]["factor"],
+               traffic_bytes_per_matvec=float(np.mean(per_mv)),
+               algorithmic_bytes_per_matvec=float(np.mean(alg)),
+               traffic_over_algorithmic=float(np.mean(per_mv) / np.mean(alg)))
+    print()
+    print("| solver traffic (timed steps) | value |")
+    print("|---|---:|")
+    print(f"| FETCH_SIZE scale (calibration copy of {args.calib_copy_mib} MiB) | {res['fetch_factor']:.3f} |")
+    print(f"| WRITE_SIZE scale | {res['write_factor']:.3f} |")
+    print(f"| measured bytes per matvec (fetch + write, scaled) | {res['traffic_bytes_per_matvec'] / 1e6:.1f} MB |")
+    print(f"| algorithmic bytes per matvec (200 M + 144 N) | {res['algorithmic_bytes_per_matvec'] / 1e6:.1f} MB |")
+    print(f"| measured / algorithmic | {res['traffic_over_algorithmic']:.2f} |")
+    with open(args.traffic_out, "w") as fh:
+        json.dump(res, fh, indent=1)
 
 
 def bench_replicas(args, capi, systems):
