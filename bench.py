@@ -199,6 +199,8 @@ def main():
         fctx.close()
 
     if not args.no_cpu:
+        # the same step on both sides: from the post-benchmark state, one more GPU step (timed alone)
+        # and the FP64 oracle step on the host cores; the two are bit-identical, iterations included
         from oracle import oracle as O
         threads = args.cpu_threads or len(os.sched_getaffinity(0))
         O.set_threads(threads)
@@ -206,19 +208,25 @@ def main():
         state = dict(rods, center=c, dir=t)
         oc = O.cfg_from_dict(cfg)
         oc.step = ctx.step_count
+        g0 = time.perf_counter()
+        grep = ctx.step()
+        tg = time.perf_counter() - g0
         ts = time.perf_counter()
         res = O.step(state, oc)
-        O.euler(state, res["u"], oc)
         tc = time.perf_counter() - ts
+        same = res["report"]["iters"] == grep["iters"] and res["I"].size == grep["n_con"]
         out["cpu_baseline"] = {
             "value": n / tc,
             "unit": "rod-steps/s",
             "cores": threads,
             "kind": "port",
-            "sample": f"1 full step (FP64 oracle, OpenMP) on the post-benchmark state: {res['I'].size} constraints, "
-                      f"{res['report']['iters']} BBPGD iterations, {tc:.2f} s; CPU {cpu_model()}",
+            "sample": f"1 full step (FP64 oracle, OpenMP) from the post-benchmark state: {res['I'].size} constraints, "
+                      f"{res['report']['iters']} BBPGD iterations, {tc:.3f} s; the GPU took {tg * 1e3:.2f} ms for the "
+                      f"same step ({grep['iters']} iterations{', bit-identical' if same else ', MISMATCH'}); "
+                      f"CPU {cpu_model()}",
         }
         out["gpu_over_cpu"] = value / (n / tc)
+        out["gpu_over_cpu_same_step"] = tc / tg
     print(json.dumps(out), flush=True)
 
 
