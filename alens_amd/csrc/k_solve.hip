@@ -199,6 +199,13 @@ __device__ inline float2 mk2<float>(float x, float y) {
     return make_float2(x, y);
 }
 
+// adjacency entries a phase-A lane loads per step (their loads are in flight together): as many as
+// the 128-register budget of the persistent kernel holds without spilling
+template <typename T>
+struct AGroup {
+    static constexpr int n = sizeof(T) == 8 ? 3 : 4;
+};
+
 // Per-rod data that stays fixed through a solve: system, degree and first slot in the sliced
 // adjacency.  Each thread keeps those of its first rod in registers.
 struct RodMeta {
@@ -247,48 +254,42 @@ __device__ inline void phase_A(const SolveArgsT<T> &a, const double *s_alpha, co
         if (s_state[rm.sy] != S_ACTIVE) continue;
         const T alpha = (T)s_alpha[rm.sy];
         T f0 = T(0), f1 = T(0), f2 = T(0), f3 = T(0), f4 = T(0), f5 = T(0);
-        // entries in pairs; the constraint indices of the next pair load with this pair's gathers
+        // entries in groups of kAG: a group's gathers and weights load together, with the constraint
+        // indices of the next group; the sum still runs over the entries one by one, in order
+        constexpr int kAG = AGroup<T>::n;
         size_t idx = rm.idx;
         const uint32_t deg = rm.deg;
         constexpr uint32_t kMask = 0x7fffffffu;
-        uint32_t k0 = deg >= 1 ? a.sellK[idx] & kMask : 0u;
-        uint32_t k1 = deg >= 2 ? a.sellK[idx + kSlice] & kMask : 0u;
-        uint32_t j = 0;
-        for (; j + 2 <= deg; j += 2, idx += 2 * kSlice) {
-            const size_t i1 = idx + kSlice;
-            const V2 x0 = a.xg[k0], x1 = a.xg[k1];
-            const T l0 = a.lb ? a.lb[k0] : T(0), l1 = a.lb ? a.lb[k1] : T(0);
-            const T p0 = a.sellW[idx], p1 = a.sellW[ep + idx], p2 = a.sellW[2 * ep + idx];
-            const T p3 = a.sellW[3 * ep + idx], p4 = a.sellW[4 * ep + idx], p5 = a.sellW[5 * ep + idx];
-            const T q0 = a.sellW[i1], q1 = a.sellW[ep + i1], q2 = a.sellW[2 * ep + i1];
-            const T q3 = a.sellW[3 * ep + i1], q4 = a.sellW[4 * ep + i1], q5 = a.sellW[5 * ep + i1];
-            const uint32_t nk0 = j + 2 < deg ? a.sellK[idx + 2 * kSlice] & kMask : 0u;
-            const uint32_t nk1 = j + 3 < deg ? a.sellK[idx + 3 * kSlice] & kMask : 0u;
-            const T v0 = a_value<MODE, T>(x0, l0, alpha);
-            const T v1 = a_value<MODE, T>(x1, l1, alpha);
-            f0 = f0 + v0 * p0;
-            f1 = f1 + v0 * p1;
-            f2 = f2 + v0 * p2;
-            f3 = f3 + v0 * p3;
-            f4 = f4 + v0 * p4;
-            f5 = f5 + v0 * p5;
-            f0 = f0 + v1 * q0;
-            f1 = f1 + v1 * q1;
-            f2 = f2 + v1 * q2;
-            f3 = f3 + v1 * q3;
-            f4 = f4 + v1 * q4;
-            f5 = f5 + v1 * q5;
-            k0 = nk0;
-            k1 = nk1;
-        }
-        if (j < deg) {
-            const T v0 = a_value<MODE, T>(a.xg[k0], a.lb ? a.lb[k0] : T(0), alpha);
-            f0 = f0 + v0 * a.sellW[idx];
-            f1 = f1 + v0 * a.sellW[ep + idx];
-            f2 = f2 + v0 * a.sellW[2 * ep + idx];
-            f3 = f3 + v0 * a.sellW[3 * ep + idx];
-            f4 = f4 + v0 * a.sellW[4 * ep + idx];
-            f5 = f5 + v0 * a.sellW[5 * ep + idx];
+        uint32_t kc[kAG];
+#pragma unroll
+        for (int g = 0; g < kAG; ++g) kc[g] = (uint32_t)g < deg ? a.sellK[idx + g * kSlice] & kMask : 0u;
+        for (uint32_t j = 0; j < deg; j += kAG, idx += kAG * kSlice) {
+            V2 xv[kAG];
+            T lv[kAG], w[kAG][6];
+            uint32_t kn[kAG];
+#pragma unroll
+            for (int g = 0; g < kAG; ++g) {
+                const bool on = j + g < deg;
+                const size_t ig = idx + g * kSlice;
+                xv[g] = on ? a.xg[kc[g]] : mk2<T>(T(0), T(0));
+                lv[g] = (on && a.lb) ? a.lb[kc[g]] : T(0);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) w[g][c] = on ? a.sellW[c * ep + ig] : T(0);
+                kn[g] = j + kAG + g < deg ? a.sellK[idx + (kAG + g) * kSlice] & kMask : 0u;
+            }
+#pragma unroll
+            for (int g = 0; g < kAG; ++g) {
+                if (j + g >= deg) break;
+                const T v = a_value<MODE, T>(xv[g], lv[g], alpha);
+                f0 = f0 + v * w[g][0];
+                f1 = f1 + v * w[g][1];
+                f2 = f2 + v * w[g][2];
+                f3 = f3 + v * w[g][3];
+                f4 = f4 + v * w[g][4];
+                f5 = f5 + v * w[g][5];
+            }
+#pragma unroll
+            for (int g = 0; g < kAG; ++g) kc[g] = kn[g];
         }
         const V2 *m2 = reinterpret_cast<const V2 *>(a.mob + 6 * (size_t)i);
         const V2 m01 = m2[0], m23 = m2[1], m45 = m2[2];
