@@ -24,7 +24,7 @@ EXPORTS = [
     "alens_set_step", "alens_get_step", "alens_step_constraints", "alens_step_euler",
     "alens_get_num_constraints", "alens_get_constraints", "alens_get_constraint_arrays", "alens_get_gamma",
     "alens_get_velocity", "alens_bcqp_solve", "alens_add_constraints", "alens_clear_constraints",
-    "alens_get_con_stress",
+    "alens_get_con_stress", "alens_autotune",
     "alens_mlp_set", "alens_mlp_get", "alens_get_features", "alens_mlp_predict", "alens_mlp_gradient",
     "alens_mlp_train",
 ]
@@ -71,6 +71,17 @@ class ConBlock(ctypes.Structure):
         ("stress", ctypes.c_double * 9),
         ("one_side", ctypes.c_int32), ("bilateral", ctypes.c_int32),
     ]
+
+
+class TuneReport(ctypes.Structure):
+    _fields_ = [
+        ("entries", ctypes.c_int32), ("bar_mode", ctypes.c_int32), ("xcd_map", ctypes.c_int32),
+        ("max_blocks", ctypes.c_int32), ("candidates", ctypes.c_int32),
+        ("default_ms", ctypes.c_float), ("best_ms", ctypes.c_float),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class Report(ctypes.Structure):
@@ -246,6 +257,12 @@ class Context:
         self.n = int(off[-1])
         self.rod_offset = off
         return off
+
+    def autotune(self):
+        """Pick the solver schedule by timing the last step's solve under candidates (alens_autotune)."""
+        r = TuneReport()
+        self._check(lib().alens_autotune(self._h, ctypes.byref(r)))
+        return r.as_dict()
 
     def system_reports(self):
         ns = ctypes.c_int32()

@@ -201,8 +201,9 @@ struct SolveArgsT {
     int32_t xcd_map;             // 1: blocks of one XCD take consecutive work (k_solve.hip)
 };
 using SolveArgs = SolveArgsT<double>;
-HipErr launch_bbpgd64(hipStream_t st, SolveArgsT<double> a, int max_blocks, int *grid_used);
-HipErr launch_bbpgd32(hipStream_t st, SolveArgsT<float> a, int max_blocks, int *grid_used);
+// entries: adjacency entries in flight per phase-A step (2, or 0 for the largest instantiated)
+HipErr launch_bbpgd64(hipStream_t st, SolveArgsT<double> a, int max_blocks, int entries, int *grid_used);
+HipErr launch_bbpgd32(hipStream_t st, SolveArgsT<float> a, int max_blocks, int entries, int *grid_used);
 HipErr launch_to_f32(hipStream_t st, const double *src, float *dst, size_t count);
 HipErr launch_xg32_to_x64(hipStream_t st, const float2 *xg32, double *x64, uint32_t m);
 HipErr launch_init_xg32(hipStream_t st, float2 *xg, const double2 *xg64, uint32_t m);

@@ -79,7 +79,9 @@ struct alens_ctx {
     DBuf<double2> xprev;  // previous-step gamma matched to this step's rows (x component)
     bool have_xprev = false;
     DBuf<uint32_t> bar;
-    int32_t bar_mode = 1, xcd_map = 0;
+    // solver schedule (the autotuner's choices; environment variables override them unless tuning)
+    int32_t bar_mode = 1, xcd_map = 0, max_blocks = 0, entries = 0;
+    bool tuning = false;
 
     DBuf<uint64_t> pkey;
     DBuf<double> pgam;
@@ -536,21 +538,22 @@ static alens_status run_bbpgd(alens_ctx *ctx, SolveArgsT<T> a, alens_solve_repor
     a.tol = ctx->cfg.tol;
     a.max_ite = ctx->cfg.max_ite;
     a.nchunks = ctx->nchunks_cur;
-    // ALENS_SOLVER_MAX_BLOCKS caps the persistent solver's grid (tuning; default: all co-resident)
-    const char *mb = getenv("ALENS_SOLVER_MAX_BLOCKS");
-    const int max_blocks = mb ? atoi(mb) : 0;
-    // ALENS_BARRIER_MODE selects the grid-barrier variant (tuning)
-    const char *bmode = getenv("ALENS_BARRIER_MODE");
-    a.bar_mode = bmode ? atoi(bmode) : ctx->bar_mode;
+    // schedule: the context's (autotuned) choice, or the environment's (ALENS_SOLVER_MAX_BLOCKS,
+    // ALENS_BARRIER_MODE, ALENS_XCD_MAP, ALENS_SOLVER_ENTRIES) outside of tuning
+    auto env = [&](const char *name, int dflt) {
+        const char *v = ctx->tuning ? nullptr : getenv(name);
+        return v ? atoi(v) : dflt;
+    };
+    const int max_blocks = env("ALENS_SOLVER_MAX_BLOCKS", ctx->max_blocks);
+    a.bar_mode = env("ALENS_BARRIER_MODE", ctx->bar_mode);
     if (a.bar_mode < 0 || a.bar_mode > 2) a.bar_mode = 0;
-    // ALENS_XCD_MAP (0/1): XCD-contiguous work mapping
-    const char *xm = getenv("ALENS_XCD_MAP");
-    a.xcd_map = xm ? atoi(xm) != 0 : ctx->xcd_map;
+    a.xcd_map = env("ALENS_XCD_MAP", ctx->xcd_map) != 0;
+    const int entries = env("ALENS_SOLVER_ENTRIES", ctx->entries);
     TRY_HIP(hipEventRecord(ctx->ev[2], ctx->st));
     if constexpr (sizeof(T) == 8)
-        TRY_K(launch_bbpgd64(ctx->st, a, max_blocks, &ctx->grid_used));
+        TRY_K(launch_bbpgd64(ctx->st, a, max_blocks, entries, &ctx->grid_used));
     else
-        TRY_K(launch_bbpgd32(ctx->st, a, max_blocks, &ctx->grid_used));
+        TRY_K(launch_bbpgd32(ctx->st, a, max_blocks, entries, &ctx->grid_used));
     TRY_HIP(hipEventRecord(ctx->ev[3], ctx->st));
     std::vector<double> o(8 * (size_t)ns);
     uint32_t abort_word = 0;
@@ -1094,6 +1097,83 @@ alens_status alens_add_constraints(alens_ctx *ctx, int64_t m, const alens_con_bl
 alens_status alens_clear_constraints(alens_ctx *ctx) {
     if (!ctx) return ALENS_E_ARG;
     ctx->ext_pending.clear();
+    return ALENS_OK;
+}
+
+alens_status alens_autotune(alens_ctx *ctx, alens_tune_report *rep_out) {
+    if (!ctx) return ALENS_E_ARG;
+    if (!ctx->have_step) return fail(ctx, ALENS_E_STATE, "autotune needs a step's constraints (run a step first)");
+    if (ctx->m == 0) return fail(ctx, ALENS_E_STATE, "autotune: the last step had no constraints");
+    TRY_HIP(hipSetDevice(ctx->dev));
+    const int32_t n = ctx->n;
+    const uint32_t M = (uint32_t)ctx->m;
+    const auto saved_rep = ctx->sysrep;
+    // one candidate = one cold solve of the last step's rows with the precision form in use; the
+    // fastest of two runs counts (results are identical under every schedule)
+    auto run = [&](float *best) -> alens_status {
+        *best = INFINITY;
+        for (int r = 0; r < 2; ++r) {
+            TRY_K(launch_init_xg(ctx->st, ctx->xg.p, nullptr, M));
+            alens_solve_report rep{};
+            float ms = 0.f;
+            alens_status s = ctx->cfg.precision == 1
+                                 ? run_solver_mixed(ctx, n, M, false, &rep, &ms)
+                                 : run_solver(ctx, n, M, ctx->mob.p, false, &rep, &ms, ctx->rsys.p, ctx->smono.p,
+                                              nullptr);
+            if (s != ALENS_OK && s != ALENS_E_NOCONV) return s;
+            *best = std::min(*best, ms);
+        }
+        return ALENS_OK;
+    };
+    alens_tune_report tr{};
+    ctx->tuning = true;
+    // coordinate descent from the defaults over the switches, one axis at a time
+    ctx->bar_mode = 1;
+    ctx->xcd_map = 0;
+    ctx->max_blocks = 0;
+    ctx->entries = 0;
+    float best = 0.f;
+    alens_status s = run(&best);
+    tr.default_ms = best;
+    tr.candidates = 1;
+    const int64_t grid = std::max<int64_t>(ctx->grid_used, 8);  // the default schedule's grid
+    struct Axis {
+        int32_t *field;
+        std::vector<int32_t> values;
+    };
+    std::vector<Axis> axes = {
+        {&ctx->entries, {0, 2}},
+        {&ctx->bar_mode, {1, 0, 2}},
+        {&ctx->xcd_map, {0, 1}},
+        {&ctx->max_blocks, {0, (int32_t)std::max<int64_t>(8, grid * 3 / 4), (int32_t)std::max<int64_t>(8, grid / 2)}},
+    };
+    for (auto &ax : axes) {
+        if (s != ALENS_OK) break;
+        const int32_t keep = *ax.field;
+        int32_t pick = keep;
+        for (int32_t v : ax.values) {
+            if (v == keep) continue;
+            *ax.field = v;
+            float t = 0.f;
+            s = run(&t);
+            if (s != ALENS_OK) break;
+            ++tr.candidates;
+            if (t < best) {
+                best = t;
+                pick = v;
+            }
+        }
+        *ax.field = pick;
+    }
+    ctx->tuning = false;
+    ctx->sysrep = saved_rep;
+    if (s != ALENS_OK) return s;
+    tr.bar_mode = ctx->bar_mode;
+    tr.xcd_map = ctx->xcd_map;
+    tr.max_blocks = ctx->max_blocks;
+    tr.entries = ctx->entries;
+    tr.best_ms = best;
+    if (rep_out) *rep_out = tr;
     return ALENS_OK;
 }
 

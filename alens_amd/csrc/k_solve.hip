@@ -199,11 +199,12 @@ __device__ inline float2 mk2<float>(float x, float y) {
     return make_float2(x, y);
 }
 
-// adjacency entries a phase-A lane loads per step (their loads are in flight together): as many as
-// the 128-register budget of the persistent kernel holds without spilling
+// adjacency entries a phase-A lane loads per step (their loads are in flight together): a template
+// parameter of the solver, at most what the 128-register budget of the persistent kernel holds
+// without spilling (3 in FP64, 4 in FP32); the autotuner picks among the instantiated values
 template <typename T>
 struct AGroup {
-    static constexpr int n = sizeof(T) == 8 ? 3 : 4;
+    static constexpr int max = sizeof(T) == 8 ? 3 : 4;
 };
 
 // Per-rod data that stays fixed through a solve: system, degree and first slot in the sliced
@@ -242,7 +243,7 @@ __device__ inline int work_block(int xcd_map) {
 // phase A: per rod, F = sum over incident constraints (ascending k) of v_k * w_e; U = M F
 // (rods of systems that have stopped are skipped).  Lane i % 64 of a wave walks rod i's entries in
 // the sliced adjacency, so each step of the 64 lanes reads consecutive words.
-template <int MODE, typename T>
+template <int MODE, typename T, int kAG>
 __device__ inline void phase_A(const SolveArgsT<T> &a, const double *s_alpha, const int *s_state,
                                const RodMeta &rm0) {
     using V2 = typename Vec2<T>::type;
@@ -256,7 +257,6 @@ __device__ inline void phase_A(const SolveArgsT<T> &a, const double *s_alpha, co
         T f0 = T(0), f1 = T(0), f2 = T(0), f3 = T(0), f4 = T(0), f5 = T(0);
         // entries in groups of kAG: a group's gathers and weights load together, with the constraint
         // indices of the next group; the sum still runs over the entries one by one, in order
-        constexpr int kAG = AGroup<T>::n;
         size_t idx = rm.idx;
         const uint32_t deg = rm.deg;
         constexpr uint32_t kMask = 0x7fffffffu;
@@ -510,7 +510,7 @@ __device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, co
 
 // Per-system report words in sysout[8 s ..]: iters, matvecs, converged, fallback, status, res0, res,
 // alpha at exit.  All blocks hold identical per-system state, so control flow stays uniform.
-template <typename T>
+template <typename T, int AG>
 __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     __shared__ double sh[kGrpB * 8 * 4];
     __shared__ int s_flag;
@@ -540,7 +540,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     uint32_t epoch = 0;
 
     // start point: x = P(x0), g = A x + b
-    phase_A<A_PROJ, T>(a, s_alpha, s_state, rm0);
+    phase_A<A_PROJ, T, AG>(a, s_alpha, s_state, rm0);
     if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
     phase_B<B_INIT, T>(a, s_alpha, s_state, sh, s_cm);
     if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
@@ -596,7 +596,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     __syncthreads();
     if (s_any) {
         // alpha0 = g^T g / g^T A g
-        phase_A<A_GRAD, T>(a, s_alpha, s_state, rm0);
+        phase_A<A_GRAD, T, AG>(a, s_alpha, s_state, rm0);
         if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
         phase_B<B_ALPHA0, T>(a, s_alpha, s_state, sh, s_cm);
         if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
@@ -613,7 +613,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
         }
         __syncthreads();
         for (int it = 1; it <= a.max_ite; ++it) {
-            phase_A<A_PROJ, T>(a, s_alpha, s_state, rm0);
+            phase_A<A_PROJ, T, AG>(a, s_alpha, s_state, rm0);
             if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
             phase_B<B_ITER, T>(a, s_alpha, s_state, sh, s_cm);
             if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
@@ -658,12 +658,12 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     }
 }
 
-template <typename T>
+template <typename T, int AG>
 static HipErr launch_bbpgd(hipStream_t st, SolveArgsT<T> a, int max_blocks, int *grid_used) {
     int dev = 0, ncu = 0, per_cu = 0;
     ALENS_HIP_TRY(hipGetDevice(&dev));
     ALENS_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    ALENS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_bbpgd<T>, kBlk, 0));
+    ALENS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_bbpgd<T, AG>, kBlk, 0));
     if (per_cu < 1) per_cu = 1;
     int64_t need = (int64_t)nblk(a.n, kBlk);
     const int64_t needB = ((int64_t)a.nchunks + kGrpB - 1) / kGrpB;
@@ -674,15 +674,17 @@ static HipErr launch_bbpgd(hipStream_t st, SolveArgsT<T> a, int max_blocks, int 
     const int grid = (int)(need < lim ? need : lim);
     *grid_used = grid;
     void *args[] = {&a};
-    ALENS_HIP_TRY(hipLaunchCooperativeKernel((const void *)k_bbpgd<T>, dim3(grid), dim3(kBlk), args, 0, st));
+    ALENS_HIP_TRY(hipLaunchCooperativeKernel((const void *)k_bbpgd<T, AG>, dim3(grid), dim3(kBlk), args, 0, st));
     return hip_ok();
 }
 
-HipErr launch_bbpgd64(hipStream_t st, SolveArgsT<double> a, int max_blocks, int *grid_used) {
-    return launch_bbpgd<double>(st, a, max_blocks, grid_used);
+HipErr launch_bbpgd64(hipStream_t st, SolveArgsT<double> a, int max_blocks, int entries, int *grid_used) {
+    if (entries == 2) return launch_bbpgd<double, 2>(st, a, max_blocks, grid_used);
+    return launch_bbpgd<double, AGroup<double>::max>(st, a, max_blocks, grid_used);
 }
-HipErr launch_bbpgd32(hipStream_t st, SolveArgsT<float> a, int max_blocks, int *grid_used) {
-    return launch_bbpgd<float>(st, a, max_blocks, grid_used);
+HipErr launch_bbpgd32(hipStream_t st, SolveArgsT<float> a, int max_blocks, int entries, int *grid_used) {
+    if (entries == 2) return launch_bbpgd<float, 2>(st, a, max_blocks, grid_used);
+    return launch_bbpgd<float, AGroup<float>::max>(st, a, max_blocks, grid_used);
 }
 
 // precision-form conversions

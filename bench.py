@@ -52,6 +52,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-form", action="store_true", help="skip the chosen-form (FP32 + warm start) pass")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default solver schedule (otherwise the autotuner picks it after warm-up)")
     ap.add_argument("--progress", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="before the run, one device-to-device copy of this many MiB (a kernel of known "
@@ -101,6 +103,12 @@ def main():
         if args.progress:
             print(f"warmup {w}: M={rep['n_con']} iters={rep['iters']} step={rep['t_step_ms']:.2f} ms",
                   file=sys.stderr, flush=True)
+    tune = None
+    if not args.no_autotune and args.warmup > 0:
+        # schedule picked on the last warm-up step's rows (outputs of that step are unchanged)
+        ctx.step()
+        tune = ctx.autotune()
+        ctx.euler()
     reps = []
     t0 = time.perf_counter()
     for s in range(args.steps):
@@ -139,6 +147,7 @@ def main():
             "box": list(cfg["box_hi"]),
             "pbc": list(cfg["pbc"]),
         },
+        "solver_schedule": tune,
         "bbpgd_iters_per_step": float(iters.mean()),
         "bbpgd_iters_per_step_plain_fp64": float(iters.mean()),
         "matvecs_per_step": [int(v) for v in mv],
@@ -176,6 +185,10 @@ def main():
         fctx.sort_rods()
         for w in range(args.warmup):
             fctx.step()
+            fctx.euler()
+        if not args.no_autotune and args.warmup > 0:
+            fctx.step()
+            fctx.autotune()
             fctx.euler()
         freps = []
         f0 = time.perf_counter()
@@ -236,13 +249,14 @@ def _csv(path):
         return list(csv.DictReader(fh))
 
 
-def _timed_window(rows, warmup):
-    """Dispatches of the timed steps: from the first k_mob_unc after the warm-up steps' solves."""
+def _timed_window(rows, steps):
+    """Dispatches of the timed steps: from the k_mob_unc that opens the step of the K-th last solve
+    (warm-up and tuning solves come before it)."""
     rows = sorted(rows, key=lambda r: int(r["Start_Timestamp"]))
     solves = [i for i, r in enumerate(rows) if "k_bbpgd" in r["Kernel_Name"]]
-    start = solves[warmup - 1] + 1 if warmup > 0 else 0
-    while start < len(rows) and "k_mob_unc" not in rows[start]["Kernel_Name"]:
-        start += 1
+    start = solves[-steps]
+    while start > 0 and "k_mob_unc" not in rows[start]["Kernel_Name"]:
+        start -= 1
     return rows[start:]
 
 
@@ -250,7 +264,7 @@ def rocprof_summary(args):
     """Markdown summary of a profiled bench run (kernel trace plus FETCH_SIZE / WRITE_SIZE passes), and
     the solver's measured traffic per matvec as JSON for the roofline object."""
     tdir, fdir, wdir = args.rocprof_summary
-    trace = _timed_window(_csv(os.path.join(tdir, "run_kernel_trace.csv")), args.warmup)
+    trace = _timed_window(_csv(os.path.join(tdir, "run_kernel_trace.csv")), args.steps)
     stats = {}
     for r in trace:
         nm = r["Kernel_Name"].split("(")[0].replace("void ", "")
@@ -271,7 +285,7 @@ def rocprof_summary(args):
         calib = [r for r in rows if "elementwise" in r["Kernel_Name"]][1]
         true_bytes = float(args.calib_copy_mib) * (1 << 20)
         factor = true_bytes / (float(calib["Counter_Value"]) * 1024.0)
-        solves = [r for r in rows if "k_bbpgd" in r["Kernel_Name"]][args.warmup:]
+        solves = [r for r in rows if "k_bbpgd" in r["Kernel_Name"]][-args.steps:]
         out[key] = dict(factor=factor, counter_kib=[float(r["Counter_Value"]) for r in solves],
                         bytes=[float(r["Counter_Value"]) * 1024.0 * factor for r in solves])
     # matvecs per timed solve from the bench line of the FETCH pass (same seed and steps)

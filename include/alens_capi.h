@@ -165,6 +165,25 @@ alens_status alens_get_con_stress(alens_ctx *ctx, double collision[9], double ex
  * 1/zpar - 1/zperp, 1/zrot} or NULL. */
 alens_status alens_get_velocity(alens_ctx *ctx, double *u, double *u_nc, double *f_con, double *mob);
 
+/* Solver autotuner (no counterpart in the reference, whose solver runs on Trilinos over MPI):
+ * times cold solves of the last step's rows under candidate schedules - adjacency entries in flight
+ * per rod step (largest, 2), grid-barrier variant (grouped, flat, grouped with longer polls),
+ * XCD-contiguous work mapping, persistent grid size (all co-resident, 3/4, 1/2) - one axis at a time
+ * from the defaults, keeps the fastest in the context for later steps, and reports the choice.
+ * Results are bit-identical under every schedule; only the time changes.  Needs a step with
+ * constraints; leaves that step's outputs untouched.  Environment variables ALENS_SOLVER_ENTRIES,
+ * ALENS_BARRIER_MODE, ALENS_XCD_MAP and ALENS_SOLVER_MAX_BLOCKS override the choice outside tuning. */
+typedef struct {
+    int32_t entries;     /* 0 = largest instantiated (3 in FP64, 4 in FP32), or 2 */
+    int32_t bar_mode;    /* 0 flat, 1 grouped, 2 grouped with longer polls */
+    int32_t xcd_map;     /* 1 = blocks of one XCD take consecutive work */
+    int32_t max_blocks;  /* 0 = all co-resident blocks */
+    int32_t candidates;  /* schedules timed */
+    float default_ms;    /* solve time of the default schedule */
+    float best_ms;       /* solve time of the chosen schedule */
+} alens_tune_report;
+alens_status alens_autotune(alens_ctx *ctx, alens_tune_report *rep);
+
 /* Stand-alone BCQP solve (BCQPSolver::solveBBPGD on A = dt D^T M D, b; reached in the reference
  * through ConstraintSolver, CMakeLists.txt:109-112).  Rows: I, J (m, I < J not required), nrm, r_i,
  * r_j (3m), b (m); mob (6n) as in alens_get_velocity; x0 (m) or NULL; gamma_out (m). */

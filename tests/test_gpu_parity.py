@@ -335,3 +335,29 @@ def test_batched_systems_match_single_system_oracle():
         for k in range(3):
             if cfg["pbc"][k]:
                 assert np.all(cc[:, k] >= cfg["box_lo"][k]) and np.all(cc[:, k] < cfg["box_hi"][k])
+
+
+def test_autotune_keeps_results_bit_identical():
+    """The autotuner times the last step's solve under candidate schedules and keeps the fastest; the
+    step's outputs stay as they were, and later steps are bit-identical to an untuned context."""
+    rods, cfg = systems.scaled("monolayer_slab", 20000, seed=5)
+    for extra in (dict(), dict(precision=1)):
+        a = capi.Context(**dict(cfg, **extra))
+        b = capi.Context(**dict(cfg, **extra))
+        for c in (a, b):
+            c.set_rods(rods)
+            c.sort_rods()
+        a.step()
+        b.step()
+        g_before = a.gamma()
+        tr = a.autotune()
+        assert tr["candidates"] >= 6 and 0 < tr["best_ms"] <= tr["default_ms"], tr
+        assert tr["entries"] in (0, 2) and tr["bar_mode"] in (0, 1, 2) and tr["xcd_map"] in (0, 1), tr
+        assert_bits(a.gamma(), g_before, "gamma after tuning")
+        assert_bits(a.velocity()["u"], b.velocity()["u"], "U after tuning")
+        for s in range(3):
+            a.euler()
+            b.euler()
+            ra, rb = a.step(), b.step()
+            assert ra["iters"] == rb["iters"], (extra, s, ra, rb)
+            assert_bits(a.gamma(), b.gamma(), f"gamma, step {s + 1} after tuning {extra}")
