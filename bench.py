@@ -209,7 +209,30 @@ def main():
             "fallbacks": int(sum(r["fallback_used"] for r in freps)),
             "iteration_ratio_plain_over_form": float(iters.mean() / max((fit32 + fit64).mean(), 1e-9)),
         }
+        # accuracy of the form, stated per result: from the post-benchmark state, the FP32 form (cold) and
+        # the plain FP64 solve (both to residual tol) against an FP64 solve to 1e-11
+        c, t = fctx.get_rods()
+        st = fctx.step_count
         fctx.close()
+        sols = {}
+        for name, kw in (("fp32_form", dict(precision=1)), ("plain_fp64", {}),
+                         ("tight_fp64", dict(tol=1e-11, max_ite=2_000_000))):
+            x = capi.Context(**dict(cfg, **kw))
+            x.set_rods(dict(rods, center=c, dir=t))
+            x.set_step(st)
+            r = x.step()
+            sols[name] = dict(gamma=x.gamma(), u=x.velocity()["u"], res=r["res"], iters=r["iters"] + r["iters_fp32"])
+            x.close()
+        gt, ut = sols["tight_fp64"]["gamma"], sols["tight_fp64"]["u"]
+        gs, us = max(np.max(np.abs(gt)), 1e-300), max(np.max(np.abs(ut)), 1e-300)
+        out["chosen_form"]["accuracy"] = {
+            "reference": "FP64 BBPGD to residual 1e-11 on the same state "
+                         f"({sols['tight_fp64']['iters']} iterations)",
+            **{name: {"gamma_max_rel_err": float(np.max(np.abs(sols[name]["gamma"] - gt)) / gs),
+                      "u_max_rel_err": float(np.max(np.abs(sols[name]["u"] - ut)) / us),
+                      "fp64_residual": float(sols[name]["res"]), "iters": int(sols[name]["iters"])}
+               for name in ("fp32_form", "plain_fp64")},
+        }
 
     if not args.no_cpu:
         # the same step on both sides: from the post-benchmark state, one more GPU step (timed alone)

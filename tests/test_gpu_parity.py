@@ -361,3 +361,72 @@ def test_autotune_keeps_results_bit_identical():
             ra, rb = a.step(), b.step()
             assert ra["iters"] == rb["iters"], (extra, s, ra, rb)
             assert_bits(a.gamma(), b.gamma(), f"gamma, step {s + 1} after tuning {extra}")
+
+
+def _mixed_rods(n, box, seed, sphere_frac=0.3, immovable_frac=0.2):
+    """Rods and spheres (L < 2r), some immovable, uniform in a box: the edge cases of the mobility
+    (sphere drag, zero mobility) and of the segment distance (point segments)."""
+    rng = np.random.default_rng(seed)
+    L = rng.uniform(0.3, 1.0, n)
+    sph = rng.random(n) < sphere_frac
+    L[sph] = rng.uniform(0.0, 0.02, sph.sum())  # shorter than the diameter 0.025: spheres
+    z = rng.uniform(-1, 1, n)
+    ph = rng.uniform(0, 2 * np.pi, n)
+    d = np.stack([np.sqrt(1 - z * z) * np.cos(ph), np.sqrt(1 - z * z) * np.sin(ph), z], 1)
+    rods = dict(gid=np.arange(n, dtype=np.int32), center=rng.uniform(0, box, (n, 3)), dir=d, length=L,
+                radius=np.full(n, 0.0125), immovable=(rng.random(n) < immovable_frac).astype(np.uint8))
+    return rods
+
+
+@pytest.mark.parametrize("pbc", [(1, 1, 1), (0, 0, 0)])
+def test_spheres_and_immovable_rods_bitexact(pbc):
+    rods = _mixed_rods(1500, 3.0, seed=17)
+    cfg = dict(systems.COMMON, box_lo=(0, 0, 0), box_hi=(3.0, 3.0, 3.0), pbc=pbc, monolayer=0)
+    ctx, g = gpu_step(rods, cfg)
+    o = O.step(rods, O.cfg_from_dict(cfg))
+    assert o["I"].size > 100
+    compare_full(rods, cfg, g, o)
+    imm = rods["immovable"].astype(bool)
+    u = g["u"].reshape(-1, 6)
+    assert np.all(u[imm] == 0.0), "immovable rods must not move"
+
+
+def test_external_force_and_monolayer_projection_bitexact():
+    """A non-null F_ext (6 per rod) enters U_nc = M F_ext + U_B, and monolayer = 1 projects it."""
+    rods, cfg = systems.small_3d(n=800, box=3.6, seed=23)
+    rng = np.random.default_rng(4)
+    f_ext = rng.normal(size=6 * 800) * 0.05
+    for mono in (0, 1):
+        c = dict(cfg, monolayer=mono)
+        ctx = capi.Context(**c)
+        ctx.set_rods(rods)
+        rep = ctx.step(f_ext=f_ext)
+        g = ctx.constraint_arrays()
+        g["gamma"] = ctx.gamma()
+        g.update(ctx.velocity())
+        g["report"] = rep
+        o = O.step(rods, O.cfg_from_dict(c), f_ext=f_ext)
+        compare_full(rods, c, g, o)
+        if mono:
+            u = g["u"].reshape(-1, 6)
+            assert np.all(u[:, 2] == 0.0) and np.all(u[:, 3] == 0.0) and np.all(u[:, 4] == 0.0)
+
+
+def test_tiny_systems():
+    """One rod (no constraints) and two touching rods (one constraint): the solver's trivial cases."""
+    cfg = dict(systems.COMMON, box_lo=(0, 0, 0), box_hi=(4, 4, 4), pbc=(1, 1, 1), monolayer=0)
+    one = dict(gid=np.array([7], np.int32), center=np.array([[1.0, 1.0, 1.0]]), dir=np.array([[0.0, 0.0, 1.0]]),
+               length=np.array([1.0]), radius=np.array([0.0125]), immovable=None)
+    ctx, g = gpu_step(one, cfg)
+    assert g["report"]["n_con"] == 0 and g["report"]["converged"] == 1 and g["report"]["iters"] == 0
+    o = O.step(one, O.cfg_from_dict(cfg))
+    assert_bits(g["u"], o["u"], "one rod U")
+    two = dict(gid=np.array([0, 1], np.int32), center=np.array([[1.0, 1.0, 1.0], [1.02, 1.0, 1.0]]),
+               dir=np.array([[0.0, 0.0, 1.0], [0.0, 1.0, 0.0]]), length=np.array([1.0, 1.0]),
+               radius=np.array([0.0125, 0.0125]), immovable=None)
+    quiet = dict(cfg, kT=-1.0)  # no Brownian kick, which could open the 5 nm overlap by itself
+    ctx, g = gpu_step(two, quiet)
+    o = O.step(two, O.cfg_from_dict(quiet))
+    assert g["report"]["n_con"] == 1
+    compare_full(two, quiet, g, o)
+    assert g["gamma"][0] > 0.0  # overlapping (gap -0.005 um): pushed apart
