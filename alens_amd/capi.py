@@ -24,7 +24,7 @@ EXPORTS = [
     "alens_set_step", "alens_get_step", "alens_step_constraints", "alens_step_euler",
     "alens_get_num_constraints", "alens_get_constraints", "alens_get_constraint_arrays", "alens_get_gamma",
     "alens_get_velocity", "alens_bcqp_solve", "alens_add_constraints", "alens_clear_constraints",
-    "alens_get_con_stress", "alens_autotune",
+    "alens_get_con_stress", "alens_autotune", "alens_set_graph", "alens_get_graph_info",
     "alens_mlp_set", "alens_mlp_get", "alens_get_features", "alens_mlp_predict", "alens_mlp_gradient",
     "alens_mlp_train",
 ]
@@ -257,6 +257,15 @@ class Context:
         self.n = int(off[-1])
         self.rod_offset = off
         return off
+
+    def set_graph(self, enable=True):
+        """Run later steps as one captured HIP graph each (alens_set_graph)."""
+        self._check(lib().alens_set_graph(self._h, 1 if enable else 0))
+
+    def graph_info(self):
+        cap, rep = ctypes.c_int32(), ctypes.c_int64()
+        self._check(lib().alens_get_graph_info(self._h, ctypes.byref(cap), ctypes.byref(rep)))
+        return dict(captured=cap.value, replays=rep.value)
 
     def autotune(self):
         """Pick the solver schedule by timing the last step's solve under candidates (alens_autotune)."""

@@ -89,7 +89,17 @@ struct CfgDev {
     double col_buf, dt, kT, mu;
     uint64_t seed;
     int64_t step;
+    const int64_t *step_dev;  // when set, the step counter is read here (a replayed graph's kernels)
 };
+
+// Counts that live on the device (a step that is replayed from a captured graph never reads them
+// back): kernels take the host capacity (grid size, buffer bound) and, when the pointer is set, use
+// min(capacity, *count).
+__device__ inline uint32_t dcount(uint32_t cap, const uint32_t *d) {
+    if (!d) return cap;
+    const uint32_t v = *d;
+    return v < cap ? v : cap;
+}
 
 // ---- scratch for the generic device primitives ----
 struct ScanScratch {
@@ -103,17 +113,21 @@ HipErr launch_aabb(hipStream_t st, const RodsDev &r, const CfgDev &c, int32_t *c
                    double *bhi, uint32_t *ecnt);
 HipErr exclusive_scan_u32(hipStream_t st, const uint32_t *in, uint32_t *out, int64_t n,
                           ScanScratch &s);
+HipErr ensure_scan_scratch(ScanScratch &s, int64_t n);
+HipErr ensure_radix_scratch(DBuf<uint32_t> &hist, ScanScratch &s, int64_t n);
+// entries beyond ecap are not written (the count eoff[n] still tells how many there were)
 HipErr launch_fill_entries(hipStream_t st, const RodsDev &r, const int32_t *clo, const int32_t *chi,
-                           const uint32_t *eoff, uint32_t *keys, uint32_t *vals);
+                           const uint32_t *eoff, uint32_t *keys, uint32_t *vals, uint32_t ecap);
+// n: capacity (grid); nd: device count or NULL
 HipErr radix_sort_pairs(hipStream_t st, uint32_t *keys, uint32_t *vals, uint32_t *keys_alt,
                         uint32_t *vals_alt, int64_t n, int keybits, DBuf<uint32_t> &hist,
-                        ScanScratch &s, bool &result_in_alt);
+                        ScanScratch &s, bool &result_in_alt, const uint32_t *nd = nullptr);
 HipErr launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, int64_t n);
 HipErr launch_cell_ranges(hipStream_t st, const uint32_t *keys, int64_t E, uint32_t *cstart,
-                          uint32_t *cend);
+                          uint32_t *cend, const uint32_t *Ed = nullptr);
 HipErr launch_sweep(hipStream_t st, const RodsDev &r, const CfgDev &c, const uint32_t *keys, int64_t E, const uint32_t *cend, const uint32_t *vals, const int32_t *clo, const double *blo,
                     const double *bhi, uint32_t *pair_count, uint32_t cap, uint32_t *pI, uint32_t *pJ,
-                    uint32_t *pImg, uint32_t *cntI, uint32_t *cntJ);
+                    uint32_t *pImg, uint32_t *cntI, uint32_t *cntJ, const uint32_t *Ed = nullptr);
 HipErr launch_morton_keys(hipStream_t st, const RodsDev &r, int32_t i0, int32_t n, uint32_t *keys,
                           uint32_t *vals);
 HipErr launch_permute_rods(hipStream_t st, int32_t n, const uint32_t *perm, const int32_t *gid,
@@ -123,14 +137,14 @@ HipErr launch_permute_rods(hipStream_t st, int32_t n, const uint32_t *perm, cons
                            double *rcol2, uint8_t *immov2);
 HipErr launch_order_I(hipStream_t st, int32_t n, uint32_t m, const uint32_t *pI,
                       const uint32_t *pJ, const uint32_t *pImg, const uint32_t *rowStart,
-                      uint32_t *cur, int32_t *cI, int32_t *cJ, uint32_t *cImg);
+                      uint32_t *cur, int32_t *cI, int32_t *cJ, uint32_t *cImg, const uint32_t *md = nullptr);
 HipErr launch_order_J(hipStream_t st, int32_t n, uint32_t m, const int32_t *cJ,
-                      const uint32_t *jStart, uint32_t *cur, uint32_t *jlist);
+                      const uint32_t *jStart, uint32_t *cur, uint32_t *jlist, const uint32_t *md = nullptr);
 // constraint geometry is stored component-major: geo[q * gs + k], q = 0..8 = {n, rI, rJ}, gs = the
 // step's total row count (collision rows first, then the caller's rows)
 HipErr launch_rows(hipStream_t st, const RodsDev &r, const CfgDev &c, uint32_t m, uint32_t gs, const int32_t *cI,
                    const int32_t *cJ, const uint32_t *cImg, const double *unc, double *delta0,
-                   double *geo, double *posI, double *posJ, double *b);
+                   double *geo, double *posI, double *posJ, double *b, const uint32_t *md = nullptr);
 HipErr launch_rows_ext(hipStream_t st, uint32_t me, uint32_t k0, uint32_t gs, const int32_t *cI, const int32_t *cJ,
                        const double *nrm, const double *posI, const double *posJ, const double *delta0,
                        const double *unc, double dt, double *geo, double *b);
@@ -150,6 +164,16 @@ HipErr launch_adjacency(hipStream_t st, int32_t n, uint32_t m, const uint32_t *r
                         const uint32_t *jStart, const uint32_t *jlist, const uint32_t *extStart,
                         const uint32_t *extEnt, uint32_t mc, const double *geo, const uint32_t *sellStart,
                         uint64_t epad, uint32_t *adjStart, uint32_t *sellK, double *sellW);
+HipErr launch_guard_entries(hipStream_t st, const uint32_t *eoff_n, uint32_t ecap, uint32_t *ee);
+HipErr launch_guard_pairs(hipStream_t st, const uint32_t *count, uint32_t cap, uint32_t *mv, uint32_t *cntI,
+                          uint32_t *cntJ, int32_t n);
+// Chunk table on the device: con_off[s] = first row of system s (rows grouped by rod I; me caller rows
+// close the last system), chunk_off (scan of the systems' 512-row chunk counts), *nchunks, and
+// chunk_sys for the first chunk_cap chunks.
+HipErr launch_chunk_table(hipStream_t st, int32_t nsys, int32_t n, const SysDev *sys, const uint32_t *rowStart,
+                          uint32_t me, uint32_t *con_off, uint32_t *chunk_off, int32_t *nchunks, int32_t *chunk_sys,
+                          int32_t chunk_cap, const uint32_t *mv);
+HipErr launch_guard_slots(hipStream_t st, const uint32_t *slots, uint32_t cap, uint32_t *mv);
 
 // solve kernels (k_solve.hip)
 template <typename T>
@@ -196,7 +220,9 @@ struct SolveArgsT {
     int32_t max_ite;
     int32_t nsys;
     int32_t warm;
-    int32_t nchunks;
+    int32_t nchunks;             // chunk capacity (stride of part)
+    const int32_t *nch_dev;      // chunk count on the device, or NULL (= nchunks)
+    const uint32_t *m_dev;       // row count on the device, or NULL (= m, which is also the geo stride)
     int32_t bar_mode;            // grid barrier variant (k_solve.hip)
     int32_t xcd_map;             // 1: blocks of one XCD take consecutive work (k_solve.hip)
 };
@@ -210,16 +236,19 @@ HipErr launch_init_xg32(hipStream_t st, float2 *xg, const double2 *xg64, uint32_
 HipErr launch_writeback(hipStream_t st, int32_t n, const int32_t *rsys, const SysDev *sys, const uint32_t *adjStart,
                         const uint32_t *sellStart, const uint32_t *sellK, const double *sellW, uint64_t epad,
                         const double2 *xg, const double *mob, const double *unc, double *F, double *U,
-                        double *gamma, uint32_t m);
+                        double *gamma, uint32_t m, const uint32_t *md = nullptr);
 HipErr launch_euler(hipStream_t st, int32_t n, const int32_t *rsys, const SysDev *sys, double *center, double *dir,
                     const double *u, const CfgDev &c);
 HipErr launch_init_xg(hipStream_t st, double2 *xg, const double *x0, uint32_t m);
 HipErr launch_set_x(hipStream_t st, double2 *xg, const double *x0, uint32_t me);
 HipErr launch_warm_match(hipStream_t st, uint32_t m, const int32_t *cI, const int32_t *cJ,
                          const int32_t *gid, uint32_t mprev, const uint64_t *prev_key,
-                         const double *prev_gamma, double2 *xg);
+                         const double *prev_gamma, double2 *xg, const uint32_t *md = nullptr,
+                         const uint32_t *mprev_d = nullptr);
+// writes the keys of this step's rows; *mprev_d (when set) receives their number
 HipErr launch_make_keys(hipStream_t st, uint32_t m, const int32_t *cI, const int32_t *cJ,
-                        const int32_t *gid, const double2 *xg, uint64_t *key, double *gam);
+                        const int32_t *gid, const double2 *xg, uint64_t *key, double *gam,
+                        const uint32_t *md = nullptr, uint32_t *mprev_d = nullptr);
 
 constexpr int kRedQ = 8;  // partial-sum slots per chunk
 // values per solver velocity row (padding rows to 8 measured slower than the compact 6)
@@ -240,11 +269,12 @@ struct MlpScratch {
 };
 HipErr launch_features(hipStream_t st, uint32_t m, uint32_t gs, const int32_t *cI, const int32_t *cJ, const double *delta0,
                        const double *geo, const double *posI, const double *posJ, const double *dir,
-                       const double *length, const double *unc, const double2 *xprev, double dt, float *X);
+                       const double *length, const double *unc, const double2 *xprev, double dt, float *X,
+                       const uint32_t *md = nullptr);
 HipErr launch_mlp_infer_f32(hipStream_t st, uint32_t m, const float *X, const float *P, const float *norm,
                             float *out);
 HipErr launch_mlp_infer_xg(hipStream_t st, uint32_t m, const float *X, const float *P, const float *norm,
-                           double2 *xg);
+                           double2 *xg, const uint32_t *md = nullptr);
 HipErr launch_gemm_bf16(hipStream_t st, int M, int N, int K, const float *A, int lda, int tA, const float *B,
                         int ldb, int tB, float *C, int ldc, int ksplit);
 HipErr launch_gather_batch(hipStream_t st, int B, const int32_t *idx, const float *X, const float *t, float *XB,

@@ -82,6 +82,25 @@ struct alens_ctx {
     // solver schedule (the autotuner's choices; environment variables override them unless tuning)
     int32_t bar_mode = 1, xcd_map = 0, max_blocks = 0, entries = 0;
     bool tuning = false;
+    // device-resident step (graph mode): capacities, device counts, the captured step
+    bool capture = false;                  // enqueueing into a graph: no host waits, no events
+    uint32_t gstride = 0;                  // row stride of geo (and the solver's row bound without cur_mdev)
+    const uint32_t *cur_mdev = nullptr;    // row count of the current step buffers on the device
+    const int32_t *cur_nchdev = nullptr;   // their chunk count on the device
+    DBuf<double> out32;
+    DBuf<uint32_t> dword;                  // [0] valid entries, [1] valid rows, [2] rows of the previous step
+    DBuf<int32_t> nch_dev;
+    DBuf<int64_t> step_dev;
+    int graph_mode = 0;
+    struct Caps {
+        uint32_t E = 0, M = 0, epad = 0;
+        int32_t nch = 0;
+    } caps;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_key = -1;                    // start-point kind and f_ext presence the graph was captured with
+    int64_t graph_steps = 0;
+    bool have_counts = false;              // E, m, epad hold a step's counts (planning input)
 
     DBuf<uint64_t> pkey;
     DBuf<double> pgam;
@@ -136,6 +155,15 @@ CfgDev make_cfgdev(const alens_ctx *ctx) {
     c.seed = ctx->cfg.seed;
     c.step = ctx->step;
     return c;
+}
+
+// forget a captured step (its buffers may be reallocated, or its settings changed)
+void drop_graph(alens_ctx *ctx) {
+    if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->gexec = nullptr;
+    ctx->graph = nullptr;
+    ctx->graph_key = -1;
 }
 
 RodsDev make_rods(const alens_ctx *ctx) {
@@ -226,6 +254,9 @@ void alens_destroy(alens_ctx *ctx) {
     rel(ctx->cImg); rel(ctx->delta0); rel(ctx->geo); rel(ctx->posI); rel(ctx->posJ); rel(ctx->b);
     rel(ctx->adjStart); rel(ctx->sellStart); rel(ctx->sellK); rel(ctx->slen); rel(ctx->sellW); rel(ctx->xg); rel(ctx->gamma); rel(ctx->part);
     rel(ctx->out); rel(ctx->bar); rel(ctx->pkey); rel(ctx->pgam);
+    rel(ctx->out32); rel(ctx->dword); rel(ctx->nch_dev); rel(ctx->step_dev);
+    if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
     rel(ctx->extStart); rel(ctx->extEnt); rel(ctx->extNrm); rel(ctx->extX0); rel(ctx->lb); rel(ctx->invk);
     rel(ctx->lb32); rel(ctx->invk32); rel(ctx->geo32); rel(ctx->sellW32); rel(ctx->mob32); rel(ctx->b32); rel(ctx->U32); rel(ctx->xg32);
     rel(ctx->mlpP); rel(ctx->mlpNorm); rel(ctx->feat); rel(ctx->xprev);
@@ -239,6 +270,10 @@ void alens_destroy(alens_ctx *ctx) {
 alens_status alens_set_systems(alens_ctx *ctx, int32_t nsys, const int32_t *rod_offset, const alens_box *boxes,
                                const int32_t *gid, const double *center, const double *dir, const double *length,
                                const double *radius, const uint8_t *immovable) {
+    if (ctx) {
+        drop_graph(ctx);
+        ctx->have_counts = false;
+    }
     if (!ctx) return ALENS_E_ARG;
     if (nsys < 1 || nsys > kMaxSys || !rod_offset || !boxes || rod_offset[0] != 0)
         return fail(ctx, ALENS_E_ARG, "set_systems: need 1..256 systems and rod_offset[0] = 0");
@@ -410,6 +445,7 @@ alens_status alens_get_rods(alens_ctx *ctx, double *center, double *dir) {
 }
 
 alens_status alens_sort_rods(alens_ctx *ctx, int32_t *perm_out) {
+    if (ctx) drop_graph(ctx);
     if (!ctx) return ALENS_E_ARG;
     if (!ctx->have_rods) return fail(ctx, ALENS_E_STATE, "sort_rods before set_rods");
     TRY_HIP(hipSetDevice(ctx->dev));
@@ -518,50 +554,17 @@ static alens_status set_chunk_table(alens_ctx *ctx, const std::vector<uint32_t> 
     return ALENS_OK;
 }
 
-// Launches one persistent BBPGD solve on the context's stream and reads the per-system report words;
-// rep receives the aggregate (max iterations / residuals, all converged, any fallback, worst status).
-template <typename T>
-static alens_status run_bbpgd(alens_ctx *ctx, SolveArgsT<T> a, alens_solve_report *rep, float *ms_solve,
-                              std::vector<alens_solve_report> *per_sys) {
+// Per-system report words of a finished solve -> per-system reports and their aggregate (max
+// iterations / residuals, all converged, any fallback, worst status).
+static alens_status read_reports(alens_ctx *ctx, const double *out_dev, alens_solve_report *rep,
+                                 std::vector<alens_solve_report> *per_sys) {
     const int32_t ns = ctx->nsys_cur;
-    TRY_HIP(ctx->part.ensure((size_t)kRedQ * std::max<int32_t>(ctx->nchunks_cur, 1)));
-    TRY_HIP(ctx->out.ensure(8 * (size_t)ns + 8));
-    TRY_HIP(hipMemsetAsync(ctx->bar.p, 0, kBarTotalWords * sizeof(uint32_t), ctx->st));
-    a.part = ctx->part.p;
-    a.bar = ctx->bar.p;
-    a.out = ctx->out.p;
-    a.con_off = ctx->con_off.p;
-    a.chunk_off = ctx->chunk_off.p;
-    a.chunk_sys = ctx->chunk_sys.p;
-    a.nsys = ns;
-    a.dt = ctx->cfg.dt;
-    a.tol = ctx->cfg.tol;
-    a.max_ite = ctx->cfg.max_ite;
-    a.nchunks = ctx->nchunks_cur;
-    // schedule: the context's (autotuned) choice, or the environment's (ALENS_SOLVER_MAX_BLOCKS,
-    // ALENS_BARRIER_MODE, ALENS_XCD_MAP, ALENS_SOLVER_ENTRIES) outside of tuning
-    auto env = [&](const char *name, int dflt) {
-        const char *v = ctx->tuning ? nullptr : getenv(name);
-        return v ? atoi(v) : dflt;
-    };
-    const int max_blocks = env("ALENS_SOLVER_MAX_BLOCKS", ctx->max_blocks);
-    a.bar_mode = env("ALENS_BARRIER_MODE", ctx->bar_mode);
-    if (a.bar_mode < 0 || a.bar_mode > 2) a.bar_mode = 0;
-    a.xcd_map = env("ALENS_XCD_MAP", ctx->xcd_map) != 0;
-    const int entries = env("ALENS_SOLVER_ENTRIES", ctx->entries);
-    TRY_HIP(hipEventRecord(ctx->ev[2], ctx->st));
-    if constexpr (sizeof(T) == 8)
-        TRY_K(launch_bbpgd64(ctx->st, a, max_blocks, entries, &ctx->grid_used));
-    else
-        TRY_K(launch_bbpgd32(ctx->st, a, max_blocks, entries, &ctx->grid_used));
-    TRY_HIP(hipEventRecord(ctx->ev[3], ctx->st));
     std::vector<double> o(8 * (size_t)ns);
     uint32_t abort_word = 0;
-    TRY_HIP(hipMemcpyAsync(o.data(), ctx->out.p, sizeof(double) * o.size(), hipMemcpyDeviceToHost, ctx->st));
+    TRY_HIP(hipMemcpyAsync(o.data(), out_dev, sizeof(double) * o.size(), hipMemcpyDeviceToHost, ctx->st));
     TRY_HIP(hipMemcpyAsync(&abort_word, ctx->bar.p + kBarWords, sizeof abort_word, hipMemcpyDeviceToHost, ctx->st));
     TRY_HIP(hipStreamSynchronize(ctx->st));
     if (abort_word != 0) return fail(ctx, ALENS_E_HIP, "BBPGD grid barrier timed out (blocks not co-resident)");
-    TRY_HIP(hipEventElapsedTime(ms_solve, ctx->ev[2], ctx->ev[3]));
     *rep = alens_solve_report{};
     rep->converged = 1;
     if (per_sys) per_sys->assign((size_t)ns, alens_solve_report{});
@@ -587,13 +590,66 @@ static alens_status run_bbpgd(alens_ctx *ctx, SolveArgsT<T> a, alens_solve_repor
     return ALENS_OK;
 }
 
+// Launches one persistent BBPGD solve on the context's stream.  Outside graph capture it waits for
+// it, times it and reads the report; while a step is being captured it only enqueues (the FP64
+// solve reports into ctx->out, the FP32 one into ctx->out32, read after the graph has run).
+template <typename T>
+static alens_status run_bbpgd(alens_ctx *ctx, SolveArgsT<T> a, alens_solve_report *rep, float *ms_solve,
+                              std::vector<alens_solve_report> *per_sys) {
+    const int32_t ns = ctx->nsys_cur;
+    TRY_HIP(ctx->part.ensure((size_t)kRedQ * std::max<int32_t>(ctx->nchunks_cur, 1)));
+    TRY_HIP(ctx->out.ensure(8 * (size_t)ns + 8));
+    TRY_HIP(ctx->out32.ensure(8 * (size_t)ns + 8));
+    TRY_K(launch_fill_u32(ctx->st, ctx->bar.p, 0u, kBarTotalWords));
+    double *out = sizeof(T) == 8 ? ctx->out.p : ctx->out32.p;
+    a.part = ctx->part.p;
+    a.bar = ctx->bar.p;
+    a.out = out;
+    a.con_off = ctx->con_off.p;
+    a.chunk_off = ctx->chunk_off.p;
+    a.chunk_sys = ctx->chunk_sys.p;
+    a.nsys = ns;
+    a.dt = ctx->cfg.dt;
+    a.tol = ctx->cfg.tol;
+    a.max_ite = ctx->cfg.max_ite;
+    a.nchunks = ctx->nchunks_cur;
+    a.nch_dev = ctx->cur_nchdev;
+    a.m_dev = ctx->cur_mdev;
+    // schedule: the context's (autotuned) choice, or the environment's (ALENS_SOLVER_MAX_BLOCKS,
+    // ALENS_BARRIER_MODE, ALENS_XCD_MAP, ALENS_SOLVER_ENTRIES) outside of tuning
+    auto env = [&](const char *name, int dflt) {
+        const char *v = ctx->tuning ? nullptr : getenv(name);
+        return v ? atoi(v) : dflt;
+    };
+    const int max_blocks = env("ALENS_SOLVER_MAX_BLOCKS", ctx->max_blocks);
+    a.bar_mode = env("ALENS_BARRIER_MODE", ctx->bar_mode);
+    if (a.bar_mode < 0 || a.bar_mode > 2) a.bar_mode = 0;
+    a.xcd_map = env("ALENS_XCD_MAP", ctx->xcd_map) != 0;
+    const int entries = env("ALENS_SOLVER_ENTRIES", ctx->entries);
+    if (!ctx->capture) TRY_HIP(hipEventRecord(ctx->ev[2], ctx->st));
+    if constexpr (sizeof(T) == 8)
+        TRY_K(launch_bbpgd64(ctx->st, a, max_blocks, entries, &ctx->grid_used));
+    else
+        TRY_K(launch_bbpgd32(ctx->st, a, max_blocks, entries, &ctx->grid_used));
+    if (ctx->capture) {
+        *ms_solve = 0.f;
+        return ALENS_OK;
+    }
+    TRY_HIP(hipEventRecord(ctx->ev[3], ctx->st));
+    alens_status s = read_reports(ctx, out, rep, per_sys);
+    if (s != ALENS_OK) return s;
+    TRY_HIP(hipEventElapsedTime(ms_solve, ctx->ev[2], ctx->ev[3]));
+    return ALENS_OK;
+}
+
 // FP64 solve on the step buffers; xg holds the start point
 static alens_status run_solver(alens_ctx *ctx, int32_t n, uint32_t m, const double *mob, bool warm,
                                alens_solve_report *rep, float *ms_solve, const int32_t *rsys, const int32_t *smono,
                                std::vector<alens_solve_report> *per_sys) {
     SolveArgsT<double> a{};
     a.n = n;
-    a.m = m;
+    a.m = ctx->gstride;  // row stride of geo: the row count, or a resident step's row capacity
+    (void)m;
     a.adjStart = ctx->adjStart.p;
     a.sellStart = ctx->sellStart.p;
     a.sellK = ctx->sellK.p;
@@ -619,6 +675,7 @@ static alens_status run_solver(alens_ctx *ctx, int32_t n, uint32_t m, const doub
 static alens_status run_solver_mixed(alens_ctx *ctx, int32_t n, uint32_t m, bool warm, alens_solve_report *rep,
                                      float *ms_solve) {
     hipStream_t st = ctx->st;
+    m = ctx->gstride;  // rows and row stride of the step buffers (a resident step: the capacity)
     const size_t M = std::max<uint32_t>(m, 1), N = (size_t)std::max(n, 1);
     TRY_HIP(ctx->geo32.ensure(9 * M));
     TRY_HIP(ctx->sellW32.ensure(6 * std::max<uint64_t>(ctx->epad, 1)));
@@ -639,7 +696,7 @@ static alens_status run_solver_mixed(alens_ctx *ctx, int32_t n, uint32_t m, bool
     TRY_K(launch_init_xg32(st, ctx->xg32.p, ctx->xg.p, m));
     SolveArgsT<float> a{};
     a.n = n;
-    a.m = m;
+    a.m = m;  // row stride
     a.adjStart = ctx->adjStart.p;
     a.sellStart = ctx->sellStart.p;
     a.sellK = ctx->sellK.p;
@@ -666,6 +723,7 @@ static alens_status run_solver_mixed(alens_ctx *ctx, int32_t n, uint32_t m, bool
     TRY_K(launch_init_xg(st, ctx->xg.p, ctx->gamma.p, m));
     s = run_solver(ctx, n, m, ctx->mob.p, true, rep, &ms64, ctx->rsys.p, ctx->smono.p, &ctx->sysrep);
     if (s != ALENS_OK) return s;
+    if (ctx->capture) return ALENS_OK;  // both reports are read after the graph has run
     rep->iters_fp32 = r32.iters;
     for (size_t q = 0; q < s32.size() && q < ctx->sysrep.size(); ++q) ctx->sysrep[q].iters_fp32 = s32[q].iters;
     *ms_solve = ms32 + ms64;
@@ -674,9 +732,9 @@ static alens_status run_solver_mixed(alens_ctx *ctx, int32_t n, uint32_t m, bool
 
 extern "C" {
 
-alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_solve_report *rep_out) {
-    if (!ctx) return ALENS_E_ARG;
-    if (!ctx->have_rods) return fail(ctx, ALENS_E_STATE, "step_constraints before set_rods");
+// One step with exact buffer sizes: the counts (cell entries, pairs, adjacency slots) are read back
+// between the phases.  Caller rows are supported here.
+static alens_status step_plain(alens_ctx *ctx, const double *f_ext, alens_solve_report *rep_out) {
     alens_solve_report rep{};
     TRY_HIP(hipSetDevice(ctx->dev));
     hipStream_t st = ctx->st;
@@ -748,7 +806,7 @@ alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_s
     TRY_HIP(ctx->vals.ensure(Ecap));
     TRY_HIP(ctx->keys2.ensure(Ecap));
     TRY_HIP(ctx->vals2.ensure(Ecap));
-    TRY_K(launch_fill_entries(st, rd, ctx->clo.p, ctx->chi.p, ctx->eoff.p, ctx->keys.p, ctx->vals.p));
+    TRY_K(launch_fill_entries(st, rd, ctx->clo.p, ctx->chi.p, ctx->eoff.p, ctx->keys.p, ctx->vals.p, E));
     bool alt = false;
     TRY_K(radix_sort_pairs(st, ctx->keys.p, ctx->vals.p, ctx->keys2.p, ctx->vals2.p, E, ctx->keybits,
                            ctx->hist, ctx->scan, alt));
@@ -788,6 +846,9 @@ alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_s
     ctx->m = M;
     ctx->mcol = Mcol;
     ctx->mext = Me;
+    ctx->gstride = M;
+    ctx->cur_mdev = nullptr;
+    ctx->cur_nchdev = nullptr;
     const size_t Mc = std::max<size_t>(M, 1);
     TRY_HIP(ctx->cI.ensure(Mc));
     TRY_HIP(ctx->cJ.ensure(Mc));
@@ -912,7 +973,9 @@ alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_s
     if (ctx->cfg.warm_start >= 1) {
         TRY_HIP(ctx->pkey.ensure(Mc));
         TRY_HIP(ctx->pgam.ensure(Mc));
-        TRY_K(launch_make_keys(st, Mcol, ctx->cI.p, ctx->cJ.p, ctx->gid.p, ctx->xg.p, ctx->pkey.p, ctx->pgam.p));
+        TRY_HIP(ctx->dword.ensure(4));
+        TRY_K(launch_make_keys(st, Mcol, ctx->cI.p, ctx->cJ.p, ctx->gid.p, ctx->xg.p, ctx->pkey.p, ctx->pgam.p, nullptr,
+                               ctx->dword.p + 2));
         ctx->mprev = Mcol;
         ctx->have_prev = true;
     }
@@ -927,11 +990,299 @@ alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_s
     rep.t_solve_ms = ms_solve;
     rep.t_step_ms = ms_step;
     ctx->have_step = true;
+    ctx->have_counts = true;
     ctx->ext_step.swap(ctx->ext_pending);
     ctx->ext_pending.clear();
     if (rep_out) *rep_out = rep;
     if (rep.status == ALENS_E_NONFINITE) return fail(ctx, ALENS_E_NONFINITE, "non-finite BBPGD residual");
     if (rep.status == ALENS_E_NOCONV) return fail(ctx, ALENS_E_NOCONV, "BBPGD did not reach tol within max_ite");
+    return ALENS_OK;
+}
+
+// ---- device-resident step (graph mode) --------------------------------------------------------
+// Capacities from the counts of the last step that sized its buffers exactly, with head room; every
+// buffer the step touches is allocated here, so that enqueuing the step allocates nothing.
+static alens_status plan_resident(alens_ctx *ctx) {
+    const int32_t n = ctx->n, ns = ctx->nsys;
+    const size_t N = (size_t)std::max(n, 1);
+    alens_ctx::Caps c;
+    // head room over the last counts (ALENS_GRAPH_HEADROOM, default 1.5; tests lower it to exercise the
+    // fallback of a replay whose counts outgrow the capacities)
+    const char *hr = getenv("ALENS_GRAPH_HEADROOM");
+    const double f = std::min(8.0, std::max(0.05, hr ? atof(hr) : 1.5));
+    const double slack = hr ? 0.0 : 1.0;
+    c.E = (uint32_t)std::min<double>(4294967295.0, (double)ctx->E * f + slack * 4096);
+    c.M = (uint32_t)std::min<double>(2147483647.0, (double)ctx->m * f + slack * 4096);
+    c.epad = (uint32_t)std::min<double>(4294967295.0, (double)ctx->epad * f + slack * 4096);
+    c.E = std::max<uint32_t>(c.E, 64);
+    c.M = std::max<uint32_t>(c.M, 64);
+    c.epad = std::max<uint32_t>(c.epad, 64);
+    c.nch = (int32_t)((c.M + kRedChunk - 1) / kRedChunk) + ns + 1;
+    for (auto *b : {&ctx->keys, &ctx->vals, &ctx->keys2, &ctx->vals2}) TRY_HIP(b->ensure(c.E));
+    TRY_K(ensure_radix_scratch(ctx->hist, ctx->scan, c.E));
+    TRY_K(ensure_scan_scratch(ctx->scan, (int64_t)n + 1));
+    const int64_t nsl = ((int64_t)n + kSlice - 1) / kSlice;
+    TRY_K(ensure_scan_scratch(ctx->scan, nsl + 1));
+    for (auto *b : {&ctx->pI, &ctx->pJ, &ctx->pImg, &ctx->cImg, &ctx->jlist}) TRY_HIP(b->ensure(c.M));
+    ctx->pcap = c.M;
+    for (auto *b : {&ctx->cI, &ctx->cJ}) TRY_HIP(b->ensure(c.M));
+    for (auto *b : {&ctx->delta0, &ctx->b, &ctx->gamma}) TRY_HIP(b->ensure(c.M));
+    // the previous step's impulse cache is read by the next step: grow it keeping its contents
+    auto grow_keep = [&](auto &buf, size_t want) -> hipError_t {
+        if (want <= buf.cap && buf.p) return hipSuccess;
+        std::remove_reference_t<decltype(buf)> nb;
+        hipError_t e = nb.ensure(want);
+        if (e != hipSuccess) return e;
+        if (buf.p) {
+            e = hipMemcpyAsync(nb.p, buf.p, sizeof(*buf.p) * buf.cap, hipMemcpyDeviceToDevice, ctx->st);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->st);
+            if (e != hipSuccess) {
+                nb.release();
+                return e;
+            }
+            buf.release();
+        }
+        buf.p = nb.p;
+        buf.cap = nb.cap;
+        nb.p = nullptr;
+        nb.cap = 0;
+        return hipSuccess;
+    };
+    TRY_HIP(grow_keep(ctx->pgam, c.M));
+    TRY_HIP(ctx->geo.ensure(9 * (size_t)c.M));
+    TRY_HIP(ctx->posI.ensure(3 * (size_t)c.M));
+    TRY_HIP(ctx->posJ.ensure(3 * (size_t)c.M));
+    TRY_HIP(ctx->xg.ensure(c.M));
+    TRY_HIP(ctx->xprev.ensure(c.M));
+    TRY_HIP(grow_keep(ctx->pkey, c.M));
+    TRY_HIP(ctx->slen.ensure((size_t)nsl + 1));
+    TRY_HIP(ctx->sellStart.ensure((size_t)nsl + 1));
+    TRY_HIP(ctx->sellK.ensure(c.epad));
+    TRY_HIP(ctx->sellW.ensure(6 * (size_t)c.epad));
+    TRY_HIP(ctx->con_off.ensure((size_t)ns + 1));
+    TRY_HIP(ctx->chunk_off.ensure((size_t)ns + 1));
+    TRY_HIP(ctx->chunk_sys.ensure((size_t)c.nch));
+    TRY_HIP(ctx->part.ensure((size_t)kRedQ * c.nch));
+    TRY_HIP(ctx->out.ensure(8 * (size_t)ns + 8));
+    TRY_HIP(ctx->out32.ensure(8 * (size_t)ns + 8));
+    TRY_HIP(ctx->dword.ensure(4));
+    TRY_HIP(ctx->nch_dev.ensure(1));
+    TRY_HIP(ctx->step_dev.ensure(1));
+    if (ctx->cfg.warm_start == 2) TRY_HIP(ctx->feat.ensure(16 * (size_t)c.M));
+    if (ctx->cfg.precision == 1) {
+        TRY_HIP(ctx->geo32.ensure(9 * (size_t)c.M));
+        TRY_HIP(ctx->sellW32.ensure(6 * (size_t)c.epad));
+        TRY_HIP(ctx->b32.ensure(c.M));
+        TRY_HIP(ctx->xg32.ensure(c.M));
+        TRY_HIP(ctx->mob32.ensure(6 * N));
+        TRY_HIP(ctx->U32.ensure(kUStride * N));
+    }
+    ctx->caps = c;
+    return ALENS_OK;
+}
+
+// The whole step (U_nc through the warm-start keys) on the stream, sized by the capacities, with
+// every count kept on the device and no host waits; buffers come from plan_resident.  warm_prev:
+// start from the previous step's impulses.
+static alens_status enqueue_resident(alens_ctx *ctx, bool fx, bool warm_prev) {
+    hipStream_t st = ctx->st;
+    const int32_t n = ctx->n;
+    const alens_ctx::Caps c = ctx->caps;
+    CfgDev cd = make_cfgdev(ctx);
+    cd.step_dev = ctx->step_dev.p;
+    const RodsDev rd = make_rods(ctx);
+    uint32_t *Ee = ctx->dword.p + 0, *Mv = ctx->dword.p + 1, *mprev = ctx->dword.p + 2;
+    TRY_K(launch_mob_unc(st, rd, ctx->length.p, ctx->radius.p, ctx->has_immov ? ctx->immov.p : nullptr,
+                         fx ? ctx->fext.p : nullptr, cd, ctx->mob.p, ctx->unc.p));
+    TRY_K(launch_aabb(st, rd, cd, ctx->clo.p, ctx->chi.p, ctx->blo.p, ctx->bhi.p, ctx->ecnt.p));
+    TRY_K(exclusive_scan_u32(st, ctx->ecnt.p, ctx->eoff.p, n, ctx->scan));
+    TRY_K(launch_guard_entries(st, ctx->eoff.p + n, c.E, Ee));
+    TRY_K(launch_fill_entries(st, rd, ctx->clo.p, ctx->chi.p, ctx->eoff.p, ctx->keys.p, ctx->vals.p, c.E));
+    bool alt = false;
+    TRY_K(radix_sort_pairs(st, ctx->keys.p, ctx->vals.p, ctx->keys2.p, ctx->vals2.p, c.E, ctx->keybits, ctx->hist,
+                           ctx->scan, alt, Ee));
+    const uint32_t *skeys = alt ? ctx->keys2.p : ctx->keys.p;
+    const uint32_t *svals = alt ? ctx->vals2.p : ctx->vals.p;
+    TRY_K(launch_fill_u32(st, ctx->cstart.p, 0u, ctx->ncell));
+    TRY_K(launch_fill_u32(st, ctx->cend.p, 0u, ctx->ncell));
+    TRY_K(launch_cell_ranges(st, skeys, c.E, ctx->cstart.p, ctx->cend.p, Ee));
+    TRY_K(launch_fill_u32(st, ctx->counter.p, 0u, 1));
+    TRY_K(launch_fill_u32(st, ctx->cntI.p, 0u, (int64_t)n + 1));
+    TRY_K(launch_fill_u32(st, ctx->cntJ.p, 0u, (int64_t)n + 1));
+    TRY_K(launch_sweep(st, rd, cd, skeys, c.E, ctx->cend.p, svals, ctx->clo.p, ctx->blo.p, ctx->bhi.p,
+                       ctx->counter.p, c.M, ctx->pI.p, ctx->pJ.p, ctx->pImg.p, ctx->cntI.p, ctx->cntJ.p, Ee));
+    TRY_K(launch_guard_pairs(st, ctx->counter.p, c.M, Mv, ctx->cntI.p, ctx->cntJ.p, n));
+    TRY_K(exclusive_scan_u32(st, ctx->cntI.p, ctx->rowStart.p, n, ctx->scan));
+    TRY_K(exclusive_scan_u32(st, ctx->cntJ.p, ctx->jStart.p, n, ctx->scan));
+    TRY_K(launch_fill_u32(st, ctx->cur.p, 0u, (int64_t)n + 1));
+    TRY_K(launch_order_I(st, n, c.M, ctx->pI.p, ctx->pJ.p, ctx->pImg.p, ctx->rowStart.p, ctx->cur.p, ctx->cI.p,
+                         ctx->cJ.p, ctx->cImg.p, Mv));
+    TRY_K(launch_fill_u32(st, ctx->cur.p, 0u, (int64_t)n + 1));
+    TRY_K(launch_order_J(st, n, c.M, ctx->cJ.p, ctx->jStart.p, ctx->cur.p, ctx->jlist.p, Mv));
+    TRY_K(launch_rows(st, rd, cd, c.M, c.M, ctx->cI.p, ctx->cJ.p, ctx->cImg.p, ctx->unc.p, ctx->delta0.p,
+                      ctx->geo.p, ctx->posI.p, ctx->posJ.p, ctx->b.p, Mv));
+    const int64_t nsl = ((int64_t)n + kSlice - 1) / kSlice;
+    TRY_K(launch_slice_sizes(st, n, ctx->rowStart.p, ctx->jStart.p, nullptr, ctx->slen.p));
+    TRY_K(exclusive_scan_u32(st, ctx->slen.p, ctx->sellStart.p, nsl, ctx->scan));
+    TRY_K(launch_guard_slots(st, ctx->sellStart.p + nsl, c.epad, Mv));
+    TRY_K(launch_adjacency(st, n, c.M, ctx->rowStart.p, ctx->jStart.p, ctx->jlist.p, nullptr, nullptr, 0u, ctx->geo.p,
+                           ctx->sellStart.p, c.epad, ctx->adjStart.p, ctx->sellK.p, ctx->sellW.p));
+    TRY_K(launch_chunk_table(st, ctx->nsys, n, ctx->sysd.p, ctx->rowStart.p, 0u, ctx->con_off.p, ctx->chunk_off.p,
+                             ctx->nch_dev.p, ctx->chunk_sys.p, c.nch, Mv));
+    // start point and solve (the solver reads the row and chunk counts from the device)
+    ctx->gstride = c.M;
+    ctx->epad = c.epad;
+    ctx->nchunks_cur = c.nch;
+    ctx->nsys_cur = ctx->nsys;
+    ctx->cur_mdev = Mv;
+    ctx->cur_nchdev = ctx->nch_dev.p;
+    const int ws = ctx->cfg.warm_start;
+    bool warm = false;
+    if (ws >= 1 && warm_prev) {
+        TRY_K(launch_warm_match(st, c.M, ctx->cI.p, ctx->cJ.p, ctx->gid.p, 0u, ctx->pkey.p, ctx->pgam.p, ctx->xg.p, Mv,
+                                mprev));
+        TRY_HIP(hipMemcpyAsync(ctx->xprev.p, ctx->xg.p, sizeof(double2) * c.M, hipMemcpyDeviceToDevice, st));
+        warm = true;
+    } else {
+        TRY_K(launch_init_xg(st, ctx->xg.p, nullptr, c.M));
+    }
+    if (ws == 2) {
+        TRY_K(launch_features(st, c.M, c.M, ctx->cI.p, ctx->cJ.p, ctx->delta0.p, ctx->geo.p, ctx->posI.p, ctx->posJ.p,
+                              ctx->dir.p, ctx->length.p, ctx->unc.p, warm ? ctx->xprev.p : nullptr, ctx->cfg.dt,
+                              ctx->feat.p, Mv));
+        TRY_K(launch_init_xg(st, ctx->xg.p, nullptr, c.M));
+        TRY_K(launch_mlp_infer_xg(st, c.M, ctx->feat.p, ctx->mlpP.p, ctx->mlpNorm.p, ctx->xg.p, Mv));
+        warm = true;
+    }
+    alens_solve_report rep{};
+    float ms = 0.f;
+    alens_status s = ctx->cfg.precision == 1
+                         ? run_solver_mixed(ctx, n, c.M, warm, &rep, &ms)
+                         : run_solver(ctx, n, c.M, ctx->mob.p, warm, &rep, &ms, ctx->rsys.p, ctx->smono.p, nullptr);
+    if (s != ALENS_OK) return s;
+    TRY_K(launch_writeback(st, n, ctx->rsys.p, ctx->sysd.p, ctx->adjStart.p, ctx->sellStart.p, ctx->sellK.p,
+                           ctx->sellW.p, c.epad, ctx->xg.p, ctx->mob.p, ctx->unc.p, ctx->F.p, ctx->U.p, ctx->gamma.p,
+                           c.M, Mv));
+    if (ws >= 1)
+        TRY_K(launch_make_keys(st, c.M, ctx->cI.p, ctx->cJ.p, ctx->gid.p, ctx->xg.p, ctx->pkey.p, ctx->pgam.p, Mv,
+                               mprev));
+    return ALENS_OK;
+}
+
+// Record the resident step into a graph (not run).  A failure leaves graph mode on plain steps.
+static alens_status capture_resident(alens_ctx *ctx, bool fx, bool warm_prev) {
+    drop_graph(ctx);
+    TRY_HIP(hipStreamSynchronize(ctx->st));
+    TRY_HIP(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+    ctx->capture = true;
+    alens_status s = enqueue_resident(ctx, fx, warm_prev);
+    ctx->capture = false;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->st, &g);
+    if (s != ALENS_OK || e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return s != ALENS_OK ? s : fail(ctx, ALENS_E_HIP, "step capture failed");
+    }
+    ctx->graph = g;
+    TRY_HIP(hipGraphInstantiate(&ctx->gexec, ctx->graph, nullptr, nullptr, 0));
+    ctx->graph_key = (fx ? 1 : 0) | (warm_prev ? 2 : 0);
+    return ALENS_OK;
+}
+
+// Replay the captured step; if a count outgrew its capacity the step is redone with exact sizes
+// (and the next step captures again).
+static alens_status step_resident(alens_ctx *ctx, const double *f_ext, alens_solve_report *rep_out) {
+    hipStream_t st = ctx->st;
+    const int32_t n = ctx->n;
+    const alens_ctx::Caps c = ctx->caps;
+    TRY_HIP(hipMemcpyAsync(ctx->step_dev.p, &ctx->step, sizeof(int64_t), hipMemcpyHostToDevice, st));
+    if (f_ext && n > 0)
+        TRY_HIP(hipMemcpyAsync(ctx->fext.p, f_ext, sizeof(double) * 6 * n, hipMemcpyHostToDevice, st));
+    TRY_HIP(hipEventRecord(ctx->ev[0], st));
+    TRY_HIP(hipGraphLaunch(ctx->gexec, st));
+    TRY_HIP(hipEventRecord(ctx->ev[4], st));
+    const int64_t nsl = ((int64_t)n + kSlice - 1) / kSlice;
+    uint32_t cnt[5] = {0, 0, 0, 0, 0};  // entries, pairs, slots, valid rows, chunks
+    TRY_HIP(hipMemcpyAsync(&cnt[0], ctx->eoff.p + n, 4, hipMemcpyDeviceToHost, st));
+    TRY_HIP(hipMemcpyAsync(&cnt[1], ctx->counter.p, 4, hipMemcpyDeviceToHost, st));
+    TRY_HIP(hipMemcpyAsync(&cnt[2], ctx->sellStart.p + nsl, 4, hipMemcpyDeviceToHost, st));
+    TRY_HIP(hipMemcpyAsync(&cnt[3], ctx->dword.p + 1, 4, hipMemcpyDeviceToHost, st));
+    TRY_HIP(hipMemcpyAsync(&cnt[4], ctx->nch_dev.p, 4, hipMemcpyDeviceToHost, st));
+    TRY_HIP(hipStreamSynchronize(st));
+    if (cnt[0] > c.E || cnt[1] > c.M || cnt[2] > c.epad) {
+        // every overflow left zero valid rows downstream, so the replay wrote no impulse cache and
+        // the plain redo starts from the same previous-step impulses
+        drop_graph(ctx);
+        return step_plain(ctx, f_ext, rep_out);
+    }
+    alens_solve_report rep{};
+    alens_status s = read_reports(ctx, ctx->out.p, &rep, &ctx->sysrep);
+    if (s != ALENS_OK) return s;
+    if (ctx->cfg.precision == 1) {
+        alens_solve_report r32{};
+        std::vector<alens_solve_report> s32;
+        s = read_reports(ctx, ctx->out32.p, &r32, &s32);
+        if (s != ALENS_OK) return s;
+        rep.iters_fp32 = r32.iters;
+        for (size_t q = 0; q < s32.size() && q < ctx->sysrep.size(); ++q) ctx->sysrep[q].iters_fp32 = s32[q].iters;
+    }
+    float ms_step = 0.f;
+    TRY_HIP(hipEventElapsedTime(&ms_step, ctx->ev[0], ctx->ev[4]));
+    ctx->E = cnt[0];
+    ctx->m = cnt[3];
+    ctx->mcol = cnt[3];
+    ctx->mext = 0;
+    ctx->have_xprev = ctx->cfg.warm_start >= 1 && (ctx->graph_key & 2);
+    if (ctx->cfg.warm_start >= 1) {
+        ctx->mprev = cnt[3];
+        ctx->have_prev = true;
+    }
+    ctx->ext_step.clear();
+    ctx->have_step = true;
+    ++ctx->graph_steps;
+    rep.n_con = cnt[3];
+    rep.n_entries = cnt[0];
+    rep.t_step_ms = ms_step;  // collection and solve run inside one graph: no split
+    if (rep_out) *rep_out = rep;
+    if (rep.status == ALENS_E_NONFINITE) return fail(ctx, ALENS_E_NONFINITE, "non-finite BBPGD residual");
+    if (rep.status == ALENS_E_NOCONV) return fail(ctx, ALENS_E_NOCONV, "BBPGD did not reach tol within max_ite");
+    return ALENS_OK;
+}
+
+alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_solve_report *rep_out) {
+    if (!ctx) return ALENS_E_ARG;
+    if (!ctx->have_rods) return fail(ctx, ALENS_E_STATE, "step_constraints before set_rods");
+    TRY_HIP(hipSetDevice(ctx->dev));
+    if (!ctx->graph_mode || !ctx->ext_pending.empty()) return step_plain(ctx, f_ext, rep_out);
+    if (f_ext && !finite_all(f_ext, 6 * (size_t)ctx->n)) return fail(ctx, ALENS_E_NONFINITE, "non-finite f_ext");
+    const bool fx = f_ext != nullptr;
+    const bool warm_prev = ctx->cfg.warm_start >= 1 && ctx->have_prev;
+    if (!ctx->gexec || ctx->graph_key != ((fx ? 1 : 0) | (warm_prev ? 2 : 0))) {
+        // the first step sizes the buffers exactly; later ones plan capacities from the last counts
+        // (the previous step's outputs are about to be replaced) and capture the resident step once
+        if (!ctx->have_counts) return step_plain(ctx, f_ext, rep_out);
+        alens_status p = plan_resident(ctx);
+        if (p == ALENS_OK) p = capture_resident(ctx, fx, warm_prev);
+        if (p != ALENS_OK) {
+            drop_graph(ctx);
+            return step_plain(ctx, f_ext, rep_out);
+        }
+    }
+    return step_resident(ctx, f_ext, rep_out);
+}
+
+alens_status alens_set_graph(alens_ctx *ctx, int32_t enable) {
+    if (!ctx) return ALENS_E_ARG;
+    ctx->graph_mode = enable ? 1 : 0;
+    if (!enable) drop_graph(ctx);
+    return ALENS_OK;
+}
+
+alens_status alens_get_graph_info(const alens_ctx *ctx, int32_t *captured, int64_t *replays) {
+    if (!ctx) return ALENS_E_ARG;
+    if (captured) *captured = ctx->gexec ? 1 : 0;
+    if (replays) *replays = ctx->graph_steps;
     return ALENS_OK;
 }
 
@@ -973,18 +1324,19 @@ alens_status alens_get_constraint_arrays(alens_ctx *ctx, int32_t *I, int32_t *J,
         im.resize(m);
         TRY_HIP(d2h(ctx, im.data(), ctx->cImg.p, m));
     }
+    const size_t gs = ctx->gstride;
     if ((nrm || r_i || r_j) && m) {
-        geo.resize(9 * m);
-        TRY_HIP(d2h(ctx, geo.data(), ctx->geo.p, 9 * m));
+        geo.resize(9 * gs);
+        TRY_HIP(d2h(ctx, geo.data(), ctx->geo.p, 9 * gs));
     }
     TRY_HIP(hipStreamSynchronize(ctx->st));
     for (size_t k = 0; k < m; ++k) {
         if (img)
             for (int a = 0; a < 3; ++a) img[3 * k + a] = img_get(im[k], a);
         for (int a = 0; a < 3; ++a) {
-            if (nrm) nrm[3 * k + a] = geo[a * m + k];
-            if (r_i) r_i[3 * k + a] = geo[(3 + a) * m + k];
-            if (r_j) r_j[3 * k + a] = geo[(6 + a) * m + k];
+            if (nrm) nrm[3 * k + a] = geo[a * gs + k];
+            if (r_i) r_i[3 * k + a] = geo[(3 + a) * gs + k];
+            if (r_j) r_j[3 * k + a] = geo[(6 + a) * gs + k];
         }
     }
     return ALENS_OK;
@@ -1101,6 +1453,7 @@ alens_status alens_clear_constraints(alens_ctx *ctx) {
 }
 
 alens_status alens_autotune(alens_ctx *ctx, alens_tune_report *rep_out) {
+    if (ctx) drop_graph(ctx);  // the schedule is part of a captured step
     if (!ctx) return ALENS_E_ARG;
     if (!ctx->have_step) return fail(ctx, ALENS_E_STATE, "autotune needs a step's constraints (run a step first)");
     if (ctx->m == 0) return fail(ctx, ALENS_E_STATE, "autotune: the last step had no constraints");
@@ -1195,8 +1548,12 @@ alens_status alens_bcqp_solve(alens_ctx *ctx, int32_t n, int64_t m, const int32_
             return fail(ctx, ALENS_E_ARG, "bcqp_solve: rod index out of range");
     TRY_HIP(hipSetDevice(ctx->dev));
     ctx->have_step = false;  // the step buffers are about to hold this solve's rows
+    drop_graph(ctx);
     ctx->mext = 0;
     ctx->mcol = 0;
+    ctx->gstride = (uint32_t)m;
+    ctx->cur_mdev = nullptr;
+    ctx->cur_nchdev = nullptr;
     // rod-major adjacency in ascending constraint index, built on the host for arbitrary rows, in the
     // sliced layout the solver reads (see launch_adjacency); geometry component-major
     const size_t M = (size_t)m, N = (size_t)std::max(n, 1);
@@ -1292,6 +1649,7 @@ alens_status alens_bcqp_solve(alens_ctx *ctx, int32_t n, int64_t m, const int32_
 }
 
 alens_status alens_mlp_set(alens_ctx *ctx, const float *params, const float *norm) {
+    if (ctx) drop_graph(ctx);
     if (!ctx || !params || !norm) return ALENS_E_ARG;
     for (int i = 0; i < kMlpNorm; ++i)
         if (!std::isfinite(norm[i])) return fail(ctx, ALENS_E_NONFINITE, "mlp_set: non-finite normalisation");
@@ -1326,10 +1684,10 @@ alens_status alens_get_features(alens_ctx *ctx, float *X) {
     TRY_HIP(ctx->feat.ensure(16 * (size_t)M));
     // collision rows (with the previous gamma when the step had one), then caller rows (previous gamma 0)
     const uint32_t mc = ctx->mcol, me = M - mc;
-    TRY_K(launch_features(ctx->st, mc, M, ctx->cI.p, ctx->cJ.p, ctx->delta0.p, ctx->geo.p, ctx->posI.p, ctx->posJ.p,
+    TRY_K(launch_features(ctx->st, mc, ctx->gstride, ctx->cI.p, ctx->cJ.p, ctx->delta0.p, ctx->geo.p, ctx->posI.p, ctx->posJ.p,
                           ctx->dir.p, ctx->length.p, ctx->unc.p, ctx->have_xprev ? ctx->xprev.p : nullptr,
                           ctx->cfg.dt, ctx->feat.p));
-    TRY_K(launch_features(ctx->st, me, M, ctx->cI.p + mc, ctx->cJ.p + mc, ctx->delta0.p + mc, ctx->geo.p + mc,
+    TRY_K(launch_features(ctx->st, me, ctx->gstride, ctx->cI.p + mc, ctx->cJ.p + mc, ctx->delta0.p + mc, ctx->geo.p + mc,
                           ctx->posI.p + 3 * (size_t)mc, ctx->posJ.p + 3 * (size_t)mc, ctx->dir.p, ctx->length.p,
                           ctx->unc.p, nullptr, ctx->cfg.dt, ctx->feat.p + 16 * (size_t)mc));
     TRY_HIP(hipMemcpyAsync(X, ctx->feat.p, sizeof(float) * 16 * M, hipMemcpyDeviceToHost, ctx->st));
@@ -1405,6 +1763,7 @@ alens_status alens_mlp_gradient(alens_ctx *ctx, int64_t m, const float *X, const
 
 alens_status alens_mlp_train(alens_ctx *ctx, int64_t m, const float *X, const float *gamma, int32_t epochs,
                              int32_t batch, float lr, uint64_t seed, float *loss_per_epoch) {
+    if (ctx) drop_graph(ctx);
     if (!ctx || m <= 0 || m > (1 << 28) || !X || !gamma || epochs < 0 || batch < 16 || batch % 16 || !(lr > 0.f))
         return ALENS_E_ARG;
     if (batch > m) batch = (int32_t)(m / 16 * 16);

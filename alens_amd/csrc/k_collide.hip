@@ -50,9 +50,10 @@ __global__ void k_mob_unc(int32_t n, const int32_t *__restrict__ rsys, const Sys
         const double pre = sqrt(2.0 * c.kT / c.dt);
         double z[6];
         const uint32_t k0 = (uint32_t)c.seed, k1 = (uint32_t)(c.seed >> 32);
+        const int64_t step = c.step_dev ? *c.step_dev : c.step;
 #pragma unroll
         for (uint32_t q = 0; q < 3; ++q) {
-            uint32_t ctr[4] = {(uint32_t)gid[i], (uint32_t)c.step, (uint32_t)((uint64_t)c.step >> 32), q};
+            uint32_t ctr[4] = {(uint32_t)gid[i], (uint32_t)step, (uint32_t)((uint64_t)step >> 32), q};
             philox4x32_10(ctr, k0, k1);
             const double u1 =
                 ((double)((((uint64_t)ctr[0]) << 21) ^ (uint64_t)(ctr[1] >> 11)) + 1.0) * 0x1p-53;
@@ -238,6 +239,19 @@ static HipErr scan_level(hipStream_t st, const uint32_t *in, uint32_t *out, int6
     return hip_ok();
 }
 
+// allocations a scan / radix sort of n items makes, done ahead of time (a step captured into a graph
+// must not allocate)
+HipErr ensure_scan_scratch(ScanScratch &s, int64_t n) {
+    for (int lvl = 0; lvl < 3; ++lvl) {
+        const int64_t nt = (n + kScanTile - 1) / kScanTile;
+        if (nt <= 1) break;
+        ALENS_HIP_TRY(s.lvl[lvl].ensure((size_t)(2 * nt + 2)));
+        n = nt;
+    }
+    return hip_ok();
+}
+HipErr ensure_radix_scratch(DBuf<uint32_t> &hist, ScanScratch &s, int64_t n);
+
 HipErr exclusive_scan_u32(hipStream_t st, const uint32_t *in, uint32_t *out, int64_t n,
                           ScanScratch &s) {
     if (n == 0) {
@@ -259,9 +273,9 @@ __device__ inline int pmod_i(int a, int n) {
 
 __global__ void k_fill_entries(RodsDev r, const int32_t *__restrict__ clo, const int32_t *__restrict__ chi,
                                const uint32_t *__restrict__ eoff, uint32_t *__restrict__ keys,
-                               uint32_t *__restrict__ vals) {
+                               uint32_t *__restrict__ vals, uint32_t ecap) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= r.n) return;
+    if (i >= r.n || eoff[i + 1] > ecap) return;  // past the capacity: the host sees eoff[n] > ecap
     const SysDev &g = r.sys[r.rsys[i]];
     uint32_t w = eoff[i];
     const int a0 = clo[3 * i], a1 = clo[3 * i + 1], a2 = clo[3 * i + 2];
@@ -281,9 +295,9 @@ __global__ void k_fill_entries(RodsDev r, const int32_t *__restrict__ clo, const
 }
 
 HipErr launch_fill_entries(hipStream_t st, const RodsDev &r, const int32_t *clo, const int32_t *chi,
-                           const uint32_t *eoff, uint32_t *keys, uint32_t *vals) {
+                           const uint32_t *eoff, uint32_t *keys, uint32_t *vals, uint32_t ecap) {
     if (r.n == 0) return hip_ok();
-    k_fill_entries<<<nblk(r.n, 256), 256, 0, st>>>(r, clo, chi, eoff, keys, vals);
+    k_fill_entries<<<nblk(r.n, 256), 256, 0, st>>>(r, clo, chi, eoff, keys, vals, ecap);
     LAUNCH_CHECK();
     return hip_ok();
 }
@@ -307,8 +321,9 @@ HipErr launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, int64_t n) {
 // ------------------------------------------------------------------------------------------------
 constexpr int kRsT = 256, kRsR = 8, kRsTile = kRsT * kRsR;
 
-__global__ void k_radix_hist(const uint32_t *__restrict__ keys, int64_t n, int shift, int ntiles,
-                             uint32_t *__restrict__ hist) {
+__global__ void k_radix_hist(const uint32_t *__restrict__ keys, int64_t ncap, int shift, int ntiles,
+                             uint32_t *__restrict__ hist, const uint32_t *__restrict__ nd) {
+    const int64_t n = dcount((uint32_t)ncap, nd);
     __shared__ uint32_t lh[256];
     lh[threadIdx.x] = 0;
     __syncthreads();
@@ -323,8 +338,10 @@ __global__ void k_radix_hist(const uint32_t *__restrict__ keys, int64_t n, int s
 }
 
 __global__ void k_radix_scatter(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
-                                int64_t n, int shift, int ntiles, const uint32_t *__restrict__ off,
-                                uint32_t *__restrict__ okeys, uint32_t *__restrict__ ovals) {
+                                int64_t ncap, int shift, int ntiles, const uint32_t *__restrict__ off,
+                                uint32_t *__restrict__ okeys, uint32_t *__restrict__ ovals,
+                                const uint32_t *__restrict__ nd) {
+    const int64_t n = dcount((uint32_t)ncap, nd);
     __shared__ uint32_t run[256];
     __shared__ uint32_t wc[kRsT / 64][256];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -365,9 +382,17 @@ __global__ void k_radix_scatter(const uint32_t *__restrict__ keys, const uint32_
     }
 }
 
+HipErr ensure_radix_scratch(DBuf<uint32_t> &hist, ScanScratch &s, int64_t n) {
+    if (n <= 1) return hip_ok();
+    const int64_t ntiles = (n + kRsTile - 1) / kRsTile;
+    const int64_t nh = 256 * ntiles;
+    ALENS_HIP_TRY(hist.ensure((size_t)(2 * nh + 1)));
+    return ensure_scan_scratch(s, nh);
+}
+
 HipErr radix_sort_pairs(hipStream_t st, uint32_t *keys, uint32_t *vals, uint32_t *keys_alt,
                         uint32_t *vals_alt, int64_t n, int keybits, DBuf<uint32_t> &hist,
-                        ScanScratch &s, bool &result_in_alt) {
+                        ScanScratch &s, bool &result_in_alt, const uint32_t *nd) {
     result_in_alt = false;
     if (n <= 1 || keybits <= 0) return hip_ok();
     const int ntiles = (int)((n + kRsTile - 1) / kRsTile);
@@ -376,11 +401,11 @@ HipErr radix_sort_pairs(hipStream_t st, uint32_t *keys, uint32_t *vals, uint32_t
     uint32_t *h = hist.p, *ho = hist.p + nh;
     uint32_t *ka = keys, *va = vals, *kb = keys_alt, *vb = vals_alt;
     for (int shift = 0; shift < keybits; shift += 8) {
-        k_radix_hist<<<ntiles, kRsT, 0, st>>>(ka, n, shift, ntiles, h);
+        k_radix_hist<<<ntiles, kRsT, 0, st>>>(ka, n, shift, ntiles, h, nd);
         LAUNCH_CHECK();
         HipErr e = exclusive_scan_u32(st, h, ho, nh, s);
         if (!e.ok()) return e;
-        k_radix_scatter<<<ntiles, kRsT, 0, st>>>(ka, va, n, shift, ntiles, ho, kb, vb);
+        k_radix_scatter<<<ntiles, kRsT, 0, st>>>(ka, va, n, shift, ntiles, ho, kb, vb, nd);
         LAUNCH_CHECK();
         uint32_t *t = ka;
         ka = kb;
@@ -396,8 +421,10 @@ HipErr radix_sort_pairs(hipStream_t st, uint32_t *keys, uint32_t *vals, uint32_t
 // ------------------------------------------------------------------------------------------------
 // cell ranges in the sorted entry list (cells without entries keep start = end = 0)
 // ------------------------------------------------------------------------------------------------
-__global__ void k_cell_ranges(const uint32_t *__restrict__ keys, int64_t E,
-                              uint32_t *__restrict__ cstart, uint32_t *__restrict__ cend) {
+__global__ void k_cell_ranges(const uint32_t *__restrict__ keys, int64_t Ecap,
+                              uint32_t *__restrict__ cstart, uint32_t *__restrict__ cend,
+                              const uint32_t *__restrict__ Ed) {
+    const int64_t E = dcount((uint32_t)Ecap, Ed);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= E) return;
     const uint32_t k = keys[i];
@@ -406,9 +433,9 @@ __global__ void k_cell_ranges(const uint32_t *__restrict__ keys, int64_t E,
 }
 
 HipErr launch_cell_ranges(hipStream_t st, const uint32_t *keys, int64_t E, uint32_t *cstart,
-                          uint32_t *cend) {
+                          uint32_t *cend, const uint32_t *Ed) {
     if (E == 0) return hip_ok();
-    k_cell_ranges<<<nblk(E, 256), 256, 0, st>>>(keys, E, cstart, cend);
+    k_cell_ranges<<<nblk(E, 256), 256, 0, st>>>(keys, E, cstart, cend, Ed);
     LAUNCH_CHECK();
     return hip_ok();
 }
@@ -435,7 +462,9 @@ __global__ __launch_bounds__(kSweepT) void k_sweep_entries(RodsDev r, CfgDev c, 
                                                        uint32_t *__restrict__ pI, uint32_t *__restrict__ pJ,
                                                        uint32_t *__restrict__ pImg,
                                                        uint32_t *__restrict__ cntI,
-                                                       uint32_t *__restrict__ cntJ) {
+                                                       uint32_t *__restrict__ cntJ,
+                                                       const uint32_t *__restrict__ Ed) {
+    E = dcount((uint32_t)E, Ed);
     __shared__ uint32_t s_n, s_base;
     __shared__ uint32_t s_pi[kStage], s_pj[kStage], s_pm[kStage];
     const int lane = threadIdx.x & 63;
@@ -555,10 +584,10 @@ __global__ __launch_bounds__(kSweepT) void k_sweep_entries(RodsDev r, CfgDev c, 
 HipErr launch_sweep(hipStream_t st, const RodsDev &r, const CfgDev &c, const uint32_t *keys, int64_t E,
                     const uint32_t *cend, const uint32_t *vals, const int32_t *clo, const double *blo,
                     const double *bhi, uint32_t *pair_count, uint32_t cap, uint32_t *pI, uint32_t *pJ,
-                    uint32_t *pImg, uint32_t *cntI, uint32_t *cntJ) {
+                    uint32_t *pImg, uint32_t *cntI, uint32_t *cntJ, const uint32_t *Ed) {
     if (E == 0) return hip_ok();
     k_sweep_entries<<<nblk(E, kSweepT), kSweepT, 0, st>>>(r, c, keys, E, cend, vals, clo, blo, bhi, pair_count, cap, pI,
-                                                  pJ, pImg, cntI, cntJ);
+                                                  pJ, pImg, cntI, cntJ, Ed);
     LAUNCH_CHECK();
     return hip_ok();
 }
@@ -571,9 +600,9 @@ HipErr launch_sweep(hipStream_t st, const RodsDev &r, const CfgDev &c, const uin
 __global__ void k_scatter_I(uint32_t m, const uint32_t *__restrict__ pI, const uint32_t *__restrict__ pJ,
                             const uint32_t *__restrict__ pImg, const uint32_t *__restrict__ rowStart,
                             uint32_t *__restrict__ cur, int32_t *__restrict__ cI,
-                            int32_t *__restrict__ cJ, uint32_t *__restrict__ cImg) {
+                            int32_t *__restrict__ cJ, uint32_t *__restrict__ cImg, const uint32_t *__restrict__ md) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= m) return;
+    if (p >= dcount(m, md)) return;
     const uint32_t i = pI[p];
     const uint32_t pos = rowStart[i] + atomicAdd(&cur[i], 1u);
     cI[pos] = (int32_t)i;
@@ -602,9 +631,9 @@ __global__ void k_sort_I(int32_t n, const uint32_t *__restrict__ rowStart, int32
 
 HipErr launch_order_I(hipStream_t st, int32_t n, uint32_t m, const uint32_t *pI,
                       const uint32_t *pJ, const uint32_t *pImg, const uint32_t *rowStart,
-                      uint32_t *cur, int32_t *cI, int32_t *cJ, uint32_t *cImg) {
+                      uint32_t *cur, int32_t *cI, int32_t *cJ, uint32_t *cImg, const uint32_t *md) {
     if (m == 0) return hip_ok();
-    k_scatter_I<<<nblk(m, 256), 256, 0, st>>>(m, pI, pJ, pImg, rowStart, cur, cI, cJ, cImg);
+    k_scatter_I<<<nblk(m, 256), 256, 0, st>>>(m, pI, pJ, pImg, rowStart, cur, cI, cJ, cImg, md);
     LAUNCH_CHECK();
     k_sort_I<<<nblk(n, 256), 256, 0, st>>>(n, rowStart, cJ, cImg);
     LAUNCH_CHECK();
@@ -612,9 +641,9 @@ HipErr launch_order_I(hipStream_t st, int32_t n, uint32_t m, const uint32_t *pI,
 }
 
 __global__ void k_scatter_J(uint32_t m, const int32_t *__restrict__ cJ, const uint32_t *__restrict__ jStart,
-                            uint32_t *__restrict__ cur, uint32_t *__restrict__ jlist) {
+                            uint32_t *__restrict__ cur, uint32_t *__restrict__ jlist, const uint32_t *__restrict__ md) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= m) return;
+    if (k >= dcount(m, md)) return;
     const uint32_t j = (uint32_t)cJ[k];
     jlist[jStart[j] + atomicAdd(&cur[j], 1u)] = k;
 }
@@ -635,9 +664,9 @@ __global__ void k_sort_J(int32_t n, const uint32_t *__restrict__ jStart, uint32_
 }
 
 HipErr launch_order_J(hipStream_t st, int32_t n, uint32_t m, const int32_t *cJ,
-                      const uint32_t *jStart, uint32_t *cur, uint32_t *jlist) {
+                      const uint32_t *jStart, uint32_t *cur, uint32_t *jlist, const uint32_t *md) {
     if (m == 0) return hip_ok();
-    k_scatter_J<<<nblk(m, 256), 256, 0, st>>>(m, cJ, jStart, cur, jlist);
+    k_scatter_J<<<nblk(m, 256), 256, 0, st>>>(m, cJ, jStart, cur, jlist, md);
     LAUNCH_CHECK();
     k_sort_J<<<nblk(n, 256), 256, 0, st>>>(n, jStart, jlist);
     LAUNCH_CHECK();
@@ -652,9 +681,9 @@ __global__ void k_rows(RodsDev r, CfgDev c, uint32_t m, uint32_t gs, const int32
                        const int32_t *__restrict__ cJ, const uint32_t *__restrict__ cImg,
                        const double *__restrict__ unc, double *__restrict__ delta0,
                        double *__restrict__ geo, double *__restrict__ posI,
-                       double *__restrict__ posJ, double *__restrict__ b) {
+                       double *__restrict__ posJ, double *__restrict__ b, const uint32_t *__restrict__ md) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= m) return;
+    if (k >= dcount(m, md)) return;
     const int32_t i = cI[k], j = cJ[k];
     const uint32_t im = cImg[k];
     const SysDev &S = r.sys[r.rsys[i]];
@@ -694,9 +723,9 @@ __global__ void k_rows(RodsDev r, CfgDev c, uint32_t m, uint32_t gs, const int32
 
 HipErr launch_rows(hipStream_t st, const RodsDev &r, const CfgDev &c, uint32_t m, uint32_t gs, const int32_t *cI,
                    const int32_t *cJ, const uint32_t *cImg, const double *unc, double *delta0,
-                   double *geo, double *posI, double *posJ, double *b) {
+                   double *geo, double *posI, double *posJ, double *b, const uint32_t *md) {
     if (m == 0) return hip_ok();
-    k_rows<<<nblk(m, 256), 256, 0, st>>>(r, c, m, gs, cI, cJ, cImg, unc, delta0, geo, posI, posJ, b);
+    k_rows<<<nblk(m, 256), 256, 0, st>>>(r, c, m, gs, cI, cJ, cImg, unc, delta0, geo, posI, posJ, b, md);
     LAUNCH_CHECK();
     return hip_ok();
 }
@@ -760,6 +789,96 @@ __global__ void k_slice_sizes(int32_t n, const uint32_t *__restrict__ rowStart, 
     if ((threadIdx.x & 63) == 0 && i < n) slice_len[i >> 6] = deg * (uint32_t)kSlice;
 }
 
+// Capacity guards between the phases of a step whose sizes are not read back: a count past its
+// capacity becomes 0 downstream (and the per-rod pair counts are cleared), so that no later kernel
+// indexes past a buffer; the host compares the raw counts with the capacities after the step.
+__global__ void k_guard_entries(const uint32_t *__restrict__ eoff_n, uint32_t ecap, uint32_t *__restrict__ ee) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *ee = *eoff_n <= ecap ? *eoff_n : 0u;
+}
+__global__ void k_guard_pairs(const uint32_t *__restrict__ count, uint32_t cap, uint32_t *__restrict__ mv,
+                              uint32_t *__restrict__ cntI, uint32_t *__restrict__ cntJ, int32_t n) {
+    const uint32_t c = *count;
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *mv = c <= cap ? c : 0u;
+    if (c > cap && i <= n) {
+        cntI[i] = 0u;
+        cntJ[i] = 0u;
+    }
+}
+__global__ void k_guard_slots(const uint32_t *__restrict__ slots, uint32_t cap, uint32_t *__restrict__ mv) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && *slots > cap) *mv = 0u;
+}
+HipErr launch_guard_slots(hipStream_t st, const uint32_t *slots, uint32_t cap, uint32_t *mv) {
+    k_guard_slots<<<1, 64, 0, st>>>(slots, cap, mv);
+    LAUNCH_CHECK();
+    return hip_ok();
+}
+HipErr launch_guard_entries(hipStream_t st, const uint32_t *eoff_n, uint32_t ecap, uint32_t *ee) {
+    k_guard_entries<<<1, 64, 0, st>>>(eoff_n, ecap, ee);
+    LAUNCH_CHECK();
+    return hip_ok();
+}
+HipErr launch_guard_pairs(hipStream_t st, const uint32_t *count, uint32_t cap, uint32_t *mv, uint32_t *cntI,
+                          uint32_t *cntJ, int32_t n) {
+    k_guard_pairs<<<nblk((int64_t)n + 1, 256), 256, 0, st>>>(count, cap, mv, cntI, cntJ, n);
+    LAUNCH_CHECK();
+    return hip_ok();
+}
+
+// Chunk table: system s owns rows [con_off[s], con_off[s+1]) (rows are grouped by rod I; the caller
+// rows close the last system) and chunks [chunk_off[s], chunk_off[s+1]) of 512 rows.
+__global__ void k_chunk_table(int32_t nsys, int32_t n, const SysDev *__restrict__ sys,
+                              const uint32_t *__restrict__ rowStart, uint32_t me, uint32_t *__restrict__ con_off,
+                              uint32_t *__restrict__ chunk_off, int32_t *__restrict__ nchunks,
+                              const uint32_t *__restrict__ mv) {
+    if (threadIdx.x != 0) return;
+    if (mv && *mv == 0u) {  // no rows, or a step past its capacities: nothing to solve
+        for (int32_t s = 0; s <= nsys; ++s) con_off[s] = chunk_off[s] = 0u;
+        *nchunks = 0;
+        return;
+    }
+    uint32_t acc = 0;
+    for (int32_t s = 0; s < nsys; ++s) {
+        const uint32_t a = rowStart[sys[s].rod_off];
+        const uint32_t b = s + 1 < nsys ? rowStart[sys[s + 1].rod_off] : rowStart[n] + me;
+        con_off[s] = a;
+        chunk_off[s] = acc;
+        acc += (b - a + (uint32_t)kRedChunk - 1u) / (uint32_t)kRedChunk;
+    }
+    con_off[nsys] = rowStart[n] + me;
+    chunk_off[nsys] = acc;
+    *nchunks = (int32_t)acc;
+}
+__global__ void k_chunk_sys(int32_t nsys, const uint32_t *__restrict__ chunk_off, const int32_t *__restrict__ nchunks,
+                            int32_t *__restrict__ chunk_sys, int32_t cap) {
+    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cap) return;
+    if (c >= *nchunks) {
+        chunk_sys[c] = -1;
+        return;
+    }
+    int32_t lo = 0, hi = nsys;  // last s with chunk_off[s] <= c
+    while (hi - lo > 1) {
+        const int32_t mid = (lo + hi) / 2;
+        if (chunk_off[mid] <= (uint32_t)c)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    chunk_sys[c] = lo;
+}
+HipErr launch_chunk_table(hipStream_t st, int32_t nsys, int32_t n, const SysDev *sys, const uint32_t *rowStart,
+                          uint32_t me, uint32_t *con_off, uint32_t *chunk_off, int32_t *nchunks, int32_t *chunk_sys,
+                          int32_t chunk_cap, const uint32_t *mv) {
+    k_chunk_table<<<1, 64, 0, st>>>(nsys, n, sys, rowStart, me, con_off, chunk_off, nchunks, mv);
+    LAUNCH_CHECK();
+    if (chunk_cap > 0) {
+        k_chunk_sys<<<nblk(chunk_cap, 256), 256, 0, st>>>(nsys, chunk_off, nchunks, chunk_sys, chunk_cap);
+        LAUNCH_CHECK();
+    }
+    return hip_ok();
+}
+
 HipErr launch_slice_sizes(hipStream_t st, int32_t n, const uint32_t *rowStart, const uint32_t *jStart,
                           const uint32_t *extStart, uint32_t *slice_len) {
     static_assert(kSlice == 64, "one wave per slice");
@@ -777,8 +896,10 @@ __global__ void k_adjacency(int32_t n, uint32_t m, const uint32_t *__restrict__ 
                             double *__restrict__ sellW) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i > n) return;
-    adjStart[i] = rowStart[i] + jStart[i] + (extStart ? extStart[i] : 0u);
-    if (i == n) return;
+    // past the slot capacity every rod gets degree 0 (the host sees sellStart[last] > epad)
+    const bool fits = sellStart[(n + kSlice - 1) / kSlice] <= epad;
+    adjStart[i] = fits ? rowStart[i] + jStart[i] + (extStart ? extStart[i] : 0u) : 0u;
+    if (i == n || !fits) return;
     const uint32_t j0 = jStart[i], j1 = jStart[i + 1];
     uint64_t idx = sellStart[i >> 6] + (uint32_t)(i & (kSlice - 1));
     for (uint32_t t = j0; t < j1; ++t, idx += kSlice) {

@@ -54,9 +54,9 @@ __global__ void k_features(uint32_t m, uint32_t gs, const int32_t *__restrict__ 
                            const double *__restrict__ posI, const double *__restrict__ posJ,
                            const double *__restrict__ dir, const double *__restrict__ length,
                            const double *__restrict__ unc, const double2 *__restrict__ xprev, double dt,
-                           float *__restrict__ X) {
+                           float *__restrict__ X, const uint32_t *__restrict__ md) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= m) return;
+    if (k >= dcount(m, md)) return;
     const int32_t i = cI[k], j = cJ[k];
     const double n[3] = {geo[k], geo[(size_t)gs + k], geo[2 * (size_t)gs + k]};  // component-major, stride gs
     const double *pi = posI + 3 * (size_t)k, *pj = posJ + 3 * (size_t)k;
@@ -91,9 +91,11 @@ __global__ void k_features(uint32_t m, uint32_t gs, const int32_t *__restrict__ 
 
 HipErr launch_features(hipStream_t st, uint32_t m, uint32_t gs, const int32_t *cI, const int32_t *cJ, const double *delta0,
                        const double *geo, const double *posI, const double *posJ, const double *dir,
-                       const double *length, const double *unc, const double2 *xprev, double dt, float *X) {
+                       const double *length, const double *unc, const double2 *xprev, double dt, float *X,
+                       const uint32_t *md) {
     if (m == 0) return hip_ok();
-    k_features<<<nblk(m, 256), 256, 0, st>>>(m, gs, cI, cJ, delta0, geo, posI, posJ, dir, length, unc, xprev, dt, X);
+    k_features<<<nblk(m, 256), 256, 0, st>>>(m, gs, cI, cJ, delta0, geo, posI, posJ, dir, length, unc, xprev, dt, X,
+                                             md);
     LAUNCH_CHECK();
     return hip_ok();
 }
@@ -108,7 +110,9 @@ __device__ inline float norm_feat(const float *X, size_t c, int f, const float *
 template <typename OutT>
 __global__ __launch_bounds__(256) void k_mlp_infer(uint32_t m, const float *__restrict__ X,
                                                    const float *__restrict__ P, const float *__restrict__ norm,
-                                                   OutT *__restrict__ out, int out_stride) {
+                                                   OutT *__restrict__ out, int out_stride,
+                                                   const uint32_t *__restrict__ md) {
+    m = dcount(m, md);
     const int lane = threadIdx.x & 63;
     const int r16 = lane & 15, q = lane >> 4;
     const float *mean = norm, *istd = norm + kMlpIn;
@@ -185,19 +189,19 @@ HipErr launch_mlp_infer_f32(hipStream_t st, uint32_t m, const float *X, const fl
     const int64_t tiles = ((int64_t)m + 15) / 16;
     int64_t blocks = (tiles + 3) / 4;
     if (blocks > 4096) blocks = 4096;
-    k_mlp_infer<float><<<(unsigned)blocks, 256, 0, st>>>(m, X, P, norm, out, 1);
+    k_mlp_infer<float><<<(unsigned)blocks, 256, 0, st>>>(m, X, P, norm, out, 1, nullptr);
     LAUNCH_CHECK();
     return hip_ok();
 }
 
 // writes gamma0 into the x component of the solver state (double2 stride 2)
 HipErr launch_mlp_infer_xg(hipStream_t st, uint32_t m, const float *X, const float *P, const float *norm,
-                           double2 *xg) {
+                           double2 *xg, const uint32_t *md) {
     if (m == 0) return hip_ok();
     const int64_t tiles = ((int64_t)m + 15) / 16;
     int64_t blocks = (tiles + 3) / 4;
     if (blocks > 4096) blocks = 4096;
-    k_mlp_infer<double><<<(unsigned)blocks, 256, 0, st>>>(m, X, P, norm, reinterpret_cast<double *>(xg), 2);
+    k_mlp_infer<double><<<(unsigned)blocks, 256, 0, st>>>(m, X, P, norm, reinterpret_cast<double *>(xg), 2, md);
     LAUNCH_CHECK();
     return hip_ok();
 }

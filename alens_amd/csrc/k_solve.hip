@@ -383,9 +383,9 @@ struct ChunkMeta {
     int sy;
 };
 template <typename T>
-__device__ inline ChunkMeta chunk_meta(const SolveArgsT<T> &a, int c) {
+__device__ inline ChunkMeta chunk_meta(const SolveArgsT<T> &a, int c, int nch) {
     ChunkMeta m{0u, 0u, -1};
-    if (c < a.nchunks) {
+    if (c < nch) {
         m.sy = a.chunk_sys[c];
         m.k0 = a.con_off[m.sy] + (uint32_t)(c - (int)a.chunk_off[m.sy]) * kRedChunk;
         const uint32_t k1 = m.k0 + kRedChunk, kend = a.con_off[m.sy + 1];
@@ -398,13 +398,13 @@ __device__ inline ChunkMeta chunk_meta(const SolveArgsT<T> &a, int c) {
 // part[q * nchunks + c]
 template <int MODE, typename T>
 __device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, const int *s_state, double *sh,
-                               const ChunkMeta (*s_cm)[kGrpB]) {
+                               const ChunkMeta (*s_cm)[kGrpB], int nch) {
     constexpr int NE = kVPT * kRedPasses;  // elements per lane
     const int g = threadIdx.x / kGL, l = threadIdx.x % kGL;
     const int ngroups = gridDim.x * kGrpB;
     const T dt = (T)a.dt;
     int r = 0;
-    for (int base = work_block(a.xcd_map) * kGrpB; base < a.nchunks; base += ngroups, ++r) {
+    for (int base = work_block(a.xcd_map) * kGrpB; base < nch; base += ngroups, ++r) {
         const int c = base + g;
         // per virtual lane: sums (s^T s, s^T y, y^T y, non-finite count), max residual, and for a warm
         // start the cold-start residual and its non-finite count
@@ -414,7 +414,7 @@ __device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, co
             vs[0][j] = vs[1][j] = vs[2][j] = vs[3][j] = 0.0;
             vx[0][j] = vrc[0][j] = vnfc[0][j] = 0.0;
         }
-        const ChunkMeta cm = r < kChunkRounds ? s_cm[r][g] : chunk_meta(a, c);
+        const ChunkMeta cm = r < kChunkRounds ? s_cm[r][g] : chunk_meta(a, c, nch);
         const int sy = cm.sy < 0 ? 0 : cm.sy;
         const bool act = cm.sy >= 0 && s_state[sy] == S_ACTIVE;
         if (act) {
@@ -521,9 +521,12 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     __shared__ ChunkMeta s_cm[kChunkRounds][kGrpB];
     const int ns = a.nsys;
     const int bm = a.bar_mode;
+    // row and chunk counts: on the device for a step replayed from a captured graph
+    const int nch = a.nch_dev ? (*a.nch_dev < a.nchunks ? *a.nch_dev : a.nchunks) : a.nchunks;
+    const uint32_t mrows = dcount(a.m, a.m_dev);
     if (threadIdx.x < kChunkRounds * kGrpB) {
         const int r = threadIdx.x / kGrpB, g = threadIdx.x % kGrpB;
-        s_cm[r][g] = chunk_meta(a, work_block(a.xcd_map) * kGrpB + g + r * gridDim.x * kGrpB);
+        s_cm[r][g] = chunk_meta(a, work_block(a.xcd_map) * kGrpB + g + r * gridDim.x * kGrpB, nch);
     }
     const int i0 = work_block(a.xcd_map) * blockDim.x + threadIdx.x;
     const RodMeta rm0 = i0 < a.n ? rod_meta(a, i0) : RodMeta{0u, 0u, 0};
@@ -542,7 +545,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     // start point: x = P(x0), g = A x + b
     phase_A<A_PROJ, T, AG>(a, s_alpha, s_state, rm0);
     if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
-    phase_B<B_INIT, T>(a, s_alpha, s_state, sh, s_cm);
+    phase_B<B_INIT, T>(a, s_alpha, s_state, sh, s_cm, nch);
     if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
     {
         const int S[1] = {Q_NF}, X[1] = {Q_RMAX};
@@ -571,7 +574,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
         __syncthreads();
         if (s_any) {
             // warm start rejected for those systems: x = 0, g = 0 + b
-            for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < a.m; k += gridDim.x * blockDim.x)
+            for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < mrows; k += gridDim.x * blockDim.x)
                 if (s_fb[a.rsys[a.cI[k]]]) a.xg[k] = mk2<T>(T(0), T(0) + a.b[k]);
             if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
         }
@@ -598,7 +601,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
         // alpha0 = g^T g / g^T A g
         phase_A<A_GRAD, T, AG>(a, s_alpha, s_state, rm0);
         if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
-        phase_B<B_ALPHA0, T>(a, s_alpha, s_state, sh, s_cm);
+        phase_B<B_ALPHA0, T>(a, s_alpha, s_state, sh, s_cm, nch);
         if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
         {
             const int S[2] = {Q_SS, Q_SY}, X[1] = {Q_RMAX};
@@ -615,7 +618,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
         for (int it = 1; it <= a.max_ite; ++it) {
             phase_A<A_PROJ, T, AG>(a, s_alpha, s_state, rm0);
             if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
-            phase_B<B_ITER, T>(a, s_alpha, s_state, sh, s_cm);
+            phase_B<B_ITER, T>(a, s_alpha, s_state, sh, s_cm, nch);
             if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
             {
                 const int S[4] = {Q_SS, Q_SY, Q_YY, Q_NF}, X[1] = {Q_RMAX};
@@ -763,22 +766,23 @@ HipErr launch_set_x(hipStream_t st, double2 *xg, const double *x0, uint32_t me) 
     return hip_ok();
 }
 
-__global__ void k_copy_gamma(const double2 *__restrict__ xg, double *__restrict__ gamma, uint32_t m) {
+__global__ void k_copy_gamma(const double2 *__restrict__ xg, double *__restrict__ gamma, uint32_t m,
+                             const uint32_t *__restrict__ md) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < m) gamma[k] = xg[k].x;
+    if (k < dcount(m, md)) gamma[k] = xg[k].x;
 }
 
 HipErr launch_writeback(hipStream_t st, int32_t n, const int32_t *rsys, const SysDev *sys, const uint32_t *adjStart,
                         const uint32_t *sellStart, const uint32_t *sellK, const double *sellW, uint64_t epad,
                         const double2 *xg, const double *mob, const double *unc, double *F, double *U,
-                        double *gamma, uint32_t m) {
+                        double *gamma, uint32_t m, const uint32_t *md) {
     if (n > 0) {
         k_writeback<<<nblk(n, 256), 256, 0, st>>>(n, rsys, sys, adjStart, sellStart, sellK, sellW, epad, xg, mob, unc,
                                                   F, U);
         LAUNCH_CHECK();
     }
     if (m > 0) {
-        k_copy_gamma<<<nblk(m, 256), 256, 0, st>>>(xg, gamma, m);
+        k_copy_gamma<<<nblk(m, 256), 256, 0, st>>>(xg, gamma, m, md);
         LAUNCH_CHECK();
     }
     return hip_ok();
@@ -852,9 +856,11 @@ HipErr launch_init_xg(hipStream_t st, double2 *xg, const double *x0, uint32_t m)
 
 __global__ void k_warm_match(uint32_t m, const int32_t *__restrict__ cI, const int32_t *__restrict__ cJ,
                              uint32_t mprev, const uint64_t *__restrict__ pkey,
-                             const double *__restrict__ pgam, double2 *__restrict__ xg) {
+                             const double *__restrict__ pgam, double2 *__restrict__ xg,
+                             const uint32_t *__restrict__ md, const uint32_t *__restrict__ mprev_d) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= m) return;
+    if (k >= dcount(m, md)) return;
+    if (mprev_d) mprev = *mprev_d;
     const uint64_t key = ((uint64_t)(uint32_t)cI[k] << 32) | (uint32_t)cJ[k];
     uint32_t lo = 0, hi = mprev;
     while (lo < hi) {
@@ -869,27 +875,40 @@ __global__ void k_warm_match(uint32_t m, const int32_t *__restrict__ cI, const i
 }
 HipErr launch_warm_match(hipStream_t st, uint32_t m, const int32_t *cI, const int32_t *cJ,
                          const int32_t *gid, uint32_t mprev, const uint64_t *prev_key,
-                         const double *prev_gamma, double2 *xg) {
+                         const double *prev_gamma, double2 *xg, const uint32_t *md, const uint32_t *mprev_d) {
     (void)gid;
     if (m == 0) return hip_ok();
-    k_warm_match<<<nblk(m, 256), 256, 0, st>>>(m, cI, cJ, mprev, prev_key, prev_gamma, xg);
+    k_warm_match<<<nblk(m, 256), 256, 0, st>>>(m, cI, cJ, mprev, prev_key, prev_gamma, xg, md, mprev_d);
     LAUNCH_CHECK();
     return hip_ok();
 }
 
+__global__ void k_fill_u32_one(uint32_t *p, uint32_t v) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
+}
+
 __global__ void k_make_keys(uint32_t m, const int32_t *__restrict__ cI, const int32_t *__restrict__ cJ,
                             const double2 *__restrict__ xg, uint64_t *__restrict__ key,
-                            double *__restrict__ gam) {
+                            double *__restrict__ gam, const uint32_t *__restrict__ md, uint32_t *__restrict__ mprev_d) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= m) return;
+    const uint32_t mm = dcount(m, md);
+    if (k == 0 && mprev_d) *mprev_d = mm;
+    if (k >= mm) return;
     key[k] = ((uint64_t)(uint32_t)cI[k] << 32) | (uint32_t)cJ[k];
     gam[k] = xg[k].x;
 }
 HipErr launch_make_keys(hipStream_t st, uint32_t m, const int32_t *cI, const int32_t *cJ,
-                        const int32_t *gid, const double2 *xg, uint64_t *key, double *gam) {
+                        const int32_t *gid, const double2 *xg, uint64_t *key, double *gam, const uint32_t *md,
+                        uint32_t *mprev_d) {
     (void)gid;
-    if (m == 0) return hip_ok();
-    k_make_keys<<<nblk(m, 256), 256, 0, st>>>(m, cI, cJ, xg, key, gam);
+    if (m == 0) {
+        if (mprev_d) {
+            k_fill_u32_one<<<1, 64, 0, st>>>(mprev_d, 0u);
+            LAUNCH_CHECK();
+        }
+        return hip_ok();
+    }
+    k_make_keys<<<nblk(m, 256), 256, 0, st>>>(m, cI, cJ, xg, key, gam, md, mprev_d);
     LAUNCH_CHECK();
     return hip_ok();
 }

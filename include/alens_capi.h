@@ -137,6 +137,17 @@ int64_t alens_get_step(const alens_ctx *ctx);
 alens_status alens_add_constraints(alens_ctx *ctx, int64_t m, const alens_con_block *blocks);
 alens_status alens_clear_constraints(alens_ctx *ctx);
 
+/* Graph mode (no counterpart in the reference): steps run as one captured HIP graph.  The first step
+ * sizes every buffer exactly (reading the counts of cell entries, pairs and adjacency slots back
+ * between its phases); the next step sets capacities with head room, records the whole step (U_nc,
+ * cell list, pair sweep, rows, adjacency, chunk table, start point, the persistent solver, write-back)
+ * into a graph once with every count kept on the device, and each later step replays it with one
+ * launch and one read-back.  A replay whose counts outgrow the capacities is redone exactly and the
+ * graph is recorded again.  Results are bit-identical to plain steps.  Steps with caller constraint
+ * blocks run plain.  The graph is dropped when rods, systems, the network or the schedule change. */
+alens_status alens_set_graph(alens_ctx *ctx, int32_t enable);
+alens_status alens_get_graph_info(const alens_ctx *ctx, int32_t *captured, int64_t *replays);
+
 /* One constraint step (rodSystem.runStep(), SRC/TubuleSystem.cpp:223): U_nc = M F_ext + U_B,
  * collision collection, BBPGD to tol, write-back F_con = D gamma, U = U_nc + M F_con.
  * f_ext is 6n (force, torque per rod) or NULL. */
