@@ -84,6 +84,14 @@ class TuneReport(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class GraphInfo(ctypes.Structure):
+    _fields_ = [
+        ("captured", ctypes.c_int32), ("reserved", ctypes.c_int32), ("replays", ctypes.c_int64),
+        ("captures", ctypes.c_int64), ("capture_failures", ctypes.c_int64), ("overflows", ctypes.c_int64),
+        ("last_failure", ctypes.c_char * 192),
+    ]
+
+
 class Report(ctypes.Structure):
     _fields_ = [
         ("iters", ctypes.c_int32), ("matvecs", ctypes.c_int32), ("converged", ctypes.c_int32),
@@ -263,9 +271,11 @@ class Context:
         self._check(lib().alens_set_graph(self._h, 1 if enable else 0))
 
     def graph_info(self):
-        cap, rep = ctypes.c_int32(), ctypes.c_int64()
-        self._check(lib().alens_get_graph_info(self._h, ctypes.byref(cap), ctypes.byref(rep)))
-        return dict(captured=cap.value, replays=rep.value)
+        g = GraphInfo()
+        self._check(lib().alens_get_graph_info(self._h, ctypes.byref(g)))
+        return dict(captured=g.captured, replays=g.replays, captures=g.captures,
+                    capture_failures=g.capture_failures, overflows=g.overflows,
+                    last_failure=g.last_failure.decode(errors="replace"))
 
     def autotune(self):
         """Pick the solver schedule by timing the last step's solve under candidates (alens_autotune)."""

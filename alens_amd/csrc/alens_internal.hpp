@@ -241,6 +241,7 @@ HipErr launch_euler(hipStream_t st, int32_t n, const int32_t *rsys, const SysDev
                     const double *u, const CfgDev &c);
 HipErr launch_init_xg(hipStream_t st, double2 *xg, const double *x0, uint32_t m);
 HipErr launch_set_x(hipStream_t st, double2 *xg, const double *x0, uint32_t me);
+HipErr launch_stamp(hipStream_t st, uint64_t *t, int idx);
 HipErr launch_warm_match(hipStream_t st, uint32_t m, const int32_t *cI, const int32_t *cJ,
                          const int32_t *gid, uint32_t mprev, const uint64_t *prev_key,
                          const double *prev_gamma, double2 *xg, const uint32_t *md = nullptr,

@@ -83,12 +83,15 @@ struct alens_ctx {
     int32_t bar_mode = 1, xcd_map = 0, max_blocks = 0, entries = 0;
     bool tuning = false;
     // device-resident step (graph mode): capacities, device counts, the captured step
-    bool capture = false;                  // enqueueing into a graph: no host waits, no events
+    bool capture = false;                  // enqueueing into a graph: no host waits
+    bool cap_solve_ev = false;             // the graph already reads the clock at solve start
     uint32_t gstride = 0;                  // row stride of geo (and the solver's row bound without cur_mdev)
     const uint32_t *cur_mdev = nullptr;    // row count of the current step buffers on the device
     const int32_t *cur_nchdev = nullptr;   // their chunk count on the device
     DBuf<double> out32;
     DBuf<uint32_t> dword;                  // [0] valid entries, [1] valid rows, [2] rows of the previous step
+    DBuf<uint64_t> stamps;                 // wall-clock readings in a replay: start, collected, solve start, solved
+    int64_t clock_khz = 0;                 // rate of that clock
     DBuf<int32_t> nch_dev;
     DBuf<int64_t> step_dev;
     int graph_mode = 0;
@@ -99,7 +102,8 @@ struct alens_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_key = -1;                    // start-point kind and f_ext presence the graph was captured with
-    int64_t graph_steps = 0;
+    int64_t graph_steps = 0, graph_captures = 0, graph_capfail = 0, graph_overflows = 0;
+    std::string graph_err;
     bool have_counts = false;              // E, m, epad hold a step's counts (planning input)
 
     DBuf<uint64_t> pkey;
@@ -626,12 +630,20 @@ static alens_status run_bbpgd(alens_ctx *ctx, SolveArgsT<T> a, alens_solve_repor
     if (a.bar_mode < 0 || a.bar_mode > 2) a.bar_mode = 0;
     a.xcd_map = env("ALENS_XCD_MAP", ctx->xcd_map) != 0;
     const int entries = env("ALENS_SOLVER_ENTRIES", ctx->entries);
-    if (!ctx->capture) TRY_HIP(hipEventRecord(ctx->ev[2], ctx->st));
+    // inside a graph the timing points are clock readings: stamp 2 before the first solver launch,
+    // stamp 3 after each one (the last reading is the one read after the replay)
+    if (!ctx->capture) {
+        TRY_HIP(hipEventRecord(ctx->ev[2], ctx->st));
+    } else if (!ctx->cap_solve_ev) {
+        TRY_K(launch_stamp(ctx->st, ctx->stamps.p, 2));
+        ctx->cap_solve_ev = true;
+    }
     if constexpr (sizeof(T) == 8)
         TRY_K(launch_bbpgd64(ctx->st, a, max_blocks, entries, &ctx->grid_used));
     else
         TRY_K(launch_bbpgd32(ctx->st, a, max_blocks, entries, &ctx->grid_used));
     if (ctx->capture) {
+        TRY_K(launch_stamp(ctx->st, ctx->stamps.p, 3));
         *ms_solve = 0.f;
         return ALENS_OK;
     }
@@ -1068,6 +1080,12 @@ static alens_status plan_resident(alens_ctx *ctx) {
     TRY_HIP(ctx->dword.ensure(4));
     TRY_HIP(ctx->nch_dev.ensure(1));
     TRY_HIP(ctx->step_dev.ensure(1));
+    TRY_HIP(ctx->stamps.ensure(4));
+    if (ctx->clock_khz <= 0) {
+        int khz = 0;
+        TRY_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->dev));
+        ctx->clock_khz = khz;
+    }
     if (ctx->cfg.warm_start == 2) TRY_HIP(ctx->feat.ensure(16 * (size_t)c.M));
     if (ctx->cfg.precision == 1) {
         TRY_HIP(ctx->geo32.ensure(9 * (size_t)c.M));
@@ -1092,6 +1110,7 @@ static alens_status enqueue_resident(alens_ctx *ctx, bool fx, bool warm_prev) {
     cd.step_dev = ctx->step_dev.p;
     const RodsDev rd = make_rods(ctx);
     uint32_t *Ee = ctx->dword.p + 0, *Mv = ctx->dword.p + 1, *mprev = ctx->dword.p + 2;
+    TRY_K(launch_stamp(st, ctx->stamps.p, 0));
     TRY_K(launch_mob_unc(st, rd, ctx->length.p, ctx->radius.p, ctx->has_immov ? ctx->immov.p : nullptr,
                          fx ? ctx->fext.p : nullptr, cd, ctx->mob.p, ctx->unc.p));
     TRY_K(launch_aabb(st, rd, cd, ctx->clo.p, ctx->chi.p, ctx->blo.p, ctx->bhi.p, ctx->ecnt.p));
@@ -1130,6 +1149,7 @@ static alens_status enqueue_resident(alens_ctx *ctx, bool fx, bool warm_prev) {
     TRY_K(launch_chunk_table(st, ctx->nsys, n, ctx->sysd.p, ctx->rowStart.p, 0u, ctx->con_off.p, ctx->chunk_off.p,
                              ctx->nch_dev.p, ctx->chunk_sys.p, c.nch, Mv));
     // start point and solve (the solver reads the row and chunk counts from the device)
+    TRY_K(launch_stamp(st, ctx->stamps.p, 1));
     ctx->gstride = c.M;
     ctx->epad = c.epad;
     ctx->nchunks_cur = c.nch;
@@ -1175,6 +1195,7 @@ static alens_status capture_resident(alens_ctx *ctx, bool fx, bool warm_prev) {
     TRY_HIP(hipStreamSynchronize(ctx->st));
     TRY_HIP(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
     ctx->capture = true;
+    ctx->cap_solve_ev = false;
     alens_status s = enqueue_resident(ctx, fx, warm_prev);
     ctx->capture = false;
     hipGraph_t g = nullptr;
@@ -1187,6 +1208,7 @@ static alens_status capture_resident(alens_ctx *ctx, bool fx, bool warm_prev) {
     ctx->graph = g;
     TRY_HIP(hipGraphInstantiate(&ctx->gexec, ctx->graph, nullptr, nullptr, 0));
     ctx->graph_key = (fx ? 1 : 0) | (warm_prev ? 2 : 0);
+    ++ctx->graph_captures;
     return ALENS_OK;
 }
 
@@ -1202,6 +1224,8 @@ static alens_status step_resident(alens_ctx *ctx, const double *f_ext, alens_sol
     TRY_HIP(hipEventRecord(ctx->ev[0], st));
     TRY_HIP(hipGraphLaunch(ctx->gexec, st));
     TRY_HIP(hipEventRecord(ctx->ev[4], st));
+    uint64_t stamp[4] = {0, 0, 0, 0};
+    TRY_HIP(hipMemcpyAsync(stamp, ctx->stamps.p, sizeof stamp, hipMemcpyDeviceToHost, st));
     const int64_t nsl = ((int64_t)n + kSlice - 1) / kSlice;
     uint32_t cnt[5] = {0, 0, 0, 0, 0};  // entries, pairs, slots, valid rows, chunks
     TRY_HIP(hipMemcpyAsync(&cnt[0], ctx->eoff.p + n, 4, hipMemcpyDeviceToHost, st));
@@ -1214,6 +1238,7 @@ static alens_status step_resident(alens_ctx *ctx, const double *f_ext, alens_sol
         // every overflow left zero valid rows downstream, so the replay wrote no impulse cache and
         // the plain redo starts from the same previous-step impulses
         drop_graph(ctx);
+        ++ctx->graph_overflows;
         return step_plain(ctx, f_ext, rep_out);
     }
     alens_solve_report rep{};
@@ -1229,6 +1254,9 @@ static alens_status step_resident(alens_ctx *ctx, const double *f_ext, alens_sol
     }
     float ms_step = 0.f;
     TRY_HIP(hipEventElapsedTime(&ms_step, ctx->ev[0], ctx->ev[4]));
+    const double tick_ms = ctx->clock_khz > 0 ? 1.0 / (double)ctx->clock_khz : 0.0;
+    const double ms_collect = (double)(int64_t)(stamp[1] - stamp[0]) * tick_ms;
+    const double ms_solve = (double)(int64_t)(stamp[3] - stamp[2]) * tick_ms;
     ctx->E = cnt[0];
     ctx->m = cnt[3];
     ctx->mcol = cnt[3];
@@ -1243,7 +1271,9 @@ static alens_status step_resident(alens_ctx *ctx, const double *f_ext, alens_sol
     ++ctx->graph_steps;
     rep.n_con = cnt[3];
     rep.n_entries = cnt[0];
-    rep.t_step_ms = ms_step;  // collection and solve run inside one graph: no split
+    rep.t_step_ms = ms_step;
+    rep.t_collect_ms = ms_collect;  // U_nc, cell list, sweep, rows, adjacency, chunk table
+    rep.t_solve_ms = ms_solve;      // the solver launch(es), from clock readings in the graph
     if (rep_out) *rep_out = rep;
     if (rep.status == ALENS_E_NONFINITE) return fail(ctx, ALENS_E_NONFINITE, "non-finite BBPGD residual");
     if (rep.status == ALENS_E_NOCONV) return fail(ctx, ALENS_E_NOCONV, "BBPGD did not reach tol within max_ite");
@@ -1265,6 +1295,8 @@ alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_s
         alens_status p = plan_resident(ctx);
         if (p == ALENS_OK) p = capture_resident(ctx, fx, warm_prev);
         if (p != ALENS_OK) {
+            ++ctx->graph_capfail;
+            ctx->graph_err = ctx->err;
             drop_graph(ctx);
             return step_plain(ctx, f_ext, rep_out);
         }
@@ -1279,10 +1311,16 @@ alens_status alens_set_graph(alens_ctx *ctx, int32_t enable) {
     return ALENS_OK;
 }
 
-alens_status alens_get_graph_info(const alens_ctx *ctx, int32_t *captured, int64_t *replays) {
+alens_status alens_get_graph_info(const alens_ctx *ctx, alens_graph_info *info) {
     if (!ctx) return ALENS_E_ARG;
-    if (captured) *captured = ctx->gexec ? 1 : 0;
-    if (replays) *replays = ctx->graph_steps;
+    if (!info) return ALENS_E_ARG;
+    *info = alens_graph_info{};
+    info->captured = ctx->gexec ? 1 : 0;
+    info->replays = ctx->graph_steps;
+    info->captures = ctx->graph_captures;
+    info->capture_failures = ctx->graph_capfail;
+    info->overflows = ctx->graph_overflows;
+    snprintf(info->last_failure, sizeof info->last_failure, "%s", ctx->graph_err.c_str());
     return ALENS_OK;
 }
 

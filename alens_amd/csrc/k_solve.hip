@@ -887,6 +887,17 @@ __global__ void k_fill_u32_one(uint32_t *p, uint32_t v) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
 }
 
+// One reading of the device's constant-rate wall clock into t[idx]: the timing points of a captured
+// step (an event record inside stream capture is rejected by some HIP runtime builds).
+__global__ void k_stamp(uint64_t *t, int idx) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) t[idx] = wall_clock64();
+}
+HipErr launch_stamp(hipStream_t st, uint64_t *t, int idx) {
+    k_stamp<<<1, 64, 0, st>>>(t, idx);
+    LAUNCH_CHECK();
+    return hip_ok();
+}
+
 __global__ void k_make_keys(uint32_t m, const int32_t *__restrict__ cI, const int32_t *__restrict__ cJ,
                             const double2 *__restrict__ xg, uint64_t *__restrict__ key,
                             double *__restrict__ gam, const uint32_t *__restrict__ md, uint32_t *__restrict__ mprev_d) {

@@ -54,6 +54,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default solver schedule (otherwise the autotuner picks it after warm-up)")
+    ap.add_argument("--graph", type=int, default=1, choices=[0, 1],
+                    help="1: each step after the first is one replay of a captured HIP graph (alens_set_graph)")
     ap.add_argument("--progress", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="before the run, one device-to-device copy of this many MiB (a kernel of known "
@@ -81,14 +83,17 @@ def main():
         torch.add(src, 1, out=dst)  # elementwise: reads nb bytes, writes nb bytes
         torch.cuda.synchronize()
         del src, dst
+    # (skipped in profiled runs: its copy kernels would sit beside the calibration copy)
+    copy_gbs = None if args.calib_copy_mib else stream_copy_gbs()
     if args.workload == "replicas":
-        return bench_replicas(args, capi, systems)
+        return bench_replicas(args, capi, systems, copy_gbs)
     gen = {"monolayer_slab": systems.monolayer_slab, "lognormal_3d": systems.lognormal_3d,
            "small_3d": systems.small_3d}[args.workload]
     rods, cfg = gen(n=args.n) if args.n else gen()
     n = len(rods["length"])
     ctx = capi.Context(**dict(cfg, precision=args.precision))
     ctx.set_rods(rods)
+    ctx.set_graph(bool(args.graph))
     # spatial (Morton cell) order, as the reference's FDPS containers keep it; later arrays follow it
     perm = ctx.sort_rods()
     rods = {k: (v[perm] if isinstance(v, np.ndarray) else v) for k, v in rods.items()}
@@ -109,6 +114,7 @@ def main():
         ctx.step()
         tune = ctx.autotune()
         ctx.euler()
+        one_step()  # untimed: in graph mode the autotuner dropped the graph, and this records it again
     reps = []
     t0 = time.perf_counter()
     for s in range(args.steps):
@@ -164,8 +170,13 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
             "algorithmic_bytes_per_launch": float(alg / K),
+            "stream_copy_gbs": copy_gbs,
+            "frac_of_stream_copy": achieved / copy_gbs if copy_gbs else None,
         },
+        "graph": ctx.graph_info() if args.graph else None,
     }
+    if args.graph and out["graph"]["replays"] == 0:
+        print(f"graph mode did not replay: {out['graph']}", file=sys.stderr, flush=True)
     # measured HBM traffic per launch: bytes per matvec from the committed rocprofv3 FETCH_SIZE /
     # WRITE_SIZE passes on this workload, times this run's matvecs per launch
     tp = os.path.join(ROOT, "profiles", "k_bbpgd_traffic.json")
@@ -182,6 +193,7 @@ def main():
     if not args.no_form:
         fctx = capi.Context(**dict(cfg, precision=1, warm_start=1))
         fctx.set_rods(rods)
+        fctx.set_graph(bool(args.graph))
         fctx.sort_rods()
         for w in range(args.warmup):
             fctx.step()
@@ -189,6 +201,8 @@ def main():
         if not args.no_autotune and args.warmup > 0:
             fctx.step()
             fctx.autotune()
+            fctx.euler()
+            fctx.step()
             fctx.euler()
         freps = []
         f0 = time.perf_counter()
@@ -334,17 +348,49 @@ def rocprof_summary(args):
         json.dump(res, fh, indent=1)
 
 
-def bench_replicas(args, capi, systems):
+def stream_copy_gbs(nbytes=1 << 30, reps=10):
+    """Device-to-device copy bandwidth on this GPU (read + write bytes per second, GB/s): the
+    practical HBM ceiling beside the published 8 TB/s."""
+    try:
+        import torch
+    except ImportError:
+        return None
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2.0 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return gbs
+
+
+def bench_replicas(args, capi, systems, copy_gbs=None):
     """64 independent replicas (rod counts log-uniform in [2k, 50k], 3-D boxes and slabs) stepped
     together: one collection pass and one persistent solver launch per step for all of them."""
     parts, cfg = systems.replicas()
     ctx = capi.Context(**dict(cfg, precision=args.precision, warm_start=1))
     offs = ctx.set_systems(parts)
+    ctx.set_graph(bool(args.graph))
     ctx.sort_rods()
     n = int(offs[-1])
     for w in range(args.warmup):
         ctx.step()
         ctx.euler()
+    if not args.no_autotune and args.warmup > 0:
+        ctx.step()
+        tune = ctx.autotune()
+        ctx.euler()
+        ctx.step()
+        ctx.euler()
+    else:
+        tune = None
     reps, sreps = [], []
     t0 = time.perf_counter()
     for s in range(args.steps):
@@ -367,6 +413,9 @@ def bench_replicas(args, capi, systems):
         "replica_iters_histogram": {"edges": [float(e) for e in edges[:-1]], "counts": hist.tolist()},
         "t_solve_ms_mean": float(np.mean([r["t_solve_ms"] for r in reps])),
         "t_collect_ms_mean": float(np.mean([r["t_collect_ms"] for r in reps])),
+        "solver_schedule": tune,
+        "graph": ctx.graph_info() if args.graph else None,
+        "stream_copy_gbs": copy_gbs,
     }
     print(json.dumps(out), flush=True)
 

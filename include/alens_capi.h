@@ -146,7 +146,16 @@ alens_status alens_clear_constraints(alens_ctx *ctx);
  * graph is recorded again.  Results are bit-identical to plain steps.  Steps with caller constraint
  * blocks run plain.  The graph is dropped when rods, systems, the network or the schedule change. */
 alens_status alens_set_graph(alens_ctx *ctx, int32_t enable);
-alens_status alens_get_graph_info(const alens_ctx *ctx, int32_t *captured, int64_t *replays);
+typedef struct {
+    int32_t captured;         /* a recorded graph is ready for the next step */
+    int32_t reserved;
+    int64_t replays;          /* steps run as one graph replay */
+    int64_t captures;         /* graphs recorded */
+    int64_t capture_failures; /* recordings that failed (those steps ran plain) */
+    int64_t overflows;        /* replays redone plainly because a count outgrew its capacity */
+    char last_failure[192];   /* message of the last failed recording, or empty */
+} alens_graph_info;
+alens_status alens_get_graph_info(const alens_ctx *ctx, alens_graph_info *info);
 
 /* One constraint step (rodSystem.runStep(), SRC/TubuleSystem.cpp:223): U_nc = M F_ext + U_B,
  * collision collection, BBPGD to tol, write-back F_con = D gamma, U = U_nc + M F_con.

@@ -39,10 +39,10 @@ def test_struct_layouts_match_c():
 #include <stddef.h>
 #include "alens_capi.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(alens_config), offsetof(alens_config, seed),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(alens_config), offsetof(alens_config, seed),
          offsetof(alens_config, warm_start), sizeof(alens_con_block), offsetof(alens_con_block, stress),
          sizeof(alens_solve_report), offsetof(alens_solve_report, t_step_ms), sizeof(alens_tune_report),
-         offsetof(alens_tune_report, best_ms));
+         offsetof(alens_tune_report, best_ms), sizeof(alens_graph_info), offsetof(alens_graph_info, last_failure));
   return 0;
 }'''
     with tempfile.TemporaryDirectory() as d:
@@ -53,7 +53,8 @@ int main(void) {
         vals = [int(v) for v in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
     want = [ctypes.sizeof(capi.Config), capi.Config.seed.offset, capi.Config.warm_start.offset,
             ctypes.sizeof(capi.ConBlock), capi.ConBlock.stress.offset, ctypes.sizeof(capi.Report),
-            capi.Report.t_step_ms.offset, ctypes.sizeof(capi.TuneReport), capi.TuneReport.best_ms.offset]
+            capi.Report.t_step_ms.offset, ctypes.sizeof(capi.TuneReport), capi.TuneReport.best_ms.offset,
+            ctypes.sizeof(capi.GraphInfo), capi.GraphInfo.last_failure.offset]
     assert vals == want
 
 

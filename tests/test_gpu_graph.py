@@ -27,6 +27,8 @@ def _run_and_compare(a, b, steps, what):
         ra, rb = a.step(), b.step()
         assert ra["n_con"] == rb["n_con"] and ra["iters"] == rb["iters"], (what, s, ra, rb)
         assert ra["iters_fp32"] == rb["iters_fp32"] and ra["converged"] == rb["converged"], (what, s)
+        # replays time collection and solve with event-record nodes inside the graph
+        assert 0.0 < ra["t_collect_ms"] and 0.0 < ra["t_solve_ms"] <= ra["t_step_ms"], (what, s, ra)
         assert_bits([ra["res0"], ra["res"]], [rb["res0"], rb["res"]], f"{what} step {s} residuals")
         assert_bits(a.gamma(), b.gamma(), f"{what} step {s} gamma")
         va, vb = a.velocity(), b.velocity()
