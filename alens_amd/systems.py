@@ -72,8 +72,9 @@ def scaled(kind, n, seed=1234):
 
 
 def read_rod_file(path):
-    """Parse the reference's rod text format.  'S' rods are stationary (immovable)."""
-    gid, rad, mi, pl, imm = [], [], [], [], []
+    """Parse the reference's rod text format (TubuleInitial.dat, SylinderAscii_*.dat).  'S' rods are
+    stationary (immovable); the optional last column is the group id (-1 when absent)."""
+    gid, rad, mi, pl, imm, grp = [], [], [], [], [], []
     with open(path) as fh:
         for line in fh:
             tok = line.split()
@@ -84,6 +85,7 @@ def read_rod_file(path):
             mi.append([float(x) for x in tok[3:6]])
             pl.append([float(x) for x in tok[6:9]])
             imm.append(1 if tok[0] == "S" else 0)
+            grp.append(int(tok[9]) if len(tok) > 9 else -1)
     mi = np.array(mi, dtype=np.float64).reshape(-1, 3)
     pl = np.array(pl, dtype=np.float64).reshape(-1, 3)
     v = pl - mi
@@ -91,21 +93,65 @@ def read_rod_file(path):
     d = v / np.where(L > 0, L, 1.0)[:, None]
     d[L == 0] = (0.0, 0.0, 1.0)
     return dict(gid=np.array(gid, np.int32), center=0.5 * (mi + pl), dir=d, length=L,
-                radius=np.array(rad, dtype=np.float64), immovable=np.array(imm, np.uint8))
+                radius=np.array(rad, dtype=np.float64), immovable=np.array(imm, np.uint8),
+                group=np.array(grp, np.int32))
 
 
 def write_rod_file(path, rods, header_time=0.0):
-    """Write rods in the reference's text format (minus end, plus end per rod)."""
+    """Write rods in the reference's text format: two header lines (rod count, time), then one line
+    per rod with type, gid, radius, minus end, plus end and group (17 significant digits, so the
+    endpoints read back exactly)."""
     c, d, L = rods["center"], rods["dir"], rods["length"]
     imm = rods.get("immovable")
+    grp = rods.get("group")
     mi = c - 0.5 * L[:, None] * d
     pl = c + 0.5 * L[:, None] * d
     with open(path, "w") as fh:
-        fh.write(f"{len(L)}\n{header_time:f}\n")
+        fh.write(f"{len(L)}\n{header_time:.17g}\n")
         for k in range(len(L)):
             t = "S" if (imm is not None and imm[k]) else "C"
+            g = int(grp[k]) if grp is not None else -1
             fh.write(f"{t} {int(rods['gid'][k])} {rods['radius'][k]:.17g} " +
-                     " ".join(f"{x:.17g}" for x in (*mi[k], *pl[k])) + "\n")
+                     " ".join(f"{x:.17g}" for x in (*mi[k], *pl[k])) + f" {g}\n")
+
+
+# Snapshots and restart.  The reference writes SylinderAscii_<snap>.dat into its result folders
+# (README.md:128-135; read back by Utils/read_write_utils.py:185-197, 210-213) and restarts when
+# TimeStepInfo.txt exists (SRC/main.cpp:28-33), resuming after the last snapshot with the snapshot
+# id and step count advanced (SRC/TubuleSystem.cpp:36-38, 48-49).  The content of TimeStepInfo.txt
+# is written by SimToolbox (absent); here it is "key value" lines naming the snapshot file.
+RESTART_FILE = "TimeStepInfo.txt"
+
+
+def write_snapshot(folder, snap_id, rods, step, time=0.0):
+    """Write SylinderAscii_<snap_id>.dat and the restart record into folder; returns the rod file path."""
+    import os
+
+    os.makedirs(folder, exist_ok=True)
+    name = f"SylinderAscii_{int(snap_id)}.dat"
+    write_rod_file(os.path.join(folder, name), rods, header_time=time)
+    with open(os.path.join(folder, RESTART_FILE), "w") as fh:
+        fh.write(f"snapID {int(snap_id)}\nstepCount {int(step)}\ntime {float(time):.17g}\nsnapFile {name}\n")
+    return os.path.join(folder, name)
+
+
+def read_restart(folder):
+    """Rods of the snapshot named by folder/TimeStepInfo.txt, and where to resume: the step count and
+    snapshot id after it (both advanced by one, as the reference's reinitialize does), and the time."""
+    import os
+
+    info = {}
+    with open(os.path.join(folder, RESTART_FILE)) as fh:
+        for line in fh:
+            tok = line.split(None, 1)
+            if len(tok) == 2:
+                info[tok[0]] = tok[1].strip()
+    for key in ("snapID", "stepCount", "snapFile"):
+        if key not in info:
+            raise ValueError(f"{RESTART_FILE}: missing '{key}'")
+    rods = read_rod_file(os.path.join(folder, info["snapFile"]))
+    return rods, dict(step=int(info["stepCount"]) + 1, snap_id=int(info["snapID"]) + 1,
+                      time=float(info.get("time", "0")))
 
 
 def replicas(nsys=64, nmin=2000, nmax=50000, seed=1234):

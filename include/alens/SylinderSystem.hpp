@@ -5,11 +5,17 @@
 // names and call shapes so that code written against them moves onto the GPU engine unchanged:
 //   SylinderSystem::initialize / prepareStep / runStep / stepEuler / getConstraintPoolNonConst /
 //   getStepCount, the per-thread ConstraintBlockPool (pool.size() == thread count, one queue per
-//   thread), ConstraintBlock's 16-argument constructor and setStress, and BCQPSolver::solveBBPGD.
+//   thread), ConstraintBlock's 16-argument constructor and setStress, and BCQPSolver::solveBBPGD;
+//   rod files (TubuleInitial.dat / SylinderAscii_*.dat), writeResult, reinitialize and getSnapID.
 // Errors surface as alens::Error exceptions carrying the alens_status and message.
 #pragma once
+#include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -89,8 +95,56 @@ struct Rods {
     std::vector<int32_t> gid;
     std::vector<double> center, direction, length, radius;  // center/direction: 3 per rod
     std::vector<uint8_t> immovable;                          // empty = all movable
+    std::vector<int32_t> group;                              // empty = -1 for every rod
     int32_t size() const { return (int32_t)gid.size(); }
 };
+
+// Rod text files (docs/source/quickstart.md:215-221): two header lines (count, time), then
+// "C|S gid radius minus(3) plus(3) [group]" per rod; 'S' = stationary.  Lines that do not start with
+// C or S (comments, links) are skipped.
+inline Rods readRodFile(const std::string &path) {
+    std::ifstream in(path);
+    if (!in) throw Error(ALENS_E_ARG, "cannot open rod file " + path);
+    Rods r;
+    std::string line;
+    while (std::getline(in, line)) {
+        std::istringstream ls(line);
+        std::string type;
+        int32_t gid = 0;
+        double rad = 0, e[6];
+        if (!(ls >> type) || (type != "C" && type != "S")) continue;
+        if (!(ls >> gid >> rad >> e[0] >> e[1] >> e[2] >> e[3] >> e[4] >> e[5])) continue;
+        int32_t grp = -1;
+        if (!(ls >> grp)) grp = -1;
+        const double v[3] = {e[3] - e[0], e[4] - e[1], e[5] - e[2]};
+        const double L = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        r.gid.push_back(gid);
+        r.radius.push_back(rad);
+        r.length.push_back(L);
+        for (int k = 0; k < 3; ++k) r.center.push_back(0.5 * (e[k] + e[3 + k]));
+        if (L > 0)
+            r.direction.insert(r.direction.end(), {v[0] / L, v[1] / L, v[2] / L});
+        else
+            r.direction.insert(r.direction.end(), {0.0, 0.0, 1.0});
+        r.immovable.push_back(type == "S" ? 1 : 0);
+        r.group.push_back(grp);
+    }
+    return r;
+}
+
+inline void writeRodFile(const std::string &path, const Rods &r, double time) {
+    std::FILE *f = std::fopen(path.c_str(), "w");
+    if (!f) throw Error(ALENS_E_ARG, "cannot write rod file " + path);
+    std::fprintf(f, "%d\n%.17g\n", r.size(), time);
+    for (int32_t i = 0; i < r.size(); ++i) {
+        const double *c = &r.center[3 * (size_t)i], *t = &r.direction[3 * (size_t)i], h = 0.5 * r.length[i];
+        std::fprintf(f, "%c %d %.17g %.17g %.17g %.17g %.17g %.17g %.17g %d\n",
+                     (!r.immovable.empty() && r.immovable[i]) ? 'S' : 'C', r.gid[i], r.radius[i], c[0] - h * t[0],
+                     c[1] - h * t[1], c[2] - h * t[2], c[0] + h * t[0], c[1] + h * t[1], c[2] + h * t[2],
+                     r.group.empty() ? -1 : r.group[i]);
+    }
+    std::fclose(f);
+}
 
 class SylinderSystem {
   public:
@@ -108,8 +162,59 @@ class SylinderSystem {
         check(ctx_, alens_set_rods(ctx_, n_, r.gid.data(), r.center.data(), r.direction.data(), r.length.data(),
                                    r.radius.data(), r.immovable.empty() ? nullptr : r.immovable.data()),
               "set_rods");
-        if (spatialSort) check(ctx_, alens_sort_rods(ctx_, nullptr), "sort_rods");
+        rods_ = r;
+        if (spatialSort && n_ > 0) {
+            std::vector<int32_t> perm((size_t)n_);
+            check(ctx_, alens_sort_rods(ctx_, perm.data()), "sort_rods");
+            Rods s;
+            for (int32_t i : perm) {
+                s.gid.push_back(r.gid[i]);
+                s.length.push_back(r.length[i]);
+                s.radius.push_back(r.radius[i]);
+                if (!r.immovable.empty()) s.immovable.push_back(r.immovable[i]);
+                if (!r.group.empty()) s.group.push_back(r.group[i]);
+            }
+            rods_ = std::move(s);
+        }
     }
+    // initialize(posFile): rods from a rod file (SRC/TubuleSystem.cpp:79 passes "TubuleInitial.dat")
+    void initialize(const std::string &posFile, bool spatialSort = true) { initialize(readRodFile(posFile), spatialSort); }
+
+    // writeResult(): SylinderAscii_<snapID>.dat with the current rods (17 significant digits) and the
+    // restart record TimeStepInfo.txt ("key value" lines: snapID, stepCount, time, snapFile) in folder;
+    // call it after stepEuler, so the file holds the positions the next step starts from.
+    void writeResult(const std::string &folder, double time) {
+        Rods r = rods_;
+        getPositions(r.center, r.direction);
+        const std::string name = "SylinderAscii_" + std::to_string(snapID_) + ".dat";
+        writeRodFile(folder + "/" + name, r, time);
+        std::ofstream info(folder + "/TimeStepInfo.txt");
+        info << "snapID " << snapID_ << "\nstepCount " << (getStepCount() - 1) << "\ntime ";
+        char buf[40];
+        std::snprintf(buf, sizeof buf, "%.17g", time);
+        info << buf << "\nsnapFile " << name << "\n";
+        if (!info) throw Error(ALENS_E_ARG, "cannot write " + folder + "/TimeStepInfo.txt");
+        ++snapID_;
+    }
+    // reinitialize(): resume from the snapshot named by a restart record (SRC/main.cpp:28-33,
+    // SRC/TubuleSystem.cpp:38): rods from its file, step count and snapshot id one past the record's.
+    // The file keeps the rods in the order the writing system held them (already spatially sorted),
+    // so by default they are not sorted again and the constraint rows come out in the same order.
+    void reinitialize(const std::string &restartFile, bool spatialSort = false) {
+        std::ifstream in(restartFile);
+        if (!in) throw Error(ALENS_E_ARG, "cannot open restart record " + restartFile);
+        std::map<std::string, std::string> kv;
+        std::string key, val;
+        while (in >> key >> val) kv[key] = val;
+        for (const char *k : {"snapID", "stepCount", "snapFile"})
+            if (!kv.count(k)) throw Error(ALENS_E_ARG, restartFile + ": missing '" + k + "'");
+        const size_t slash = restartFile.find_last_of('/');
+        const std::string folder = slash == std::string::npos ? "." : restartFile.substr(0, slash);
+        initialize(readRodFile(folder + "/" + kv["snapFile"]), spatialSort);
+        check(ctx_, alens_set_step(ctx_, std::stoll(kv["stepCount"]) + 1), "set_step");
+        snapID_ = std::stoi(kv["snapID"]) + 1;
+    }
+    int getSnapID() const { return snapID_; }
     // prepareStep(): clears the constraint pool for this step's external blocks
     void prepareStep() {
         for (auto &q : pool_) q.clear();
@@ -162,6 +267,8 @@ class SylinderSystem {
   private:
     alens_ctx *ctx_ = nullptr;
     int32_t n_ = 0;
+    int snapID_ = 0;
+    Rods rods_;  // per-rod data in the context's (sorted) order; positions live on the device
     ConstraintBlockPool pool_;
     alens_solve_report report_{};
 };

@@ -58,10 +58,10 @@ int main(void) {
     assert vals == want
 
 
-def build_adapter_driver(outdir):
-    exe = os.path.join(outdir, "adapter_step")
+def build_adapter_driver(outdir, name="adapter_step"):
+    exe = os.path.join(outdir, name)
     subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
-                    os.path.join(ROOT, "tests", "cpp", "adapter_step.cpp"), "-o", exe,
+                    os.path.join(ROOT, "tests", "cpp", name + ".cpp"), "-o", exe,
                     "-L", os.path.dirname(capi.LIB_PATH), "-lalens", "-Wl,-rpath," + os.path.dirname(capi.LIB_PATH)],
                    check=True)
     return exe
@@ -71,3 +71,18 @@ def test_cpp_adapter_compiles_and_links():
     with tempfile.TemporaryDirectory() as d:
         exe = build_adapter_driver(d)
         assert os.path.exists(exe)
+
+
+def test_cpp_rod_file_reader():
+    """The C++ adapter's rod-file reader on the reference's inputs (group column and 'S' rods)."""
+    golden = os.path.join(ROOT, "tests", "golden")
+    with tempfile.TemporaryDirectory() as d:
+        exe = build_adapter_driver(d, "adapter_restart")
+        out = {}
+        for f in ("dense_monolayer_TubuleInitial.dat", "mix_motor_sliding_TubuleInitial.dat"):
+            p = subprocess.run([exe, "read", os.path.join(golden, f)], capture_output=True, text=True, timeout=60)
+            assert p.returncode == 0, p.stdout + p.stderr
+            tok = p.stdout.split()
+            out[f] = dict(zip(tok[0::2], (int(v) for v in tok[1::2])))
+    assert out["dense_monolayer_TubuleInitial.dat"] == dict(rods=9700, stationary=0, group0=-1)
+    assert out["mix_motor_sliding_TubuleInitial.dat"] == dict(rods=2, stationary=1, group0=-1)
