@@ -144,7 +144,8 @@ alens_status alens_clear_constraints(alens_ctx *ctx);
  * into a graph once with every count kept on the device, and each later step replays it with one
  * launch and one read-back.  A replay whose counts outgrow the capacities is redone exactly and the
  * graph is recorded again.  Results are bit-identical to plain steps.  Steps with caller constraint
- * blocks run plain.  The graph is dropped when rods, systems, the network or the schedule change. */
+ * blocks run plain.  The graph is dropped when rods, systems, the network or the schedule change.
+ * A replay's report times collection and solve from device clock readings taken inside the graph. */
 alens_status alens_set_graph(alens_ctx *ctx, int32_t enable);
 typedef struct {
     int32_t captured;         /* a recorded graph is ready for the next step */
