@@ -773,20 +773,56 @@ HipErr launch_rows_ext(hipStream_t st, uint32_t me, uint32_t k0, uint32_t gs, co
 // (rods kSlice*s ..) holds kSlice * (largest degree in the slice) slots, entry j of rod i at
 // sellStart[s] + j * kSlice + i % kSlice.
 // ------------------------------------------------------------------------------------------------
-__global__ void k_slice_sizes(int32_t n, const uint32_t *__restrict__ rowStart, const uint32_t *__restrict__ jStart,
-                              const uint32_t *__restrict__ extStart, uint32_t *__restrict__ slice_len) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;  // one lane per rod, one wave per slice
-    uint32_t deg = 0;
-    if (i < n) {
-        deg = (rowStart[i + 1] - rowStart[i]) + (jStart[i + 1] - jStart[i]);
-        if (extStart) deg += extStart[i + 1] - extStart[i];
+__device__ inline uint32_t rod_degree(const uint32_t *rowStart, const uint32_t *jStart, const uint32_t *extStart,
+                                      int32_t i) {
+    uint32_t deg = (rowStart[i + 1] - rowStart[i]) + (jStart[i + 1] - jStart[i]);
+    if (extStart) deg += extStart[i + 1] - extStart[i];
+    return deg;
+}
+
+// one block per window of kOrderWin rods: rank = number of rods of the window with a larger degree,
+// or the same degree and a smaller index
+__global__ __launch_bounds__(kOrderWin) void k_slot_order(int32_t n, const uint32_t *__restrict__ rowStart,
+                                                          const uint32_t *__restrict__ jStart,
+                                                          const uint32_t *__restrict__ extStart,
+                                                          uint32_t *__restrict__ spos, uint32_t *__restrict__ sord) {
+    __shared__ int64_t s_deg[kOrderWin];
+    const int32_t base = blockIdx.x * kOrderWin;
+    const int t = threadIdx.x;
+    const int32_t i = base + t;
+    const int64_t d = i < n ? (int64_t)rod_degree(rowStart, jStart, extStart, i) : -1;
+    s_deg[t] = d;
+    __syncthreads();
+    if (i >= n) return;
+    int rank = 0;
+    for (int j = 0; j < kOrderWin; ++j) {
+        const int64_t e = s_deg[j];
+        rank += (e > d || (e == d && j < t)) ? 1 : 0;
     }
+    spos[i] = (uint32_t)(base + rank);
+    sord[base + rank] = (uint32_t)i;
+}
+
+HipErr launch_slot_order(hipStream_t st, int32_t n, const uint32_t *rowStart, const uint32_t *jStart,
+                         const uint32_t *extStart, uint32_t *spos, uint32_t *sord) {
+    if (n == 0) return hip_ok();
+    k_slot_order<<<nblk(n, kOrderWin), kOrderWin, 0, st>>>(n, rowStart, jStart, extStart, spos, sord);
+    LAUNCH_CHECK();
+    return hip_ok();
+}
+
+__global__ void k_slice_sizes(int32_t n, const uint32_t *__restrict__ rowStart, const uint32_t *__restrict__ jStart,
+                              const uint32_t *__restrict__ extStart, uint32_t *__restrict__ slice_len,
+                              const uint32_t *__restrict__ sord) {
+    const int32_t p = blockIdx.x * blockDim.x + threadIdx.x;  // one lane per slice position, one wave per slice
+    uint32_t deg = 0;
+    if (p < n) deg = rod_degree(rowStart, jStart, extStart, sord ? (int32_t)sord[p] : p);
 #pragma unroll
     for (int h = 32; h >= 1; h >>= 1) {
         const uint32_t o = __shfl_xor(deg, h, 64);
         deg = o > deg ? o : deg;
     }
-    if ((threadIdx.x & 63) == 0 && i < n) slice_len[i >> 6] = deg * (uint32_t)kSlice;
+    if ((threadIdx.x & 63) == 0 && p < n) slice_len[p >> 6] = deg * (uint32_t)kSlice;
 }
 
 // Capacity guards between the phases of a step whose sizes are not read back: a count past its
@@ -880,10 +916,11 @@ HipErr launch_chunk_table(hipStream_t st, int32_t nsys, int32_t n, const SysDev 
 }
 
 HipErr launch_slice_sizes(hipStream_t st, int32_t n, const uint32_t *rowStart, const uint32_t *jStart,
-                          const uint32_t *extStart, uint32_t *slice_len) {
+                          const uint32_t *extStart, uint32_t *slice_len, const uint32_t *sord) {
     static_assert(kSlice == 64, "one wave per slice");
+    static_assert(kOrderWin % kSlice == 0, "order windows hold whole slices");
     if (n == 0) return hip_ok();
-    k_slice_sizes<<<nblk(n, 256), 256, 0, st>>>(n, rowStart, jStart, extStart, slice_len);
+    k_slice_sizes<<<nblk(n, 256), 256, 0, st>>>(n, rowStart, jStart, extStart, slice_len, sord);
     LAUNCH_CHECK();
     return hip_ok();
 }
@@ -893,7 +930,7 @@ __global__ void k_adjacency(int32_t n, uint32_t m, const uint32_t *__restrict__ 
                             const uint32_t *__restrict__ extStart, const uint32_t *__restrict__ extEnt, uint32_t mc,
                             const double *__restrict__ geo, const uint32_t *__restrict__ sellStart, uint64_t epad,
                             uint32_t *__restrict__ adjStart, uint32_t *__restrict__ sellK,
-                            double *__restrict__ sellW) {
+                            double *__restrict__ sellW, const uint32_t *__restrict__ spos) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i > n) return;
     // past the slot capacity every rod gets degree 0 (the host sees sellStart[last] > epad)
@@ -901,7 +938,8 @@ __global__ void k_adjacency(int32_t n, uint32_t m, const uint32_t *__restrict__ 
     adjStart[i] = fits ? rowStart[i] + jStart[i] + (extStart ? extStart[i] : 0u) : 0u;
     if (i == n || !fits) return;
     const uint32_t j0 = jStart[i], j1 = jStart[i + 1];
-    uint64_t idx = sellStart[i >> 6] + (uint32_t)(i & (kSlice - 1));
+    const uint32_t p = spos ? spos[i] : (uint32_t)i;
+    uint64_t idx = sellStart[p >> 6] + (p & (kSlice - 1));
     for (uint32_t t = j0; t < j1; ++t, idx += kSlice) {
         const uint32_t k = jlist[t];
         sellK[idx] = k | 0x80000000u;
@@ -942,9 +980,9 @@ __global__ void k_adjacency(int32_t n, uint32_t m, const uint32_t *__restrict__ 
 HipErr launch_adjacency(hipStream_t st, int32_t n, uint32_t m, const uint32_t *rowStart,
                         const uint32_t *jStart, const uint32_t *jlist, const uint32_t *extStart,
                         const uint32_t *extEnt, uint32_t mc, const double *geo, const uint32_t *sellStart,
-                        uint64_t epad, uint32_t *adjStart, uint32_t *sellK, double *sellW) {
+                        uint64_t epad, uint32_t *adjStart, uint32_t *sellK, double *sellW, const uint32_t *spos) {
     k_adjacency<<<nblk((int64_t)n + 1, 256), 256, 0, st>>>(n, m, rowStart, jStart, jlist, extStart, extEnt, mc, geo,
-                                                          sellStart, epad, adjStart, sellK, sellW);
+                                                          sellStart, epad, adjStart, sellK, sellW, spos);
     LAUNCH_CHECK();
     return hip_ok();
 }

@@ -207,18 +207,20 @@ struct AGroup {
     static constexpr int max = sizeof(T) == 8 ? 3 : 4;
 };
 
-// Per-rod data that stays fixed through a solve: system, degree and first slot in the sliced
-// adjacency.  Each thread keeps those of its first rod in registers.
+// Per-rod data that stays fixed through a solve, for the rod at slice position p: its index,
+// system, degree and first slot in the sliced adjacency.  Each thread keeps those of its first
+// position in registers.
 struct RodMeta {
     uint32_t deg, idx;
-    int sy;
+    int sy, rod;
 };
 template <typename T>
-__device__ inline RodMeta rod_meta(const SolveArgsT<T> &a, int i) {
+__device__ inline RodMeta rod_meta(const SolveArgsT<T> &a, int p) {
     RodMeta r;
-    r.sy = a.rsys[i];
-    r.deg = a.adjStart[i + 1] - a.adjStart[i];
-    r.idx = a.sellStart[i >> 6] + (uint32_t)(i & (kSlice - 1));
+    r.rod = a.sord ? (int)a.sord[p] : p;
+    r.sy = a.rsys[r.rod];
+    r.deg = a.adjStart[r.rod + 1] - a.adjStart[r.rod];
+    r.idx = a.sellStart[p >> 6] + (uint32_t)(p & (kSlice - 1));
     return r;
 }
 
@@ -241,8 +243,8 @@ __device__ inline int work_block(int xcd_map) {
 }
 
 // phase A: per rod, F = sum over incident constraints (ascending k) of v_k * w_e; U = M F
-// (rods of systems that have stopped are skipped).  Lane i % 64 of a wave walks rod i's entries in
-// the sliced adjacency, so each step of the 64 lanes reads consecutive words.
+// (rods of systems that have stopped are skipped).  Lane p % 64 of a wave walks the entries of the
+// rod at slice position p, so each step of the 64 lanes reads consecutive words.
 template <int MODE, typename T, int kAG>
 __device__ inline void phase_A(const SolveArgsT<T> &a, const double *s_alpha, const int *s_state,
                                const RodMeta &rm0) {
@@ -250,8 +252,9 @@ __device__ inline void phase_A(const SolveArgsT<T> &a, const double *s_alpha, co
     const int stride = gridDim.x * blockDim.x;
     const int i0 = work_block(a.xcd_map) * blockDim.x + threadIdx.x;
     const size_t ep = a.epad;
-    for (int i = i0; i < a.n; i += stride) {
-        const RodMeta rm = i == i0 ? rm0 : rod_meta(a, i);
+    for (int p = i0; p < a.n; p += stride) {
+        const RodMeta rm = p == i0 ? rm0 : rod_meta(a, p);
+        const size_t i = (size_t)rm.rod;
         if (s_state[rm.sy] != S_ACTIVE) continue;
         const T alpha = (T)s_alpha[rm.sy];
         T f0 = T(0), f1 = T(0), f2 = T(0), f3 = T(0), f4 = T(0), f5 = T(0);
@@ -529,7 +532,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
         s_cm[r][g] = chunk_meta(a, work_block(a.xcd_map) * kGrpB + g + r * gridDim.x * kGrpB, nch);
     }
     const int i0 = work_block(a.xcd_map) * blockDim.x + threadIdx.x;
-    const RodMeta rm0 = i0 < a.n ? rod_meta(a, i0) : RodMeta{0u, 0u, 0};
+    const RodMeta rm0 = i0 < a.n ? rod_meta(a, i0) : RodMeta{0u, 0u, 0, 0};
     for (int q = threadIdx.x; q < ns; q += blockDim.x) {
         s_alpha[q] = 0.0;
         s_state[q] = S_ACTIVE;
@@ -730,13 +733,14 @@ __global__ void k_writeback(int32_t n, const int32_t *__restrict__ rsys, const S
                             const uint32_t *__restrict__ sellK, const double *__restrict__ sellW, uint64_t epad,
                             const double2 *__restrict__ xg,
                             const double *__restrict__ mob, const double *__restrict__ unc,
-                            double *__restrict__ F, double *__restrict__ U) {
+                            double *__restrict__ F, double *__restrict__ U, const uint32_t *__restrict__ spos) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int mono = sys[rsys[i]].mono;
     double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const uint32_t deg = adjStart[i + 1] - adjStart[i];
-    size_t idx = (size_t)sellStart[i >> 6] + (uint32_t)(i & (kSlice - 1));
+    const uint32_t p = spos ? spos[i] : (uint32_t)i;
+    size_t idx = (size_t)sellStart[p >> 6] + (p & (kSlice - 1));
     for (uint32_t j = 0; j < deg; ++j, idx += kSlice) {
         const uint32_t k = sellK[idx] & 0x7fffffffu;
         const double v = xg[k].x;
@@ -775,10 +779,10 @@ __global__ void k_copy_gamma(const double2 *__restrict__ xg, double *__restrict_
 HipErr launch_writeback(hipStream_t st, int32_t n, const int32_t *rsys, const SysDev *sys, const uint32_t *adjStart,
                         const uint32_t *sellStart, const uint32_t *sellK, const double *sellW, uint64_t epad,
                         const double2 *xg, const double *mob, const double *unc, double *F, double *U,
-                        double *gamma, uint32_t m, const uint32_t *md) {
+                        double *gamma, uint32_t m, const uint32_t *md, const uint32_t *spos) {
     if (n > 0) {
         k_writeback<<<nblk(n, 256), 256, 0, st>>>(n, rsys, sys, adjStart, sellStart, sellK, sellW, epad, xg, mob, unc,
-                                                  F, U);
+                                                  F, U, spos);
         LAUNCH_CHECK();
     }
     if (m > 0) {
