@@ -64,6 +64,7 @@ struct Grid {
 // One independent rod system of a context (a context holds 1 or many): its box and cell grid.
 struct SysDev {
     double box_lo[3], len[3], h[3];
+    double infl;        // AABB / tube inflation: max collision radius + colBuf / 2 + margin
     int32_t pbc[3], nc[3];
     int32_t mono;
     uint32_t cell_off;  // first global cell id of this system

@@ -326,12 +326,13 @@ alens_status alens_set_systems(alens_ctx *ctx, int32_t nsys, const int32_t *rod_
             if (!(bx.box_hi[k] - bx.box_lo[k] >= 2.0 * (lmax + 2.0 * rmax + cf.col_buf)))
                 return fail(ctx, ALENS_E_BOX, "periodic box shorter than twice the largest interaction range");
         }
-        // cell grid: edge = configured size, or half the mean collision length (at least twice the
-        // largest AABB inflation); capped at 8 cells per rod
+        // cell grid: edge = configured size, or a quarter of the mean collision length (at least twice
+        // the largest inflation); capped at 8 cells per rod.  Rods enter only the cells their inflated
+        // segment meets, so small cells keep few rods per cell.
         const int32_t ns = r1 - r0;
         double h = cf.cell_size;
         if (!(h > 0.0)) {
-            h = ns ? 0.5 * lsum / ns : 1.0;
+            h = ns ? 0.25 * lsum / ns : 1.0;
             if (h < 2.0 * imax) h = 2.0 * imax;
             if (!(h > 0.0)) h = 1.0;
         }
@@ -353,6 +354,7 @@ alens_status alens_set_systems(alens_ctx *ctx, int32_t nsys, const int32_t *rod_
             h *= 1.25;
         }
         g.mono = bx.monolayer ? 1 : 0;
+        g.infl = imax;
         g.cell_off = (uint32_t)ncell;
         g.rod_off = r0;
         g.rod_end = r1;

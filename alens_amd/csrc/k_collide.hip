@@ -113,14 +113,66 @@ HipErr launch_mob_unc(hipStream_t st, const RodsDev &r, const double *length, co
 // AABB (inflated by collision radius + colBuf/2 + margin) and its cell range per axis.  Periodic
 // ranges are unwrapped integers; non-periodic ranges are clamped to the grid.
 // ------------------------------------------------------------------------------------------------
+__device__ inline int pmod_i(int a, int n) {
+    const int m = a % n;
+    return m < 0 ? m + n : m;
+}
+
+// Parameter interval [s0, s1] (within [-hh, hh]) where the segment x(s) = c + s t lies inside the
+// slab of cell u on axis k grown by rho; a non-periodic axis's first and last cells reach to -inf /
+// +inf (positions outside the box are clamped into them).  Returns false when empty.
+__device__ inline bool slab_interval(const SysDev &S, int k, double ck, double tk, double hh, int u, double rho,
+                                     double &s0, double &s1) {
+    const double lo = (!S.pbc[k] && u <= 0) ? -INFINITY : S.box_lo[k] + (double)u * S.h[k] - rho;
+    const double hi = (!S.pbc[k] && u >= S.nc[k] - 1) ? INFINITY : S.box_lo[k] + (double)(u + 1) * S.h[k] + rho;
+    if (tk == 0.0 || hh == 0.0) return ck >= lo && ck <= hi;
+    double sa = (lo - ck) / tk, sb = (hi - ck) / tk;
+    if (sa > sb) {
+        const double w = sa;
+        sa = sb;
+        sb = w;
+    }
+    s0 = sa > s0 ? sa : s0;
+    s1 = sb < s1 ? sb : s1;
+    return s0 <= s1;
+}
+
+// Calls f(u) for the cells (unwrapped indices, inside the box range a..b) that the segment grown by
+// rho meets: for each (y, z) row, the parameter interval inside the row's grown slabs gives the
+// segment's x extent there, and the row's cells are those whose grown x range meets it.  A superset
+// of the cells within rho of the segment (rho carries a 1e-6 um margin, far above rounding).
+template <class F>
+__device__ inline void for_tube_cells(const SysDev &S, const double c[3], const double t[3], double hh,
+                                      const int a[3], const int b[3], double rho, F f) {
+    int u[3];
+    for (u[2] = a[2]; u[2] <= b[2]; ++u[2]) {
+        double z0 = -hh, z1 = hh;
+        if (!slab_interval(S, 2, c[2], t[2], hh, u[2], rho, z0, z1)) continue;
+        for (u[1] = a[1]; u[1] <= b[1]; ++u[1]) {
+            double s0 = z0, s1 = z1;
+            if (!slab_interval(S, 1, c[1], t[1], hh, u[1], rho, s0, s1)) continue;
+            const double xa = c[0] + s0 * t[0], xb = c[0] + s1 * t[0];
+            const double xlo = xa < xb ? xa : xb, xhi = xa < xb ? xb : xa;
+            int ulo = (int)floor((xlo - rho - S.box_lo[0]) / S.h[0]);
+            int uhi = (int)floor((xhi + rho - S.box_lo[0]) / S.h[0]);
+            ulo = ulo < a[0] ? a[0] : ulo;
+            uhi = uhi > b[0] ? b[0] : uhi;
+            for (u[0] = ulo; u[0] <= uhi; ++u[0]) f(u);
+        }
+    }
+}
+
 __global__ void k_aabb(RodsDev r, CfgDev c, int32_t *__restrict__ clo, int32_t *__restrict__ chi,
                        double *__restrict__ blo, double *__restrict__ bhi, uint32_t *__restrict__ ecnt) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= r.n) return;
     const SysDev &S = r.sys[r.rsys[i]];
     const double hh = r.hcol[i];
-    const double infl = r.rcol[i] + 0.5 * c.col_buf + kAabbMargin;
-    uint32_t cnt = 1;
+    // one inflation per system (the largest radius): a contact's closest-point midpoint then lies
+    // inside both rods' boxes and tubes, whatever their radii
+    const double infl = S.infl;
+    (void)c;
+    int a3[3], b3[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const double hw = hh * fabs(r.dir[3 * i + k]);
@@ -136,8 +188,14 @@ __global__ void k_aabb(RodsDev r, CfgDev c, int32_t *__restrict__ clo, int32_t *
         chi[3 * i + k] = b;
         blo[3 * i + k] = lo;
         bhi[3 * i + k] = hi;
-        cnt *= (uint32_t)(b - a + 1);
+        a3[k] = a;
+        b3[k] = b;
     }
+    // entries: the cells of the box range that the inflated segment meets
+    const double ci[3] = {r.center[3 * i], r.center[3 * i + 1], r.center[3 * i + 2]};
+    const double ti[3] = {r.dir[3 * i], r.dir[3 * i + 1], r.dir[3 * i + 2]};
+    uint32_t cnt = 0;
+    for_tube_cells(S, ci, ti, hh, a3, b3, infl, [&](const int *) { ++cnt; });
     ecnt[i] = cnt;
 }
 
@@ -266,11 +324,6 @@ HipErr exclusive_scan_u32(hipStream_t st, const uint32_t *in, uint32_t *out, int
 // cell entries: rod i writes (wrapped cell id, i) for every cell of its range, rod-major, so the
 // stable sort below leaves each cell's rods in ascending index order.
 // ------------------------------------------------------------------------------------------------
-__device__ inline int pmod_i(int a, int n) {
-    int r = a % n;
-    return r < 0 ? r + n : r;
-}
-
 __global__ void k_fill_entries(RodsDev r, const int32_t *__restrict__ clo, const int32_t *__restrict__ chi,
                                const uint32_t *__restrict__ eoff, uint32_t *__restrict__ keys,
                                uint32_t *__restrict__ vals, uint32_t ecap) {
@@ -278,20 +331,22 @@ __global__ void k_fill_entries(RodsDev r, const int32_t *__restrict__ clo, const
     if (i >= r.n || eoff[i + 1] > ecap) return;  // past the capacity: the host sees eoff[n] > ecap
     const SysDev &g = r.sys[r.rsys[i]];
     uint32_t w = eoff[i];
-    const int a0 = clo[3 * i], a1 = clo[3 * i + 1], a2 = clo[3 * i + 2];
-    const int b0 = chi[3 * i], b1 = chi[3 * i + 1], b2 = chi[3 * i + 2];
-    for (int z = a2; z <= b2; ++z) {
-        const uint32_t cz = (uint32_t)pmod_i(z, g.nc[2]);
-        for (int y = a1; y <= b1; ++y) {
-            const uint32_t cy = (uint32_t)pmod_i(y, g.nc[1]);
-            for (int x = a0; x <= b0; ++x) {
-                const uint32_t cx = (uint32_t)pmod_i(x, g.nc[0]);
-                keys[w] = g.cell_off + (cz * (uint32_t)g.nc[1] + cy) * (uint32_t)g.nc[0] + cx;
-                vals[w] = (uint32_t)i;
-                ++w;
-            }
-        }
-    }
+    const uint32_t wend = eoff[i + 1];
+    const double ci[3] = {r.center[3 * i], r.center[3 * i + 1], r.center[3 * i + 2]};
+    const double ti[3] = {r.dir[3 * i], r.dir[3 * i + 1], r.dir[3 * i + 2]};
+    const double hh = r.hcol[i];
+    const int a3[3] = {clo[3 * i], clo[3 * i + 1], clo[3 * i + 2]};
+    const int b3[3] = {chi[3 * i], chi[3 * i + 1], chi[3 * i + 2]};
+    // the same cells, in the same order, as k_aabb counted (never more than its count)
+    for_tube_cells(g, ci, ti, hh, a3, b3, g.infl, [&](const int *u) {
+        if (w >= wend) return;
+        const uint32_t cx = (uint32_t)pmod_i(u[0], g.nc[0]);
+        const uint32_t cy = (uint32_t)pmod_i(u[1], g.nc[1]);
+        const uint32_t cz = (uint32_t)pmod_i(u[2], g.nc[2]);
+        keys[w] = g.cell_off + (cz * (uint32_t)g.nc[1] + cy) * (uint32_t)g.nc[0] + cx;
+        vals[w] = (uint32_t)i;
+        ++w;
+    });
 }
 
 HipErr launch_fill_entries(hipStream_t st, const RodsDev &r, const int32_t *clo, const int32_t *chi,
@@ -441,10 +496,13 @@ HipErr launch_cell_ranges(hipStream_t st, const uint32_t *keys, int64_t E, uint3
 }
 
 // ------------------------------------------------------------------------------------------------
-// Pair sweep.  For entries a < b of one cell (rods i < j) the pair is examined there only if, on every
-// axis, the cell is the first cell of i's or of j's range (unwrapped), which visits each
-// (pair, image) once; the image shift of j follows from the unwrapped cell indices.  AABB reject, then
-// the FP64 segment distance and delta0 < colBuf.  Per-rod counts of both sides feed the ordering.
+// Pair sweep.  For entries a < b of one cell (rods i < j): the image shift of j follows from the
+// unwrapped cell indices (each rod's box range is shorter than the periodic length), then AABB
+// reject, the FP64 segment distance and delta0 < colBuf.  A contact is emitted only in the cell that
+// holds the midpoint of its two closest points: that point lies within both rods' inflated tubes
+// (half the contact distance is at most the inflation), so both rods have an entry there, and every
+// shared cell computes the same midpoint -- each (pair, image) is emitted exactly once.  Per-rod
+// counts of both sides feed the ordering.
 // ------------------------------------------------------------------------------------------------
 // One thread per sorted entry e (rod i in cell c): pairs with the later entries of the same cell.
 // Lanes of a wave run the same trip count (the wave maximum) so emission stays wave-aggregated.
@@ -477,7 +535,7 @@ __global__ __launch_bounds__(kSweepT) void k_sweep_entries(RodsDev r, CfgDev c, 
     double bli[3] = {0, 0, 0}, bhi_i[3] = {0, 0, 0}, ci[3] = {0, 0, 0}, ti[3] = {0, 0, 0};
     double hi_ = 0.0, ri = 0.0;
     int pbc[3] = {0, 0, 0}, nc[3] = {1, 1, 1};
-    double len[3] = {0, 0, 0};
+    double len[3] = {0, 0, 0}, blo0[3] = {0, 0, 0}, hcell[3] = {1, 1, 1};
     if (e < E) {
         const uint32_t cell = keys[e];
         s1 = cend[cell];
@@ -489,6 +547,8 @@ __global__ __launch_bounds__(kSweepT) void k_sweep_entries(RodsDev r, CfgDev c, 
             pbc[k] = g.pbc[k];
             nc[k] = g.nc[k];
             len[k] = g.len[k];
+            blo0[k] = g.box_lo[k];
+            hcell[k] = g.h[k];
         }
         const uint32_t lc = cell - g.cell_off;
         cc[0] = (int)(lc % (uint32_t)nc[0]);
@@ -525,7 +585,6 @@ __global__ __launch_bounds__(kSweepT) void k_sweep_entries(RodsDev r, CfgDev c, 
             for (int k = 0; k < 3; ++k) {
                 const int clj = clo[3 * j + k];
                 const int uj = pbc[k] ? clj + pmod_i(cc[k] - clj, nc[k]) : cc[k];
-                ok = ok && (ui[k] == cli[k] || uj == clj);
                 m[k] = pbc[k] ? (ui[k] - uj) / nc[k] : 0;
                 const double sh = pbc[k] ? (double)m[k] * len[k] : 0.0;
                 ok = ok && (blo[3 * j + k] + sh <= bhi_i[k]) && (bli[k] <= bhi[3 * j + k] + sh);
@@ -539,6 +598,15 @@ __global__ __launch_bounds__(kSweepT) void k_sweep_entries(RodsDev r, CfgDev c, 
                                          r.dir[3 * j], r.dir[3 * j + 1], r.dir[3 * j + 2], r.hcol[j]);
                 const double d0 = (sg.dist - ri) - r.rcol[j];
                 pass = d0 < c.col_buf;
+                // the cell of the closest-point midpoint, in i's unwrapped frame
+                const double D[3] = {sg.D0, sg.D1, sg.D2};
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const double mk = (ci[k] + sg.s * ti[k]) - 0.5 * D[k];
+                    int um = (int)floor((mk - blo0[k]) / hcell[k]);
+                    if (!pbc[k]) um = um < 0 ? 0 : (um >= nc[k] ? nc[k] - 1 : um);
+                    pass = pass && um == ui[k];
+                }
             }
         }
         const uint64_t mask = __ballot(pass);

@@ -49,6 +49,8 @@ def main():
                     choices=["monolayer_slab", "lognormal_3d", "small_3d", "replicas"])
     ap.add_argument("--precision", type=int, default=0, help="1: FP32 BBPGD + FP64 refinement for the timed pass")
     ap.add_argument("--n", type=int, default=0, help="rod count (default: the workload's own)")
+    ap.add_argument("--cell-size", type=float, default=0.0,
+                    help="cell-list edge in um (default: the library's choice, a quarter of the mean rod length)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-form", action="store_true", help="skip the chosen-form (FP32 + warm start) pass")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -90,6 +92,8 @@ def main():
     gen = {"monolayer_slab": systems.monolayer_slab, "lognormal_3d": systems.lognormal_3d,
            "small_3d": systems.small_3d}[args.workload]
     rods, cfg = gen(n=args.n) if args.n else gen()
+    if args.cell_size > 0:
+        cfg = dict(cfg, cell_size=args.cell_size)
     n = len(rods["length"])
     ctx = capi.Context(**dict(cfg, precision=args.precision))
     ctx.set_rods(rods)
