@@ -391,6 +391,28 @@ def test_spheres_and_immovable_rods_bitexact(pbc):
     assert np.all(u[imm] == 0.0), "immovable rods must not move"
 
 
+@pytest.mark.parametrize("pbc", [(1, 1, 1), (0, 0, 0), (1, 0, 1)])
+def test_mixed_radii_pair_set_bitexact(pbc):
+    """Radii from 0.01 to 0.06 um (spheres the largest): a contact's closest-point midpoint can lie
+    farther than the smaller rod's own radius + colBuf/2 from it, so the cell list must grow every
+    rod's tube by the system's largest radius.  Rods also leave the box on the open axes."""
+    rng = np.random.default_rng(31)
+    n = 1200
+    rods = _mixed_rods(n, 3.2, seed=29, immovable_frac=0.0)
+    rods["radius"] = rng.uniform(0.01, 0.06, n)
+    if not pbc[2]:
+        rods["center"][:, 2] = rng.uniform(-0.3, 3.5, n)  # past both faces of the open z axis
+    cfg = dict(systems.COMMON, box_lo=(0, 0, 0), box_hi=(3.2, 3.2, 3.2), pbc=pbc, monolayer=0)
+    ctx, g = gpu_step(rods, cfg)
+    o = O.step(rods, O.cfg_from_dict(cfg))
+    assert o["I"].size > 150
+    compare_full(rods, cfg, g, o)
+    # the oracle's own cell search against brute force on the same input
+    oc = O.cfg_from_dict(cfg)
+    I, J, img = O.pairs(rods, oc, "brute")
+    assert np.array_equal(I, o["I"]) and np.array_equal(J, o["J"]) and np.array_equal(img, o["img"])
+
+
 def test_external_force_and_monolayer_projection_bitexact():
     """A non-null F_ext (6 per rod) enters U_nc = M F_ext + U_B, and monolayer = 1 projects it."""
     rods, cfg = systems.small_3d(n=800, box=3.6, seed=23)
