@@ -853,7 +853,7 @@ __device__ inline uint32_t rod_degree(const uint32_t *rowStart, const uint32_t *
 __global__ __launch_bounds__(kOrderWin) void k_slot_order(int32_t n, const uint32_t *__restrict__ rowStart,
                                                           const uint32_t *__restrict__ jStart,
                                                           const uint32_t *__restrict__ extStart,
-                                                          uint32_t *__restrict__ spos, uint32_t *__restrict__ sord) {
+                                                          uint32_t *__restrict__ sord) {
     __shared__ int64_t s_deg[kOrderWin];
     const int32_t base = blockIdx.x * kOrderWin;
     const int t = threadIdx.x;
@@ -867,14 +867,13 @@ __global__ __launch_bounds__(kOrderWin) void k_slot_order(int32_t n, const uint3
         const int64_t e = s_deg[j];
         rank += (e > d || (e == d && j < t)) ? 1 : 0;
     }
-    spos[i] = (uint32_t)(base + rank);
     sord[base + rank] = (uint32_t)i;
 }
 
 HipErr launch_slot_order(hipStream_t st, int32_t n, const uint32_t *rowStart, const uint32_t *jStart,
-                         const uint32_t *extStart, uint32_t *spos, uint32_t *sord) {
+                         const uint32_t *extStart, uint32_t *sord) {
     if (n == 0) return hip_ok();
-    k_slot_order<<<nblk(n, kOrderWin), kOrderWin, 0, st>>>(n, rowStart, jStart, extStart, spos, sord);
+    k_slot_order<<<nblk(n, kOrderWin), kOrderWin, 0, st>>>(n, rowStart, jStart, extStart, sord);
     LAUNCH_CHECK();
     return hip_ok();
 }
@@ -998,15 +997,18 @@ __global__ void k_adjacency(int32_t n, uint32_t m, const uint32_t *__restrict__ 
                             const uint32_t *__restrict__ extStart, const uint32_t *__restrict__ extEnt, uint32_t mc,
                             const double *__restrict__ geo, const uint32_t *__restrict__ sellStart, uint64_t epad,
                             uint32_t *__restrict__ adjStart, uint32_t *__restrict__ sellK,
-                            double *__restrict__ sellW, const uint32_t *__restrict__ spos) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > n) return;
+                            double *__restrict__ sellW, const uint32_t *__restrict__ sord) {
+    const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > n) return;
     // past the slot capacity every rod gets degree 0 (the host sees sellStart[last] > epad)
     const bool fits = sellStart[(n + kSlice - 1) / kSlice] <= epad;
-    adjStart[i] = fits ? rowStart[i] + jStart[i] + (extStart ? extStart[i] : 0u) : 0u;
-    if (i == n || !fits) return;
+    adjStart[t] = fits ? rowStart[t] + jStart[t] + (extStart ? extStart[t] : 0u) : 0u;
+    if (t == n || !fits) return;
+    // thread t fills slice position t (consecutive lanes write consecutive slots) with the entries of
+    // the rod placed there
+    const uint32_t p = (uint32_t)t;
+    const int32_t i = sord ? (int32_t)sord[p] : t;
     const uint32_t j0 = jStart[i], j1 = jStart[i + 1];
-    const uint32_t p = spos ? spos[i] : (uint32_t)i;
     uint64_t idx = sellStart[p >> 6] + (p & (kSlice - 1));
     for (uint32_t t = j0; t < j1; ++t, idx += kSlice) {
         const uint32_t k = jlist[t];
@@ -1048,9 +1050,9 @@ __global__ void k_adjacency(int32_t n, uint32_t m, const uint32_t *__restrict__ 
 HipErr launch_adjacency(hipStream_t st, int32_t n, uint32_t m, const uint32_t *rowStart,
                         const uint32_t *jStart, const uint32_t *jlist, const uint32_t *extStart,
                         const uint32_t *extEnt, uint32_t mc, const double *geo, const uint32_t *sellStart,
-                        uint64_t epad, uint32_t *adjStart, uint32_t *sellK, double *sellW, const uint32_t *spos) {
+                        uint64_t epad, uint32_t *adjStart, uint32_t *sellK, double *sellW, const uint32_t *sord) {
     k_adjacency<<<nblk((int64_t)n + 1, 256), 256, 0, st>>>(n, m, rowStart, jStart, jlist, extStart, extEnt, mc, geo,
-                                                          sellStart, epad, adjStart, sellK, sellW, spos);
+                                                          sellStart, epad, adjStart, sellK, sellW, sord);
     LAUNCH_CHECK();
     return hip_ok();
 }

@@ -46,23 +46,27 @@ def train(X, gamma, epochs=12, batch=4096, lr=2e-3, seed=1):
     return P, norm, losses
 
 
-def evaluate(kind, n, steps, P, norm, seed=99, warmup=2, precision=0):
-    """Same rod states fed to three contexts (cold / previous gamma / network) at every step;
-    returns the per-step BBPGD iteration counts of each."""
+def evaluate(kind, n, steps, P, norm, seed=99, warmup=2, precision=0, forms=None):
+    """Same rod states fed to several contexts at every step (a plain FP64 driver moves the rods);
+    returns each context's per-step BBPGD iterations (FP32 + FP64) and its warm-start fallbacks.
+    forms: name -> (warm_start, precision); default cold / previous gamma / network at `precision`."""
+    if forms is None:
+        forms = {"cold": (0, precision), "prev": (1, precision), "mlp": (2, precision)}
     rods, cfg = systems.scaled(kind, n, seed=seed) if kind != "small_3d" else systems.small_3d(seed=seed)
     driver = capi.Context(**cfg)
     driver.set_rods(rods)
     perm = driver.sort_rods()
     rods = {k: (v[perm] if isinstance(v, np.ndarray) else v) for k, v in rods.items()}
     ctxs = {}
-    for name, ws in (("cold", 0), ("prev", 1), ("mlp", 2)):
-        c = capi.Context(**dict(cfg, warm_start=ws, precision=precision))
+    for name, (ws, prec) in forms.items():
+        c = capi.Context(**dict(cfg, warm_start=ws, precision=prec))
         c.set_rods(rods)
         if ws == 2:
             c.mlp_set(P, norm)
         ctxs[name] = c
     out = {k: [] for k in ctxs}
-    out["fallback_mlp"] = 0
+    for name in ctxs:
+        out["fallback_" + name] = 0
     for s in range(warmup + steps):
         c, t = driver.get_rods()
         state = dict(rods, center=c, dir=t)
@@ -72,8 +76,7 @@ def evaluate(kind, n, steps, P, norm, seed=99, warmup=2, precision=0):
             r = ctx.step()
             if s >= warmup:
                 out[name].append(r["iters"] + r["iters_fp32"])
-                if name == "mlp":
-                    out["fallback_mlp"] += r["fallback_used"]
+                out["fallback_" + name] += r["fallback_used"]
         driver.set_step(s)
         driver.step()
         driver.euler()

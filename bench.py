@@ -192,10 +192,14 @@ def main():
             out["roofline"]["traffic_source"] = ("profiles/k_bbpgd_traffic.json: rocprofv3 FETCH_SIZE + WRITE_SIZE "
                                                  f"per matvec ({tr['traffic_over_algorithmic']:.2f} x algorithmic)")
 
-    # chosen form on the same workload: FP32 iterations + FP64 residual/refinement, warm start from
-    # the previous step's impulses
+    # chosen form on the same workload: FP32 iterations + FP64 residual/refinement, started from the
+    # learned network (the shipped weights); plain BBPGD finishes every solve
     if not args.no_form:
-        fctx = capi.Context(**dict(cfg, precision=1, warm_start=1))
+        from alens_amd import warmstart as W
+
+        P, norm = W.load()
+        fctx = capi.Context(**dict(cfg, precision=1, warm_start=2))
+        fctx.mlp_set(P, norm)
         fctx.set_rods(rods)
         fctx.set_graph(bool(args.graph))
         fctx.sort_rods()
@@ -217,7 +221,7 @@ def main():
         fit32 = np.array([r["iters_fp32"] for r in freps])
         fit64 = np.array([r["iters"] for r in freps])
         out["chosen_form"] = {
-            "form": "FP32 BBPGD + FP64 residual/refinement, warm start from previous-step gamma",
+            "form": "FP32 BBPGD + FP64 residual/refinement, warm start from the learned network",
             "value": n * K / (f1 - f0),
             "unit": "rod-steps/s",
             "ms_per_step": 1e3 * (f1 - f0) / K,
@@ -226,6 +230,20 @@ def main():
             "fp64_refine_iters_per_step": float(fit64.mean()),
             "fallbacks": int(sum(r["fallback_used"] for r in freps)),
             "iteration_ratio_plain_over_form": float(iters.mean() / max((fit32 + fit64).mean(), 1e-9)),
+        }
+        # iterations on identical states (a plain FP64 driver moves the rods; each form keeps its own
+        # previous-step impulses): plain FP64 against the FP32 form with each warm start
+        ev = W.evaluate(args.workload, n, 4, P, norm, seed=cfg["seed"], warmup=8,
+                        forms={"plain_fp64": (0, 0), "fp32_cold": (0, 1), "fp32_prev_gamma": (1, 1),
+                               "fp32_network": (2, 1), "fp64_network": (2, 0)})
+        pm = float(ev["plain_fp64"].mean())
+        out["chosen_form"]["same_state_iterations"] = {
+            "states": "4 steps after 8 warm-up from the workload's seeded start",
+            **{k: float(ev[k].mean()) for k in ("plain_fp64", "fp32_cold", "fp32_prev_gamma", "fp32_network",
+                                                "fp64_network")},
+            "fallbacks_fp32_network": int(ev["fallback_fp32_network"]),
+            "ratio_plain_over_fp32_network": pm / max(float(ev["fp32_network"].mean()), 1e-9),
+            "target_ratio": 2.0,
         }
         # accuracy of the form, stated per result: from the post-benchmark state, the FP32 form (cold) and
         # the plain FP64 solve (both to residual tol) against an FP64 solve to 1e-11
@@ -375,11 +393,10 @@ def stream_copy_gbs(nbytes=1 << 30, reps=10):
     return gbs
 
 
-def bench_replicas(args, capi, systems, copy_gbs=None):
-    """64 independent replicas (rod counts log-uniform in [2k, 50k], 3-D boxes and slabs) stepped
-    together: one collection pass and one persistent solver launch per step for all of them."""
-    parts, cfg = systems.replicas()
-    ctx = capi.Context(**dict(cfg, precision=args.precision, warm_start=1))
+def _run_replicas(args, capi, parts, cfg, precision, warm_start, P=None, norm=None):
+    ctx = capi.Context(**dict(cfg, precision=precision, warm_start=warm_start))
+    if warm_start == 2:
+        ctx.mlp_set(P, norm)
     offs = ctx.set_systems(parts)
     ctx.set_graph(bool(args.graph))
     ctx.sort_rods()
@@ -387,14 +404,13 @@ def bench_replicas(args, capi, systems, copy_gbs=None):
     for w in range(args.warmup):
         ctx.step()
         ctx.euler()
+    tune = None
     if not args.no_autotune and args.warmup > 0:
         ctx.step()
         tune = ctx.autotune()
         ctx.euler()
         ctx.step()
         ctx.euler()
-    else:
-        tune = None
     reps, sreps = [], []
     t0 = time.perf_counter()
     for s in range(args.steps):
@@ -402,25 +418,53 @@ def bench_replicas(args, capi, systems, copy_gbs=None):
         sreps.append(ctx.system_reports())
         ctx.euler()
     el = time.perf_counter() - t0
-    K = args.steps
     it = np.array([[r["iters"] + r["iters_fp32"] for r in sr] for sr in sreps])  # steps x replicas
     hist, edges = np.histogram(it.mean(0), bins=[0, 25, 50, 100, 200, 400, 800, 1600, 1e9])
+    res = dict(n=n, elapsed=el, reps=reps, tune=tune, graph=ctx.graph_info() if args.graph else None,
+               iters_mean_per_replica=float(it.mean()),
+               hist={"edges": [float(e) for e in edges[:-1]], "counts": hist.tolist()},
+               fallbacks=int(sum(r["fallback_used"] for sr in sreps for r in sr)))
+    ctx.close()
+    return res
+
+
+def bench_replicas(args, capi, systems, copy_gbs=None):
+    """64 independent replicas (rod counts log-uniform in [2k, 50k], 3-D boxes and slabs) stepped
+    together: one collection pass and one persistent solver launch per step for all of them (one
+    graph replay per step in graph mode).  Plain FP64 from a cold start gives `value`; the chosen
+    form (FP32 + FP64 refinement, learned warm start) runs the same workload beside it."""
+    parts, cfg = systems.replicas()
+    K = args.steps
+    a = _run_replicas(args, capi, parts, cfg, args.precision, 0)
+    n, reps = a["n"], a["reps"]
     out = {
         "metric": "rod-steps/sec at BBPGD residual 1e-5 (64 replicas in one launch per step)",
-        "value": n * K / el, "unit": "rod-steps/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
-        "ms_per_step": 1e3 * el / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "value": n * K / a["elapsed"], "unit": "rod-steps/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": 1e3 * a["elapsed"] / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64" if args.precision else "f64", "data": "synthetic",
-        "config": {"workload": f"replicas: {len(parts)} systems, {n} rods, warm start from previous gamma",
-                   "rods": n, "replicas": len(parts)},
+        "config": {"workload": f"replicas: {len(parts)} systems, {n} rods, cold start", "rods": n,
+                   "replicas": len(parts)},
         "constraints_per_step": float(np.mean([r["n_con"] for r in reps])),
-        "solver_iters_max_per_step": float(np.mean([r["iters"] for r in reps])),
-        "replica_iters_histogram": {"edges": [float(e) for e in edges[:-1]], "counts": hist.tolist()},
+        "solver_iters_max_per_step": float(np.mean([r["iters"] + r["iters_fp32"] for r in reps])),
+        "replica_iters_mean": a["iters_mean_per_replica"],
+        "replica_iters_histogram": a["hist"],
         "t_solve_ms_mean": float(np.mean([r["t_solve_ms"] for r in reps])),
         "t_collect_ms_mean": float(np.mean([r["t_collect_ms"] for r in reps])),
-        "solver_schedule": tune,
-        "graph": ctx.graph_info() if args.graph else None,
+        "solver_schedule": a["tune"],
+        "graph": a["graph"],
         "stream_copy_gbs": copy_gbs,
     }
+    if not args.no_form:
+        from alens_amd import warmstart as W
+
+        P, norm = W.load()
+        b = _run_replicas(args, capi, parts, cfg, 1, 2, P, norm)
+        out["chosen_form"] = {
+            "form": "FP32 BBPGD + FP64 residual/refinement, warm start from the learned network",
+            "value": n * K / b["elapsed"], "unit": "rod-steps/s", "ms_per_step": 1e3 * b["elapsed"] / K,
+            "replica_iters_mean": b["iters_mean_per_replica"], "replica_iters_histogram": b["hist"],
+            "fallbacks": b["fallbacks"], "solver_schedule": b["tune"], "graph": b["graph"],
+        }
     print(json.dumps(out), flush=True)
 
 
