@@ -331,23 +331,27 @@ __device__ inline T du_row(const SolveArgsT<T> &a, uint32_t k) {
 
 enum : int { B_INIT = 0, B_ALPHA0 = 1, B_ITER = 2 };
 
-// Phase-B reduction groups.  A 512-element chunk is reduced by kGL lanes; in the fixed order element
-// e belongs to virtual lane e % 256 (pass e / 256), and lane l of a group holds the virtual lanes
-// l + j * kGL (j < kVPT).  Each physical wave folds its kVPT virtual waves separately by halving
-// (lane l += lane l + h, h = 32 .. 1) and the four virtual-wave totals are added left to right, so
-// the result is the one a 256-lane group gives.
-constexpr int kGL = 128;
-constexpr int kVPT = kRedThreads / kGL;
-constexpr int kGrpB = kBlk / kGL;
+// Phase-B reduction groups.  A 512-element chunk is reduced by GL lanes (GL = 256, or 128 when that
+// saves a round of chunks); in the fixed order element e belongs to virtual lane e % 256 (pass
+// e / 256), and lane l of a group holds the virtual lanes l + j * GL (j < 256 / GL).  Each physical
+// wave folds its virtual waves separately by halving (lane l += lane l + h, h = 32 .. 1) and the four
+// virtual-wave totals are added left to right, so the result is the one a 256-lane group gives.
+constexpr int kGrpB = kBlk / 128;  // most groups per block (GL = 128)
+template <int GL>
+struct GroupShape {
+    static constexpr int vpt = kRedThreads / GL;  // virtual lanes per lane
+    static constexpr int per_block = kBlk / GL;   // groups per block
+};
 
-template <int NQ, bool MAX>
-__device__ inline void vgroup_fold(const double (&v)[NQ][kVPT], double *sh /* kGrpB * NQ * 4 */,
+template <int NQ, bool MAX, int GL>
+__device__ inline void vgroup_fold(const double (&v)[NQ][GroupShape<GL>::vpt], double *sh /* groups * NQ * 4 */,
                                    double (&tot)[NQ]) {
-    const int l = threadIdx.x % kGL, g = threadIdx.x / kGL, lane = l & 63, pw = l >> 6;
+    constexpr int VPT = GroupShape<GL>::vpt;
+    const int l = threadIdx.x % GL, g = threadIdx.x / GL, lane = l & 63, pw = l >> 6;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
 #pragma unroll
-        for (int j = 0; j < kVPT; ++j) {
+        for (int j = 0; j < VPT; ++j) {
             double x = v[q][j];
 #pragma unroll
             for (int h = 32; h >= 1; h >>= 1) {
@@ -357,7 +361,7 @@ __device__ inline void vgroup_fold(const double (&v)[NQ][kVPT], double *sh /* kG
                 else
                     x = x + o;
             }
-            if (lane == 0) sh[(g * NQ + q) * 4 + pw + j * (kGL / 64)] = x;
+            if (lane == 0) sh[(g * NQ + q) * 4 + pw + j * (GL / 64)] = x;
         }
     }
     __syncthreads();
@@ -377,8 +381,8 @@ __device__ inline void vgroup_fold(const double (&v)[NQ][kVPT], double *sh /* kG
     __syncthreads();
 }
 
-// Chunk table of a block, fixed through a solve: for round r (chunk c = work_block * kGrpB + g +
-// r * gridDim * kGrpB) its system and constraint range.  The first kChunkRounds rounds are kept in
+// Chunk table of a block, fixed through a solve: for round r (chunk c = work_block * G + g +
+// r * gridDim * G, G groups per block) its system and constraint range.  The first kChunkRounds rounds are kept in
 // LDS; later rounds (very large systems) read the tables.
 constexpr int kChunkRounds = 16;
 struct ChunkMeta {
@@ -397,17 +401,18 @@ __device__ inline ChunkMeta chunk_meta(const SolveArgsT<T> &a, int c, int nch) {
     return m;
 }
 
-// phase B: constraints in 512-element chunks, one chunk per kGL-lane group; chunk partials go to
+// phase B: constraints in 512-element chunks, one chunk per GL-lane group; chunk partials go to
 // part[q * nchunks + c]
-template <int MODE, typename T>
+template <int MODE, typename T, int GL>
 __device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, const int *s_state, double *sh,
                                const ChunkMeta (*s_cm)[kGrpB], int nch) {
+    constexpr int kVPT = GroupShape<GL>::vpt, G = GroupShape<GL>::per_block;
     constexpr int NE = kVPT * kRedPasses;  // elements per lane
-    const int g = threadIdx.x / kGL, l = threadIdx.x % kGL;
-    const int ngroups = gridDim.x * kGrpB;
+    const int g = threadIdx.x / GL, l = threadIdx.x % GL;
+    const int ngroups = gridDim.x * G;
     const T dt = (T)a.dt;
     int r = 0;
-    for (int base = work_block(a.xcd_map) * kGrpB; base < nch; base += ngroups, ++r) {
+    for (int base = work_block(a.xcd_map) * G; base < nch; base += ngroups, ++r) {
         const int c = base + g;
         // per virtual lane: sums (s^T s, s^T y, y^T y, non-finite count), max residual, and for a warm
         // start the cold-start residual and its non-finite count
@@ -430,7 +435,7 @@ __device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, co
 #pragma unroll
                 for (int p = 0; p < kRedPasses; ++p) {
                     const int q = j * kRedPasses + p;
-                    kk[q] = cm.k0 + (uint32_t)(l + j * kGL + p * kRedThreads);
+                    kk[q] = cm.k0 + (uint32_t)(l + j * GL + p * kRedThreads);
                     ok[q] = kk[q] < cm.k1;
                     if (ok[q]) {
                         const typename Vec2<T>::type xg = a.xg[kk[q]];
@@ -484,8 +489,8 @@ __device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, co
             }
         }
         double ts[4], tx[1];
-        vgroup_fold<4, false>(vs, sh, ts);
-        vgroup_fold<1, true>(vx, sh, tx);
+        vgroup_fold<4, false, GL>(vs, sh, ts);
+        vgroup_fold<1, true, GL>(vx, sh, tx);
         double *pc = a.part + c;
         const int nc = a.nchunks;
         if (act && l == 0) {
@@ -501,8 +506,8 @@ __device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, co
         }
         if (MODE == B_INIT && a.warm) {
             double t2[1], t3[1];
-            vgroup_fold<1, false>(vnfc, sh, t2);
-            vgroup_fold<1, true>(vrc, sh, t3);
+            vgroup_fold<1, false, GL>(vnfc, sh, t2);
+            vgroup_fold<1, true, GL>(vrc, sh, t3);
             if (act && l == 0) {
                 pc[(size_t)Q_RC * nc] = t3[0];
                 pc[(size_t)Q_NFC * nc] = t2[0];
@@ -527,9 +532,14 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     // row and chunk counts: on the device for a step replayed from a captured graph
     const int nch = a.nch_dev ? (*a.nch_dev < a.nchunks ? *a.nch_dev : a.nchunks) : a.nchunks;
     const uint32_t mrows = dcount(a.m, a.m_dev);
-    if (threadIdx.x < kChunkRounds * kGrpB) {
-        const int r = threadIdx.x / kGrpB, g = threadIdx.x % kGrpB;
-        s_cm[r][g] = chunk_meta(a, work_block(a.xcd_map) * kGrpB + g + r * gridDim.x * kGrpB, nch);
+    // 256-lane groups (two elements per lane) unless 128-lane groups save a round of chunks
+    const int G = kGrpB / 2;
+    const bool wide =
+        (nch + gridDim.x * G - 1) / (gridDim.x * G) == (nch + gridDim.x * kGrpB - 1) / (gridDim.x * kGrpB);
+    const int gb = wide ? G : kGrpB;
+    if (threadIdx.x < kChunkRounds * gb) {
+        const int r = threadIdx.x / gb, g = threadIdx.x % gb;
+        s_cm[r][g] = chunk_meta(a, work_block(a.xcd_map) * gb + g + r * gridDim.x * gb, nch);
     }
     const int i0 = work_block(a.xcd_map) * blockDim.x + threadIdx.x;
     const RodMeta rm0 = i0 < a.n ? rod_meta(a, i0) : RodMeta{0u, 0u, 0, 0};
@@ -548,7 +558,10 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     // start point: x = P(x0), g = A x + b
     phase_A<A_PROJ, T, AG>(a, s_alpha, s_state, rm0);
     if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
-    phase_B<B_INIT, T>(a, s_alpha, s_state, sh, s_cm, nch);
+    if (wide)
+        phase_B<B_INIT, T, 256>(a, s_alpha, s_state, sh, s_cm, nch);
+    else
+        phase_B<B_INIT, T, 128>(a, s_alpha, s_state, sh, s_cm, nch);
     if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
     {
         const int S[1] = {Q_NF}, X[1] = {Q_RMAX};
@@ -604,7 +617,10 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
         // alpha0 = g^T g / g^T A g
         phase_A<A_GRAD, T, AG>(a, s_alpha, s_state, rm0);
         if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
-        phase_B<B_ALPHA0, T>(a, s_alpha, s_state, sh, s_cm, nch);
+        if (wide)
+            phase_B<B_ALPHA0, T, 256>(a, s_alpha, s_state, sh, s_cm, nch);
+        else
+            phase_B<B_ALPHA0, T, 128>(a, s_alpha, s_state, sh, s_cm, nch);
         if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
         {
             const int S[2] = {Q_SS, Q_SY}, X[1] = {Q_RMAX};
@@ -621,7 +637,10 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
         for (int it = 1; it <= a.max_ite; ++it) {
             phase_A<A_PROJ, T, AG>(a, s_alpha, s_state, rm0);
             if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
-            phase_B<B_ITER, T>(a, s_alpha, s_state, sh, s_cm, nch);
+            if (wide)
+                phase_B<B_ITER, T, 256>(a, s_alpha, s_state, sh, s_cm, nch);
+            else
+                phase_B<B_ITER, T, 128>(a, s_alpha, s_state, sh, s_cm, nch);
             if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
             {
                 const int S[4] = {Q_SS, Q_SY, Q_YY, Q_NF}, X[1] = {Q_RMAX};
