@@ -47,7 +47,7 @@ typedef struct {
     double tol;                  /* conResTol */
     double diam_col_ratio;       /* sylinderDiameterColRatio */
     double len_col_ratio;        /* sylinderLengthColRatio */
-    double cell_size;            /* cell-list edge (um); <= 0 picks half the mean collision length */
+    double cell_size;            /* cell-list edge (um); <= 0 picks a quarter of the mean collision length */
     int32_t max_ite;             /* conMaxIte */
     int32_t solver;              /* conSolverChoice; 0 = BBPGD (the only solver provided) */
     int32_t precision;           /* 0 = FP64; 1 = FP32 iterations + FP64 residual and refinement */
