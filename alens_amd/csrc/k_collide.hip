@@ -510,6 +510,15 @@ HipErr launch_cell_ranges(hipStream_t st, const uint32_t *keys, int64_t E, uint3
 // atomic per block at the end; a block that fills its stage reserves the rest one by one.
 constexpr int kSweepT = 256;
 constexpr int kStage = 1024;
+constexpr int kSweepS = kSweepT + 64;  // entries whose rod data a block stages: its own and 64 beyond
+
+// Rod data of one sweep entry, from the block's LDS stage or, past it, from global memory.
+struct SweepRod {
+    uint32_t id;
+    int cl[3];
+    double lo[3], hi[3];
+};
+
 __global__ __launch_bounds__(kSweepT) void k_sweep_entries(RodsDev r, CfgDev c, const uint32_t *__restrict__ keys,
                                                        int64_t E, const uint32_t *__restrict__ cend,
                                                        const uint32_t *__restrict__ vals,
@@ -525,14 +534,36 @@ __global__ __launch_bounds__(kSweepT) void k_sweep_entries(RodsDev r, CfgDev c, 
     E = dcount((uint32_t)E, Ed);
     __shared__ uint32_t s_n, s_base;
     __shared__ uint32_t s_pi[kStage], s_pj[kStage], s_pm[kStage];
+    // the rods of entries e0 .. e0 + kSweepS, structure of arrays: box range start, inflated box,
+    // centre, direction, collision half-length and radius
+    __shared__ uint32_t s_id[kSweepS];
+    __shared__ int s_cl[3][kSweepS];
+    __shared__ double s_lo[3][kSweepS], s_hi[3][kSweepS], s_c[3][kSweepS], s_t[3][kSweepS], s_h[kSweepS],
+        s_r[kSweepS];
     const int lane = threadIdx.x & 63;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t e0 = (int64_t)blockIdx.x * blockDim.x;
+    const int64_t e = e0 + threadIdx.x;
+    for (int q = threadIdx.x; q < kSweepS; q += blockDim.x) {
+        if (e0 + q >= E) break;
+        const uint32_t j = vals[e0 + q];
+        s_id[q] = j;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            s_cl[k][q] = clo[3 * j + k];
+            s_lo[k][q] = blo[3 * j + k];
+            s_hi[k][q] = bhi[3 * j + k];
+            s_c[k][q] = r.center[3 * j + k];
+            s_t[k][q] = r.dir[3 * j + k];
+        }
+        s_h[q] = r.hcol[j];
+        s_r[q] = r.rcol[j];
+    }
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
     uint32_t cnt = 0, i = 0, s1 = 0;
-    int cc[3] = {0, 0, 0}, cli[3] = {0, 0, 0}, ui[3] = {0, 0, 0};
-    double bli[3] = {0, 0, 0}, bhi_i[3] = {0, 0, 0}, ci[3] = {0, 0, 0}, ti[3] = {0, 0, 0};
+    int cc[3] = {0, 0, 0}, ui[3] = {0, 0, 0};
+    double bhi_i[3] = {0, 0, 0}, bli[3] = {0, 0, 0}, ci[3] = {0, 0, 0}, ti[3] = {0, 0, 0};
     double hi_ = 0.0, ri = 0.0;
     int pbc[3] = {0, 0, 0}, nc[3] = {1, 1, 1};
     double len[3] = {0, 0, 0}, blo0[3] = {0, 0, 0}, hcell[3] = {1, 1, 1};
@@ -540,7 +571,8 @@ __global__ __launch_bounds__(kSweepT) void k_sweep_entries(RodsDev r, CfgDev c, 
         const uint32_t cell = keys[e];
         s1 = cend[cell];
         cnt = s1 - (uint32_t)e - 1u;
-        i = vals[e];
+        const int q = threadIdx.x;
+        i = s_id[q];
         const SysDev &g = r.sys[r.rsys[i]];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -557,15 +589,15 @@ __global__ __launch_bounds__(kSweepT) void k_sweep_entries(RodsDev r, CfgDev c, 
         if (cnt) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                cli[k] = clo[3 * i + k];
-                ui[k] = pbc[k] ? cli[k] + pmod_i(cc[k] - cli[k], nc[k]) : cc[k];
-                bli[k] = blo[3 * i + k];
-                bhi_i[k] = bhi[3 * i + k];
-                ci[k] = r.center[3 * i + k];
-                ti[k] = r.dir[3 * i + k];
+                const int cli = s_cl[k][q];
+                ui[k] = pbc[k] ? cli + pmod_i(cc[k] - cli, nc[k]) : cc[k];
+                bli[k] = s_lo[k][q];
+                bhi_i[k] = s_hi[k][q];
+                ci[k] = s_c[k][q];
+                ti[k] = s_t[k][q];
             }
-            hi_ = r.hcol[i];
-            ri = r.rcol[i];
+            hi_ = s_h[q];
+            ri = s_r[q];
         }
     }
     uint32_t maxc = cnt;
@@ -579,24 +611,49 @@ __global__ __launch_bounds__(kSweepT) void k_sweep_entries(RodsDev r, CfgDev c, 
         uint32_t j = 0;
         int m[3] = {0, 0, 0};
         if (t < cnt) {
-            j = vals[e + 1 + t];
+            const int64_t ej = e + 1 + t;
+            const int qj = (int)(ej - e0);
+            const bool staged = qj < kSweepS;
+            SweepRod w;
+            if (staged) {
+                w.id = s_id[qj];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    w.cl[k] = s_cl[k][qj];
+                    w.lo[k] = s_lo[k][qj];
+                    w.hi[k] = s_hi[k][qj];
+                }
+            } else {
+                w.id = vals[ej];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    w.cl[k] = clo[3 * w.id + k];
+                    w.lo[k] = blo[3 * w.id + k];
+                    w.hi[k] = bhi[3 * w.id + k];
+                }
+            }
+            j = w.id;
             bool ok = true;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                const int clj = clo[3 * j + k];
-                const int uj = pbc[k] ? clj + pmod_i(cc[k] - clj, nc[k]) : cc[k];
+                const int uj = pbc[k] ? w.cl[k] + pmod_i(cc[k] - w.cl[k], nc[k]) : cc[k];
                 m[k] = pbc[k] ? (ui[k] - uj) / nc[k] : 0;
                 const double sh = pbc[k] ? (double)m[k] * len[k] : 0.0;
-                ok = ok && (blo[3 * j + k] + sh <= bhi_i[k]) && (bli[k] <= bhi[3 * j + k] + sh);
+                ok = ok && (w.lo[k] + sh <= bhi_i[k]) && (bli[k] <= w.hi[k] + sh);
             }
             if (ok) {
-                double cj[3];
+                double cj[3], tj[3], hj, rj;
 #pragma unroll
-                for (int k = 0; k < 3; ++k)
-                    cj[k] = pbc[k] ? r.center[3 * j + k] + (double)m[k] * len[k] : r.center[3 * j + k];
-                const SegRes sg = segseg(ci[0], ci[1], ci[2], ti[0], ti[1], ti[2], hi_, cj[0], cj[1], cj[2],
-                                         r.dir[3 * j], r.dir[3 * j + 1], r.dir[3 * j + 2], r.hcol[j]);
-                const double d0 = (sg.dist - ri) - r.rcol[j];
+                for (int k = 0; k < 3; ++k) {
+                    const double ck = staged ? s_c[k][qj] : r.center[3 * j + k];
+                    cj[k] = pbc[k] ? ck + (double)m[k] * len[k] : ck;
+                    tj[k] = staged ? s_t[k][qj] : r.dir[3 * j + k];
+                }
+                hj = staged ? s_h[qj] : r.hcol[j];
+                rj = staged ? s_r[qj] : r.rcol[j];
+                const SegRes sg = segseg(ci[0], ci[1], ci[2], ti[0], ti[1], ti[2], hi_, cj[0], cj[1], cj[2], tj[0], tj[1],
+                                         tj[2], hj);
+                const double d0 = (sg.dist - ri) - rj;
                 pass = d0 < c.col_buf;
                 // the cell of the closest-point midpoint, in i's unwrapped frame
                 const double D[3] = {sg.D0, sg.D1, sg.D2};
