@@ -124,6 +124,21 @@ HipErr radix_sort_pairs(hipStream_t st, uint32_t *keys, uint32_t *vals, uint32_t
                         uint32_t *vals_alt, int64_t n, int keybits, DBuf<uint32_t> &hist,
                         ScanScratch &s, bool &result_in_alt, const uint32_t *nd = nullptr);
 HipErr launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, int64_t n);
+// several u32 buffers zeroed by one launch
+struct ZeroJobs {
+    uint32_t *p[8];
+    int64_t n[8];
+    int count;
+};
+HipErr launch_zero_many(hipStream_t st, const ZeroJobs &z);
+// End of a captured step: the counts, clock readings, barrier abort word and solver reports gathered
+// into one buffer (read back with one copy): tail[0..4] = entries, pairs, adjacency slots, valid rows,
+// chunks; tail[5..8] = clock readings; tail[9] = abort word; tail[16 ..] = the FP64 reports (nrep
+// words), then the FP32 ones when out32 is given.
+constexpr int kTailHead = 16;
+HipErr launch_step_tail(hipStream_t st, const uint32_t *entries, const uint32_t *pairs, const uint32_t *slots,
+                        const uint32_t *rows, const int32_t *chunks, const uint64_t *stamps, const uint32_t *abort_word,
+                        const double *out, const double *out32, int nrep, double *tail);
 HipErr launch_cell_ranges(hipStream_t st, const uint32_t *keys, int64_t E, uint32_t *cstart,
                           uint32_t *cend, const uint32_t *Ed = nullptr);
 HipErr launch_sweep(hipStream_t st, const RodsDev &r, const CfgDev &c, const uint32_t *keys, int64_t E, const uint32_t *cend, const uint32_t *vals, const int32_t *clo, const double *blo,

@@ -53,7 +53,7 @@ struct alens_ctx {
     DBuf<uint32_t> ecnt, eoff, keys, vals, keys2, vals2, cstart, cend, hist;
     ScanScratch scan;
 
-    DBuf<uint32_t> counter, pI, pJ, pImg, cntI, cntJ, rowStart, jStart, cur, jlist;
+    DBuf<uint32_t> counter, pI, pJ, pImg, cntI, cntJ, rowStart, jStart, cur, cur2, jlist;
     uint32_t pcap = 0;
     DBuf<int32_t> cI, cJ;
     DBuf<uint32_t> cImg;
@@ -93,6 +93,8 @@ struct alens_ctx {
     DBuf<double> out32;
     DBuf<uint32_t> dword;                  // [0] valid entries, [1] valid rows, [2] rows of the previous step
     DBuf<uint64_t> stamps;                 // wall-clock readings in a replay: start, collected, solve start, solved
+    DBuf<double> tail;                     // a replay's counts, readings and reports, read back in one copy
+    bool bar_clean = false;                // the captured step has zeroed the barrier words already
     int64_t clock_khz = 0;                 // rate of that clock
     DBuf<int32_t> nch_dev;
     DBuf<int64_t> step_dev;
@@ -262,7 +264,7 @@ void alens_destroy(alens_ctx *ctx) {
     rel(ctx->cend); rel(ctx->hist);
     for (auto &l : ctx->scan.lvl) rel(l);
     rel(ctx->counter); rel(ctx->pI); rel(ctx->pJ); rel(ctx->pImg); rel(ctx->cntI); rel(ctx->cntJ);
-    rel(ctx->rowStart); rel(ctx->jStart); rel(ctx->cur); rel(ctx->jlist); rel(ctx->cI); rel(ctx->cJ);
+    rel(ctx->rowStart); rel(ctx->jStart); rel(ctx->cur); rel(ctx->cur2); rel(ctx->jlist); rel(ctx->cI); rel(ctx->cJ);
     rel(ctx->cImg); rel(ctx->delta0); rel(ctx->geo); rel(ctx->posI); rel(ctx->posJ); rel(ctx->b);
     rel(ctx->adjStart); rel(ctx->sellStart); rel(ctx->sellK); rel(ctx->slen); rel(ctx->sellW); rel(ctx->xg); rel(ctx->gamma); rel(ctx->part);
     rel(ctx->out); rel(ctx->bar); rel(ctx->pkey); rel(ctx->pgam);
@@ -383,7 +385,7 @@ alens_status alens_set_systems(alens_ctx *ctx, int32_t nsys, const int32_t *rod_
     TRY_HIP(ctx->bhi.ensure(3 * N));
     TRY_HIP(ctx->ecnt.ensure(N + 1));
     TRY_HIP(ctx->eoff.ensure(N + 1));
-    for (auto *bf : {&ctx->cntI, &ctx->cntJ, &ctx->rowStart, &ctx->jStart, &ctx->cur, &ctx->adjStart})
+    for (auto *bf : {&ctx->cntI, &ctx->cntJ, &ctx->rowStart, &ctx->jStart, &ctx->cur, &ctx->cur2, &ctx->adjStart})
         TRY_HIP(bf->ensure(N + 1));
     TRY_HIP(ctx->cstart.ensure((size_t)ncell));
     TRY_HIP(ctx->cend.ensure((size_t)ncell));
@@ -570,20 +572,16 @@ static alens_status set_chunk_table(alens_ctx *ctx, const std::vector<uint32_t> 
 
 // Per-system report words of a finished solve -> per-system reports and their aggregate (max
 // iterations / residuals, all converged, any fallback, worst status).
-static alens_status read_reports(alens_ctx *ctx, const double *out_dev, alens_solve_report *rep,
-                                 std::vector<alens_solve_report> *per_sys) {
+// Per-system report words (8 per system, host copy) into the step report and the system reports.
+static alens_status parse_reports(alens_ctx *ctx, const double *o, uint32_t abort_word, alens_solve_report *rep,
+                                  std::vector<alens_solve_report> *per_sys) {
     const int32_t ns = ctx->nsys_cur;
-    std::vector<double> o(8 * (size_t)ns);
-    uint32_t abort_word = 0;
-    TRY_HIP(hipMemcpyAsync(o.data(), out_dev, sizeof(double) * o.size(), hipMemcpyDeviceToHost, ctx->st));
-    TRY_HIP(hipMemcpyAsync(&abort_word, ctx->bar.p + kBarWords, sizeof abort_word, hipMemcpyDeviceToHost, ctx->st));
-    TRY_HIP(hipStreamSynchronize(ctx->st));
     if (abort_word != 0) return fail(ctx, ALENS_E_HIP, "BBPGD grid barrier timed out (blocks not co-resident)");
     *rep = alens_solve_report{};
     rep->converged = 1;
     if (per_sys) per_sys->assign((size_t)ns, alens_solve_report{});
     for (int32_t sy = 0; sy < ns; ++sy) {
-        const double *w = o.data() + 8 * (size_t)sy;
+        const double *w = o + 8 * (size_t)sy;
         alens_solve_report r{};
         r.iters = (int32_t)w[0];
         r.matvecs = (int32_t)w[1];
@@ -604,6 +602,16 @@ static alens_status read_reports(alens_ctx *ctx, const double *out_dev, alens_so
     return ALENS_OK;
 }
 
+static alens_status read_reports(alens_ctx *ctx, const double *out_dev, alens_solve_report *rep,
+                                 std::vector<alens_solve_report> *per_sys) {
+    std::vector<double> o(8 * (size_t)ctx->nsys_cur);
+    uint32_t abort_word = 0;
+    TRY_HIP(hipMemcpyAsync(o.data(), out_dev, sizeof(double) * o.size(), hipMemcpyDeviceToHost, ctx->st));
+    TRY_HIP(hipMemcpyAsync(&abort_word, ctx->bar.p + kBarWords, sizeof abort_word, hipMemcpyDeviceToHost, ctx->st));
+    TRY_HIP(hipStreamSynchronize(ctx->st));
+    return parse_reports(ctx, o.data(), abort_word, rep, per_sys);
+}
+
 // Launches one persistent BBPGD solve on the context's stream.  Outside graph capture it waits for
 // it, times it and reads the report; while a step is being captured it only enqueues (the FP64
 // solve reports into ctx->out, the FP32 one into ctx->out32, read after the graph has run).
@@ -614,7 +622,10 @@ static alens_status run_bbpgd(alens_ctx *ctx, SolveArgsT<T> a, alens_solve_repor
     TRY_HIP(ctx->part.ensure((size_t)kRedQ * std::max<int32_t>(ctx->nchunks_cur, 1)));
     TRY_HIP(ctx->out.ensure(8 * (size_t)ns + 8));
     TRY_HIP(ctx->out32.ensure(8 * (size_t)ns + 8));
-    TRY_K(launch_fill_u32(ctx->st, ctx->bar.p, 0u, kBarTotalWords));
+    if (ctx->capture && ctx->bar_clean)
+        ctx->bar_clean = false;  // the captured step zeroed them with its other counters
+    else
+        TRY_K(launch_fill_u32(ctx->st, ctx->bar.p, 0u, kBarTotalWords));
     double *out = sizeof(T) == 8 ? ctx->out.p : ctx->out32.p;
     a.part = ctx->part.p;
     a.bar = ctx->bar.p;
@@ -1100,6 +1111,7 @@ static alens_status plan_resident(alens_ctx *ctx) {
     TRY_HIP(ctx->nch_dev.ensure(1));
     TRY_HIP(ctx->step_dev.ensure(1));
     TRY_HIP(ctx->stamps.ensure(4));
+    TRY_HIP(ctx->tail.ensure((size_t)kTailHead + 16 * (size_t)ns));
     if (ctx->clock_khz <= 0) {
         int khz = 0;
         TRY_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->dev));
@@ -1141,22 +1153,30 @@ static alens_status enqueue_resident(alens_ctx *ctx, bool fx, bool warm_prev) {
                            ctx->scan, alt, Ee));
     const uint32_t *skeys = alt ? ctx->keys2.p : ctx->keys.p;
     const uint32_t *svals = alt ? ctx->vals2.p : ctx->vals.p;
-    TRY_K(launch_fill_u32(st, ctx->cstart.p, 0u, ctx->ncell));
-    TRY_K(launch_fill_u32(st, ctx->cend.p, 0u, ctx->ncell));
+    {
+        // every counter and cursor of the step, and the solver's barrier words, in one launch
+        ZeroJobs z{};
+        uint32_t *bufs[8] = {ctx->cstart.p, ctx->cend.p, ctx->counter.p, ctx->cntI.p,
+                             ctx->cntJ.p,   ctx->cur.p,  ctx->cur2.p,    ctx->bar.p};
+        const int64_t lens[8] = {ctx->ncell, ctx->ncell, 1, (int64_t)n + 1, (int64_t)n + 1, (int64_t)n + 1,
+                                 (int64_t)n + 1, kBarTotalWords};
+        for (int q = 0; q < 8; ++q) {
+            z.p[q] = bufs[q];
+            z.n[q] = lens[q];
+        }
+        z.count = 8;
+        TRY_K(launch_zero_many(st, z));
+        ctx->bar_clean = true;
+    }
     TRY_K(launch_cell_ranges(st, skeys, c.E, ctx->cstart.p, ctx->cend.p, Ee));
-    TRY_K(launch_fill_u32(st, ctx->counter.p, 0u, 1));
-    TRY_K(launch_fill_u32(st, ctx->cntI.p, 0u, (int64_t)n + 1));
-    TRY_K(launch_fill_u32(st, ctx->cntJ.p, 0u, (int64_t)n + 1));
     TRY_K(launch_sweep(st, rd, cd, skeys, c.E, ctx->cend.p, svals, ctx->clo.p, ctx->blo.p, ctx->bhi.p,
                        ctx->counter.p, c.M, ctx->pI.p, ctx->pJ.p, ctx->pImg.p, ctx->cntI.p, ctx->cntJ.p, Ee));
     TRY_K(launch_guard_pairs(st, ctx->counter.p, c.M, Mv, ctx->cntI.p, ctx->cntJ.p, n));
     TRY_K(exclusive_scan_u32(st, ctx->cntI.p, ctx->rowStart.p, n, ctx->scan));
     TRY_K(exclusive_scan_u32(st, ctx->cntJ.p, ctx->jStart.p, n, ctx->scan));
-    TRY_K(launch_fill_u32(st, ctx->cur.p, 0u, (int64_t)n + 1));
     TRY_K(launch_order_I(st, n, c.M, ctx->pI.p, ctx->pJ.p, ctx->pImg.p, ctx->rowStart.p, ctx->cur.p, ctx->cI.p,
                          ctx->cJ.p, ctx->cImg.p, Mv));
-    TRY_K(launch_fill_u32(st, ctx->cur.p, 0u, (int64_t)n + 1));
-    TRY_K(launch_order_J(st, n, c.M, ctx->cJ.p, ctx->jStart.p, ctx->cur.p, ctx->jlist.p, Mv));
+    TRY_K(launch_order_J(st, n, c.M, ctx->cJ.p, ctx->jStart.p, ctx->cur2.p, ctx->jlist.p, Mv));
     TRY_K(launch_rows(st, rd, cd, c.M, c.M, ctx->cI.p, ctx->cJ.p, ctx->cImg.p, ctx->unc.p, ctx->delta0.p,
                       ctx->geo.p, ctx->posI.p, ctx->posJ.p, ctx->b.p, Mv));
     const int64_t nsl = ((int64_t)n + kSlice - 1) / kSlice;
@@ -1210,6 +1230,10 @@ static alens_status enqueue_resident(alens_ctx *ctx, bool fx, bool warm_prev) {
     if (ws >= 1)
         TRY_K(launch_make_keys(st, c.M, ctx->cI.p, ctx->cJ.p, ctx->gid.p, ctx->xg.p, ctx->pkey.p, ctx->pgam.p, Mv,
                                mprev));
+    // everything the host reads after a replay, in one buffer
+    TRY_K(launch_step_tail(st, ctx->eoff.p + n, ctx->counter.p, ctx->sellStart.p + nsl, Mv, ctx->nch_dev.p,
+                           ctx->stamps.p, ctx->bar.p + kBarWords, ctx->out.p,
+                           ctx->cfg.precision == 1 ? ctx->out32.p : nullptr, 8 * ctx->nsys, ctx->tail.p));
     return ALENS_OK;
 }
 
@@ -1248,16 +1272,12 @@ static alens_status step_resident(alens_ctx *ctx, const double *f_ext, alens_sol
     TRY_HIP(hipEventRecord(ctx->ev[0], st));
     TRY_HIP(hipGraphLaunch(ctx->gexec, st));
     TRY_HIP(hipEventRecord(ctx->ev[4], st));
-    uint64_t stamp[4] = {0, 0, 0, 0};
-    TRY_HIP(hipMemcpyAsync(stamp, ctx->stamps.p, sizeof stamp, hipMemcpyDeviceToHost, st));
-    const int64_t nsl = ((int64_t)n + kSlice - 1) / kSlice;
-    uint32_t cnt[5] = {0, 0, 0, 0, 0};  // entries, pairs, slots, valid rows, chunks
-    TRY_HIP(hipMemcpyAsync(&cnt[0], ctx->eoff.p + n, 4, hipMemcpyDeviceToHost, st));
-    TRY_HIP(hipMemcpyAsync(&cnt[1], ctx->counter.p, 4, hipMemcpyDeviceToHost, st));
-    TRY_HIP(hipMemcpyAsync(&cnt[2], ctx->sellStart.p + nsl, 4, hipMemcpyDeviceToHost, st));
-    TRY_HIP(hipMemcpyAsync(&cnt[3], ctx->dword.p + 1, 4, hipMemcpyDeviceToHost, st));
-    TRY_HIP(hipMemcpyAsync(&cnt[4], ctx->nch_dev.p, 4, hipMemcpyDeviceToHost, st));
+    const int nrep = 8 * ctx->nsys;
+    std::vector<double> tl((size_t)kTailHead + 2 * (size_t)nrep);
+    TRY_HIP(hipMemcpyAsync(tl.data(), ctx->tail.p, sizeof(double) * tl.size(), hipMemcpyDeviceToHost, st));
     TRY_HIP(hipStreamSynchronize(st));
+    uint32_t cnt[5];  // entries, pairs, slots, valid rows, chunks
+    for (int q = 0; q < 5; ++q) cnt[q] = (uint32_t)tl[q];
     if (cnt[0] > c.E || cnt[1] > c.M || cnt[2] > c.epad) {
         // every overflow left zero valid rows downstream, so the replay wrote no impulse cache and
         // the plain redo starts from the same previous-step impulses
@@ -1266,16 +1286,18 @@ static alens_status step_resident(alens_ctx *ctx, const double *f_ext, alens_sol
         return step_plain(ctx, f_ext, rep_out);
     }
     alens_solve_report rep{};
-    alens_status s = read_reports(ctx, ctx->out.p, &rep, &ctx->sysrep);
+    const uint32_t abort_word = (uint32_t)tl[9];
+    alens_status s = parse_reports(ctx, tl.data() + kTailHead, abort_word, &rep, &ctx->sysrep);
     if (s != ALENS_OK) return s;
     if (ctx->cfg.precision == 1) {
         alens_solve_report r32{};
         std::vector<alens_solve_report> s32;
-        s = read_reports(ctx, ctx->out32.p, &r32, &s32);
+        s = parse_reports(ctx, tl.data() + kTailHead + nrep, abort_word, &r32, &s32);
         if (s != ALENS_OK) return s;
         rep.iters_fp32 = r32.iters;
         for (size_t q = 0; q < s32.size() && q < ctx->sysrep.size(); ++q) ctx->sysrep[q].iters_fp32 = s32[q].iters;
     }
+    const uint64_t stamp[4] = {(uint64_t)tl[5], (uint64_t)tl[6], (uint64_t)tl[7], (uint64_t)tl[8]};
     float ms_step = 0.f;
     TRY_HIP(hipEventElapsedTime(&ms_step, ctx->ev[0], ctx->ev[4]));
     const double tick_ms = ctx->clock_khz > 0 ? 1.0 / (double)ctx->clock_khz : 0.0;

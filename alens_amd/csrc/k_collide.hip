@@ -361,6 +361,20 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
+__global__ void k_zero_many(ZeroJobs z) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int j = 0; j < z.count; ++j)
+        if (i < z.n[j]) z.p[j][i] = 0u;
+}
+HipErr launch_zero_many(hipStream_t st, const ZeroJobs &z) {
+    int64_t mx = 0;
+    for (int j = 0; j < z.count; ++j) mx = z.n[j] > mx ? z.n[j] : mx;
+    if (mx == 0) return hip_ok();
+    k_zero_many<<<nblk(mx, 256), 256, 0, st>>>(z);
+    LAUNCH_CHECK();
+    return hip_ok();
+}
+
 HipErr launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, int64_t n) {
     if (n == 0) return hip_ok();
     k_fill_u32<<<nblk(n, 256), 256, 0, st>>>(p, v, n);
@@ -905,26 +919,33 @@ __device__ inline uint32_t rod_degree(const uint32_t *rowStart, const uint32_t *
     return deg;
 }
 
-// one block per window of kOrderWin rods: rank = number of rods of the window with a larger degree,
-// or the same degree and a smaller index
+// one block per window of kOrderWin rods: a bitonic sort in LDS of the keys (descending degree,
+// then ascending index); rods past n sort last
 __global__ __launch_bounds__(kOrderWin) void k_slot_order(int32_t n, const uint32_t *__restrict__ rowStart,
                                                           const uint32_t *__restrict__ jStart,
                                                           const uint32_t *__restrict__ extStart,
                                                           uint32_t *__restrict__ sord) {
-    __shared__ int64_t s_deg[kOrderWin];
+    __shared__ uint64_t s_key[kOrderWin];
     const int32_t base = blockIdx.x * kOrderWin;
     const int t = threadIdx.x;
     const int32_t i = base + t;
-    const int64_t d = i < n ? (int64_t)rod_degree(rowStart, jStart, extStart, i) : -1;
-    s_deg[t] = d;
+    s_key[t] = i < n ? ((uint64_t)(0xffffffffu - rod_degree(rowStart, jStart, extStart, i)) << 32) | (uint32_t)t
+                     : ~0ull;
     __syncthreads();
-    if (i >= n) return;
-    int rank = 0;
-    for (int j = 0; j < kOrderWin; ++j) {
-        const int64_t e = s_deg[j];
-        rank += (e > d || (e == d && j < t)) ? 1 : 0;
+    for (int k = 2; k <= kOrderWin; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const int u = t ^ j;
+            if (u > t) {
+                const uint64_t a = s_key[t], b = s_key[u];
+                if ((a > b) == ((t & k) == 0)) {
+                    s_key[t] = b;
+                    s_key[u] = a;
+                }
+            }
+            __syncthreads();
+        }
     }
-    sord[base + rank] = (uint32_t)i;
+    if (i < n) sord[i] = (uint32_t)base + (uint32_t)(s_key[t] & 0xffffffffu);
 }
 
 HipErr launch_slot_order(hipStream_t st, int32_t n, const uint32_t *rowStart, const uint32_t *jStart,

@@ -911,6 +911,35 @@ __global__ void k_fill_u32_one(uint32_t *p, uint32_t v) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
 }
 
+__global__ void k_step_tail(const uint32_t *__restrict__ entries, const uint32_t *__restrict__ pairs,
+                            const uint32_t *__restrict__ slots, const uint32_t *__restrict__ rows,
+                            const int32_t *__restrict__ chunks, const uint64_t *__restrict__ stamps,
+                            const uint32_t *__restrict__ abort_word, const double *__restrict__ out,
+                            const double *__restrict__ out32, int nrep, double *__restrict__ tail) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) {
+        tail[0] = (double)*entries;
+        tail[1] = (double)*pairs;
+        tail[2] = (double)*slots;
+        tail[3] = (double)*rows;
+        tail[4] = (double)*chunks;
+        for (int q = 0; q < 4; ++q) tail[5 + q] = (double)stamps[q];
+        tail[9] = (double)*abort_word;
+    }
+    for (int i = t; i < nrep; i += gridDim.x * blockDim.x) {
+        tail[kTailHead + i] = out[i];
+        if (out32) tail[kTailHead + nrep + i] = out32[i];
+    }
+}
+HipErr launch_step_tail(hipStream_t st, const uint32_t *entries, const uint32_t *pairs, const uint32_t *slots,
+                        const uint32_t *rows, const int32_t *chunks, const uint64_t *stamps, const uint32_t *abort_word,
+                        const double *out, const double *out32, int nrep, double *tail) {
+    k_step_tail<<<nblk(nrep > 0 ? nrep : 1, 256), 256, 0, st>>>(entries, pairs, slots, rows, chunks, stamps, abort_word,
+                                                                out, out32, nrep, tail);
+    LAUNCH_CHECK();
+    return hip_ok();
+}
+
 // One reading of the device's constant-rate wall clock into t[idx]: the timing points of a captured
 // step (an event record inside stream capture is rejected by some HIP runtime builds).
 __global__ void k_stamp(uint64_t *t, int idx) {
