@@ -250,6 +250,7 @@ struct SolveArgsT {
     const uint32_t *m_dev;       // row count on the device, or NULL (= m, which is also the geo stride)
     int32_t bar_mode;            // grid barrier variant (k_solve.hip)
     int32_t xcd_map;             // 1: blocks of one XCD take consecutive work (k_solve.hip)
+    int32_t split_ok;            // 1: heavy rods may take helper lanes in phase A (k_solve.hip)
 };
 using SolveArgs = SolveArgsT<double>;
 // entries: adjacency entries in flight per phase-A step (2, or 0 for the largest instantiated)

@@ -650,6 +650,7 @@ static alens_status run_bbpgd(alens_ctx *ctx, SolveArgsT<T> a, alens_solve_repor
     a.bar_mode = env("ALENS_BARRIER_MODE", ctx->bar_mode);
     if (a.bar_mode < 0 || a.bar_mode > 2) a.bar_mode = 0;
     a.xcd_map = env("ALENS_XCD_MAP", ctx->xcd_map) != 0;
+    a.split_ok = env("ALENS_SOLVER_SPLIT", 1) != 0;
     const int entries = env("ALENS_SOLVER_ENTRIES", ctx->entries);
     // inside a graph the timing points are clock readings: stamp 2 before the first solver launch,
     // stamp 3 after each one (the last reading is the one read after the replay)

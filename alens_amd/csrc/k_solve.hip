@@ -242,68 +242,142 @@ __device__ inline int work_block(int xcd_map) {
     return x * (G / kBarGroups) + (x < G % kBarGroups ? x : G % kBarGroups) + q;
 }
 
+// Walks the adjacency entries j0 <= j < j1 of the rod whose first slot is idx0, in groups of kAG (a
+// group's gathers and weights load together, with the constraint indices of the next group).  With
+// PROD the products v_k * w_e go to prod[j - j0][.]; otherwise they are added to f one entry at a
+// time, in order.  The products are rounded the same way either way (no contraction), so a sum that
+// adds stored products in order equals the running sum.
+template <int MODE, typename T, int kAG, bool PROD>
+__device__ inline void walk_entries(const SolveArgsT<T> &a, size_t idx0, uint32_t j0, uint32_t j1, T alpha,
+                                    T (&f)[6], T (*prod)[6]) {
+    using V2 = typename Vec2<T>::type;
+    const size_t ep = a.epad;
+    constexpr uint32_t kMask = 0x7fffffffu;
+    size_t idx = idx0 + (size_t)j0 * kSlice;
+    uint32_t kc[kAG];
+#pragma unroll
+    for (int g = 0; g < kAG; ++g) kc[g] = j0 + g < j1 ? a.sellK[idx + g * kSlice] & kMask : 0u;
+    for (uint32_t j = j0; j < j1; j += kAG, idx += kAG * kSlice) {
+        V2 xv[kAG];
+        T lv[kAG], w[kAG][6];
+        uint32_t kn[kAG];
+#pragma unroll
+        for (int g = 0; g < kAG; ++g) {
+            const bool on = j + g < j1;
+            const size_t ig = idx + g * kSlice;
+            xv[g] = on ? a.xg[kc[g]] : mk2<T>(T(0), T(0));
+            lv[g] = (on && a.lb) ? a.lb[kc[g]] : T(0);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) w[g][c] = on ? a.sellW[c * ep + ig] : T(0);
+            kn[g] = j + kAG + g < j1 ? a.sellK[idx + (kAG + g) * kSlice] & kMask : 0u;
+        }
+#pragma unroll
+        for (int g = 0; g < kAG; ++g) {
+            if (j + g >= j1) break;
+            const T v = a_value<MODE, T>(xv[g], lv[g], alpha);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                if (PROD)
+                    prod[j + g - j0][c] = v * w[g][c];
+                else
+                    f[c] = f[c] + v * w[g][c];
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < kAG; ++g) kc[g] = kn[g];
+    }
+}
+
+// U = M F of a rod
+template <typename T>
+__device__ inline void store_U(const SolveArgsT<T> &a, const RodMeta &rm, const T (&f)[6]) {
+    using V2 = typename Vec2<T>::type;
+    const size_t i = (size_t)rm.rod;
+    const V2 *m2 = reinterpret_cast<const V2 *>(a.mob + 6 * i);
+    const V2 m01 = m2[0], m23 = m2[1], m45 = m2[2];
+    const T mb[6] = {m01.x, m01.y, m23.x, m23.y, m45.x, m45.y};
+    T u[6];
+    apply_M(mb, f, a.smono[rm.sy], u);
+    V2 *U2 = reinterpret_cast<V2 *>(a.U + kUStride * i);
+    U2[0] = mk2<T>(u[0], u[1]);
+    U2[1] = mk2<T>(u[2], u[3]);
+    U2[2] = mk2<T>(u[4], u[5]);
+}
+
+// Split of heavy rods (one slice position per lane): in a block, the rod at position t < kSplitN whose
+// degree exceeds kSplitH gets lane kBlk - 1 - t (the lightest end of the degree-ordered window) as a
+// helper when that lane's own rod has at most kSplitLight entries.  The helper forms the products of
+// entries kSplitH .. kSplitH + kSplitMax - 1 in LDS; the owner sums its first kSplitH entries, then adds
+// the stored products and walks the rest, so each rod's sum keeps its ascending order.
+constexpr int kSplitN = 128, kSplitH = 9, kSplitMax = 9;
+constexpr uint32_t kSplitLight = 3;
+struct SplitMeta {
+    uint32_t idx, deg;  // owner's first slot and degree; deg 0 = no helper
+    int sy;
+};
+
 // phase A: per rod, F = sum over incident constraints (ascending k) of v_k * w_e; U = M F
 // (rods of systems that have stopped are skipped).  Lane p % 64 of a wave walks the entries of the
 // rod at slice position p, so each step of the 64 lanes reads consecutive words.
-template <int MODE, typename T, int kAG>
+template <int MODE, typename T, int kAG, bool SPLIT>
 __device__ inline void phase_A(const SolveArgsT<T> &a, const double *s_alpha, const int *s_state,
-                               const RodMeta &rm0) {
-    using V2 = typename Vec2<T>::type;
+                               const RodMeta &rm0, bool split, const SplitMeta *s_help,
+                               T (*s_prod)[kSplitMax][6]) {
     const int stride = gridDim.x * blockDim.x;
     const int i0 = work_block(a.xcd_map) * blockDim.x + threadIdx.x;
-    const size_t ep = a.epad;
+    if (SPLIT && split) {
+        // one position per lane (n <= stride): helpers first, then every lane's own rod; owners finish
+        // after the block has synchronised.  The lane index passes through an empty asm so that the
+        // helper addresses are formed here each time instead of being kept live across the solve.
+        int t = threadIdx.x;
+        asm volatile("" : "+v"(t));
+        const volatile SplitMeta *vh = s_help;  // read per iteration, not kept live
+        if (t >= kBlk - kSplitN) {
+            const SplitMeta h{vh[kBlk - 1 - t].idx, vh[kBlk - 1 - t].deg, vh[kBlk - 1 - t].sy};
+            if (h.deg > 0 && s_state[h.sy] == S_ACTIVE) {
+                const uint32_t j1 =
+                    h.deg < (uint32_t)(kSplitH + kSplitMax - 1) ? h.deg : (uint32_t)(kSplitH + kSplitMax - 1);
+                T none[6];
+                walk_entries<MODE, T, kAG, true>(a, h.idx, kSplitH, j1, (T)s_alpha[h.sy], none, s_prod[kBlk - 1 - t]);
+            }
+        }
+        // an owner parks its partial sum in the last product row of its helper slot (rows past the
+        // helper's entries are free: the helper takes at most kSplitMax - 1 entries below)
+        bool owner = false;
+        if (i0 < a.n && s_state[rm0.sy] == S_ACTIVE) {
+            T f[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+            owner = t < kSplitN && vh[t].deg > 0;
+            walk_entries<MODE, T, kAG, false>(a, rm0.idx, 0, owner ? (uint32_t)kSplitH : rm0.deg,
+                                              (T)s_alpha[rm0.sy], f, nullptr);
+            if (owner) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) s_prod[t][kSplitMax - 1][c] = f[c];
+            } else {
+                store_U(a, rm0, f);
+            }
+        }
+        __syncthreads();
+        if (owner) {
+            T f[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) f[c] = s_prod[t][kSplitMax - 1][c];
+            const uint32_t jh = rm0.deg < (uint32_t)(kSplitH + kSplitMax - 1) ? rm0.deg
+                                                                               : (uint32_t)(kSplitH + kSplitMax - 1);
+            for (uint32_t q = 0; q < jh - kSplitH; ++q) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) f[c] = f[c] + s_prod[t][q][c];
+            }
+            walk_entries<MODE, T, kAG, false>(a, rm0.idx, jh, rm0.deg, (T)s_alpha[rm0.sy], f, nullptr);
+            store_U(a, rm0, f);
+        }
+        return;
+    }
     for (int p = i0; p < a.n; p += stride) {
         const RodMeta rm = p == i0 ? rm0 : rod_meta(a, p);
-        const size_t i = (size_t)rm.rod;
         if (s_state[rm.sy] != S_ACTIVE) continue;
-        const T alpha = (T)s_alpha[rm.sy];
-        T f0 = T(0), f1 = T(0), f2 = T(0), f3 = T(0), f4 = T(0), f5 = T(0);
-        // entries in groups of kAG: a group's gathers and weights load together, with the constraint
-        // indices of the next group; the sum still runs over the entries one by one, in order
-        size_t idx = rm.idx;
-        const uint32_t deg = rm.deg;
-        constexpr uint32_t kMask = 0x7fffffffu;
-        uint32_t kc[kAG];
-#pragma unroll
-        for (int g = 0; g < kAG; ++g) kc[g] = (uint32_t)g < deg ? a.sellK[idx + g * kSlice] & kMask : 0u;
-        for (uint32_t j = 0; j < deg; j += kAG, idx += kAG * kSlice) {
-            V2 xv[kAG];
-            T lv[kAG], w[kAG][6];
-            uint32_t kn[kAG];
-#pragma unroll
-            for (int g = 0; g < kAG; ++g) {
-                const bool on = j + g < deg;
-                const size_t ig = idx + g * kSlice;
-                xv[g] = on ? a.xg[kc[g]] : mk2<T>(T(0), T(0));
-                lv[g] = (on && a.lb) ? a.lb[kc[g]] : T(0);
-#pragma unroll
-                for (int c = 0; c < 6; ++c) w[g][c] = on ? a.sellW[c * ep + ig] : T(0);
-                kn[g] = j + kAG + g < deg ? a.sellK[idx + (kAG + g) * kSlice] & kMask : 0u;
-            }
-#pragma unroll
-            for (int g = 0; g < kAG; ++g) {
-                if (j + g >= deg) break;
-                const T v = a_value<MODE, T>(xv[g], lv[g], alpha);
-                f0 = f0 + v * w[g][0];
-                f1 = f1 + v * w[g][1];
-                f2 = f2 + v * w[g][2];
-                f3 = f3 + v * w[g][3];
-                f4 = f4 + v * w[g][4];
-                f5 = f5 + v * w[g][5];
-            }
-#pragma unroll
-            for (int g = 0; g < kAG; ++g) kc[g] = kn[g];
-        }
-        const V2 *m2 = reinterpret_cast<const V2 *>(a.mob + 6 * (size_t)i);
-        const V2 m01 = m2[0], m23 = m2[1], m45 = m2[2];
-        const T mb[6] = {m01.x, m01.y, m23.x, m23.y, m45.x, m45.y};
-        const T f[6] = {f0, f1, f2, f3, f4, f5};
-        T u[6];
-        apply_M(mb, f, a.smono[rm.sy], u);
-        V2 *U2 = reinterpret_cast<V2 *>(a.U + kUStride * (size_t)i);
-        U2[0] = mk2<T>(u[0], u[1]);
-        U2[1] = mk2<T>(u[2], u[3]);
-        U2[2] = mk2<T>(u[4], u[5]);
+        T f[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+        walk_entries<MODE, T, kAG, false>(a, rm.idx, 0, rm.deg, (T)s_alpha[rm.sy], f, nullptr);
+        store_U(a, rm, f);
     }
 }
 
@@ -518,7 +592,7 @@ __device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, co
 
 // Per-system report words in sysout[8 s ..]: iters, matvecs, converged, fallback, status, res0, res,
 // alpha at exit.  All blocks hold identical per-system state, so control flow stays uniform.
-template <typename T, int AG>
+template <typename T, int AG, bool SPLIT>
 __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     __shared__ double sh[kGrpB * 8 * 4];
     __shared__ int s_flag;
@@ -543,6 +617,20 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     }
     const int i0 = work_block(a.xcd_map) * blockDim.x + threadIdx.x;
     const RodMeta rm0 = i0 < a.n ? rod_meta(a, i0) : RodMeta{0u, 0u, 0, 0};
+    // heavy-rod helpers (phase A), fixed through the solve
+    __shared__ SplitMeta s_help[kSplitN];
+    __shared__ T s_prod[kSplitN][kSplitMax][6];
+    const bool split = SPLIT && a.n <= (int)(gridDim.x * blockDim.x);
+    {
+        __shared__ uint32_t s_deg[kBlk];
+        s_deg[threadIdx.x] = i0 < a.n ? rm0.deg : 0xffffffffu;
+        __syncthreads();
+        if (threadIdx.x < kSplitN) {
+            const uint32_t pd = s_deg[kBlk - 1 - threadIdx.x];
+            const bool help = split && i0 < a.n && rm0.deg > (uint32_t)kSplitH && pd <= kSplitLight;
+            s_help[threadIdx.x] = help ? SplitMeta{rm0.idx, rm0.deg, rm0.sy} : SplitMeta{0u, 0u, 0};
+        }
+    }
     for (int q = threadIdx.x; q < ns; q += blockDim.x) {
         s_alpha[q] = 0.0;
         s_state[q] = S_ACTIVE;
@@ -556,7 +644,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     uint32_t epoch = 0;
 
     // start point: x = P(x0), g = A x + b
-    phase_A<A_PROJ, T, AG>(a, s_alpha, s_state, rm0);
+    phase_A<A_PROJ, T, AG, SPLIT>(a, s_alpha, s_state, rm0, split, s_help, s_prod);
     if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
     if (wide)
         phase_B<B_INIT, T, 256>(a, s_alpha, s_state, sh, s_cm, nch);
@@ -615,7 +703,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     __syncthreads();
     if (s_any) {
         // alpha0 = g^T g / g^T A g
-        phase_A<A_GRAD, T, AG>(a, s_alpha, s_state, rm0);
+        phase_A<A_GRAD, T, AG, SPLIT>(a, s_alpha, s_state, rm0, split, s_help, s_prod);
         if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
         if (wide)
             phase_B<B_ALPHA0, T, 256>(a, s_alpha, s_state, sh, s_cm, nch);
@@ -635,7 +723,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
         }
         __syncthreads();
         for (int it = 1; it <= a.max_ite; ++it) {
-            phase_A<A_PROJ, T, AG>(a, s_alpha, s_state, rm0);
+            phase_A<A_PROJ, T, AG, SPLIT>(a, s_alpha, s_state, rm0, split, s_help, s_prod);
             if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
             if (wide)
                 phase_B<B_ITER, T, 256>(a, s_alpha, s_state, sh, s_cm, nch);
@@ -683,12 +771,12 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     }
 }
 
-template <typename T, int AG>
-static HipErr launch_bbpgd(hipStream_t st, SolveArgsT<T> a, int max_blocks, int *grid_used) {
+template <typename T, int AG, bool SPLIT>
+static HipErr launch_bbpgd_k(hipStream_t st, SolveArgsT<T> a, int max_blocks, int *grid_used) {
     int dev = 0, ncu = 0, per_cu = 0;
     ALENS_HIP_TRY(hipGetDevice(&dev));
     ALENS_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    ALENS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_bbpgd<T, AG>, kBlk, 0));
+    ALENS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_bbpgd<T, AG, SPLIT>, kBlk, 0));
     if (per_cu < 1) per_cu = 1;
     int64_t need = (int64_t)nblk(a.n, kBlk);
     const int64_t needB = ((int64_t)a.nchunks + kGrpB - 1) / kGrpB;
@@ -699,8 +787,23 @@ static HipErr launch_bbpgd(hipStream_t st, SolveArgsT<T> a, int max_blocks, int 
     const int grid = (int)(need < lim ? need : lim);
     *grid_used = grid;
     void *args[] = {&a};
-    ALENS_HIP_TRY(hipLaunchCooperativeKernel((const void *)k_bbpgd<T, AG>, dim3(grid), dim3(kBlk), args, 0, st));
+    ALENS_HIP_TRY(hipLaunchCooperativeKernel((const void *)k_bbpgd<T, AG, SPLIT>, dim3(grid), dim3(kBlk), args, 0, st));
     return hip_ok();
+}
+
+// The heavy-rod split variant (FP32 only) when every lane holds at most one rod.  In FP64 the split
+// kernel exceeds the 128-register budget and spills; measured on the 200k-rod slab it was slower there
+// (50.0 against 48.0 us per iteration) and faster in FP32 (3.99 against 4.15 ms per solve).
+template <typename T, int AG>
+static HipErr launch_bbpgd(hipStream_t st, SolveArgsT<T> a, int max_blocks, int *grid_used) {
+    if constexpr (sizeof(T) == 4) {
+        int dev = 0, ncu = 0;
+        ALENS_HIP_TRY(hipGetDevice(&dev));
+        ALENS_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        const int64_t lanes = (int64_t)(max_blocks > 0 && max_blocks < ncu ? max_blocks : ncu) * kBlk;
+        if (a.split_ok && (int64_t)a.n <= lanes) return launch_bbpgd_k<T, AG, true>(st, a, max_blocks, grid_used);
+    }
+    return launch_bbpgd_k<T, AG, false>(st, a, max_blocks, grid_used);
 }
 
 HipErr launch_bbpgd64(hipStream_t st, SolveArgsT<double> a, int max_blocks, int entries, int *grid_used) {

@@ -413,6 +413,33 @@ def test_mixed_radii_pair_set_bitexact(pbc):
     assert np.array_equal(I, o["I"]) and np.array_equal(J, o["J"]) and np.array_equal(img, o["img"])
 
 
+def test_heavy_rod_split_is_bit_identical(monkeypatch):
+    """In FP32 the solver lets light lanes form the products of heavy rods' later adjacency entries and
+    the owners add them in order: a trajectory with that split equals one without it bit for bit."""
+    rods, cfg = systems.scaled("monolayer_slab", 20000, seed=4)
+    runs = []
+    for sp in ("1", "0"):
+        monkeypatch.setenv("ALENS_SOLVER_SPLIT", sp)
+        ctx = capi.Context(**dict(cfg, precision=1))
+        ctx.set_rods(rods)
+        ctx.sort_rods()
+        its, gam, deg = [], [], 0
+        for _ in range(3):
+            r = ctx.step()
+            a = ctx.constraint_arrays()
+            deg = max(deg, int((np.bincount(a["I"], minlength=20000) + np.bincount(a["J"], minlength=20000)).max()))
+            its.append((r["iters_fp32"], r["iters"]))
+            gam.append(ctx.gamma())
+            u = ctx.velocity()["u"]
+            ctx.euler()
+        runs.append((its, gam, u, deg))
+    assert runs[0][3] > 9, "the system must hold rods heavy enough to be split"
+    assert runs[0][0] == runs[1][0]
+    for ga, gb in zip(runs[0][1], runs[1][1]):
+        assert_bits(ga, gb, "gamma with and without the split")
+    assert_bits(runs[0][2], runs[1][2], "U with and without the split")
+
+
 def test_external_force_and_monolayer_projection_bitexact():
     """A non-null F_ext (6 per rod) enters U_nc = M F_ext + U_B, and monolayer = 1 projects it."""
     rods, cfg = systems.small_3d(n=800, box=3.6, seed=23)
