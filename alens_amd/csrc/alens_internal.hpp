@@ -175,19 +175,19 @@ HipErr launch_rows_ext(hipStream_t st, uint32_t me, uint32_t k0, uint32_t gs, co
 // list, per rod in ascending e, e with the top bit set on the J side.
 // Slot order (optional): within each window of kOrderWin consecutive rods, rods take slice positions
 // by descending degree (ties by index), so a slice's 64 rods have similar degrees and few padding
-// slots.  sord[p] = rod at position p; NULL = identity.  Only which lane
+// slots.  sord[p] = rod at position p, spos[i] = position of rod i; NULL = identity.  Only which lane
 // walks which rod changes: every rod's sums still run over its entries in ascending k.
 constexpr int kSlice = 64;
 constexpr int kOrderWin = 1024;
 HipErr launch_slot_order(hipStream_t st, int32_t n, const uint32_t *rowStart, const uint32_t *jStart,
-                         const uint32_t *extStart, uint32_t *sord);
+                         const uint32_t *extStart, uint32_t *sord, uint32_t *spos);
 HipErr launch_slice_sizes(hipStream_t st, int32_t n, const uint32_t *rowStart, const uint32_t *jStart,
                           const uint32_t *extStart, uint32_t *slice_len, const uint32_t *sord = nullptr);
 HipErr launch_adjacency(hipStream_t st, int32_t n, uint32_t m, const uint32_t *rowStart,
                         const uint32_t *jStart, const uint32_t *jlist, const uint32_t *extStart,
                         const uint32_t *extEnt, uint32_t mc, const double *geo, const uint32_t *sellStart,
                         uint64_t epad, uint32_t *adjStart, uint32_t *sellK, double *sellW,
-                        const uint32_t *sord = nullptr);
+                        const uint32_t *spos = nullptr);
 HipErr launch_guard_entries(hipStream_t st, const uint32_t *eoff_n, uint32_t ecap, uint32_t *ee);
 HipErr launch_guard_pairs(hipStream_t st, const uint32_t *count, uint32_t cap, uint32_t *mv, uint32_t *cntI,
                           uint32_t *cntJ, int32_t n);
@@ -262,7 +262,7 @@ HipErr launch_init_xg32(hipStream_t st, float2 *xg, const double2 *xg64, uint32_
 HipErr launch_writeback(hipStream_t st, int32_t n, const int32_t *rsys, const SysDev *sys, const uint32_t *adjStart,
                         const uint32_t *sellStart, const uint32_t *sellK, const double *sellW, uint64_t epad,
                         const double2 *xg, const double *mob, const double *unc, double *F, double *U,
-                        double *gamma, uint32_t m, const uint32_t *md = nullptr, const uint32_t *sord = nullptr);
+                        double *gamma, uint32_t m, const uint32_t *md = nullptr, const uint32_t *spos = nullptr);
 HipErr launch_euler(hipStream_t st, int32_t n, const int32_t *rsys, const SysDev *sys, double *center, double *dir,
                     const double *u, const CfgDev &c);
 HipErr launch_init_xg(hipStream_t st, double2 *xg, const double *x0, uint32_t m);

@@ -59,8 +59,8 @@ struct alens_ctx {
     DBuf<uint32_t> cImg;
     DBuf<double> delta0, geo, posI, posJ, b;
     DBuf<uint32_t> adjStart, sellStart, sellK, slen;
-    DBuf<uint32_t> sord;                   // rod at each slice position (degree order within windows)
-    const uint32_t *sord_cur = nullptr;    // the last step's (NULL: identity)
+    DBuf<uint32_t> sord, spos;             // rod at each slice position (degree order within windows), inverse
+    const uint32_t *sord_cur = nullptr, *spos_cur = nullptr;  // the last step's (NULL: identity)
     DBuf<double> sellW;
     uint64_t epad = 0;  // padded entry count of the sliced adjacency (component stride of sellW)
 
@@ -264,7 +264,7 @@ void alens_destroy(alens_ctx *ctx) {
     rel(ctx->cend); rel(ctx->hist);
     for (auto &l : ctx->scan.lvl) rel(l);
     rel(ctx->counter); rel(ctx->pI); rel(ctx->pJ); rel(ctx->pImg); rel(ctx->cntI); rel(ctx->cntJ);
-    rel(ctx->rowStart); rel(ctx->jStart); rel(ctx->cur); rel(ctx->cur2); rel(ctx->jlist); rel(ctx->cI); rel(ctx->cJ);
+    rel(ctx->rowStart); rel(ctx->jStart); rel(ctx->cur); rel(ctx->cur2); rel(ctx->sord); rel(ctx->spos); rel(ctx->jlist); rel(ctx->cI); rel(ctx->cJ);
     rel(ctx->cImg); rel(ctx->delta0); rel(ctx->geo); rel(ctx->posI); rel(ctx->posJ); rel(ctx->b);
     rel(ctx->adjStart); rel(ctx->sellStart); rel(ctx->sellK); rel(ctx->slen); rel(ctx->sellW); rel(ctx->xg); rel(ctx->gamma); rel(ctx->part);
     rel(ctx->out); rel(ctx->bar); rel(ctx->pkey); rel(ctx->pgam);
@@ -942,11 +942,13 @@ static alens_status step_plain(alens_ctx *ctx, const double *f_ext, alens_solve_
         const int64_t nsl = ((int64_t)n + kSlice - 1) / kSlice;
         TRY_HIP(ctx->slen.ensure((size_t)nsl + 1));
         TRY_HIP(ctx->sellStart.ensure((size_t)nsl + 1));
-        ctx->sord_cur = nullptr;
+        ctx->sord_cur = ctx->spos_cur = nullptr;
         if (slot_order_on()) {
             TRY_HIP(ctx->sord.ensure((size_t)std::max(n, 1)));
-            TRY_K(launch_slot_order(st, n, ctx->rowStart.p, ctx->jStart.p, xs, ctx->sord.p));
+            TRY_HIP(ctx->spos.ensure((size_t)std::max(n, 1)));
+            TRY_K(launch_slot_order(st, n, ctx->rowStart.p, ctx->jStart.p, xs, ctx->sord.p, ctx->spos.p));
             ctx->sord_cur = ctx->sord.p;
+            ctx->spos_cur = ctx->spos.p;
         }
         TRY_K(launch_slice_sizes(st, n, ctx->rowStart.p, ctx->jStart.p, xs, ctx->slen.p, ctx->sord_cur));
         TRY_K(exclusive_scan_u32(st, ctx->slen.p, ctx->sellStart.p, nsl, ctx->scan));
@@ -963,7 +965,7 @@ static alens_status step_plain(alens_ctx *ctx, const double *f_ext, alens_solve_
         TRY_HIP(ctx->sellW.ensure(6 * std::max<size_t>(ep, 1)));
         TRY_K(launch_adjacency(st, n, M, ctx->rowStart.p, ctx->jStart.p, ctx->jlist.p, xs, Me ? ctx->extEnt.p : nullptr,
                                Mcol, ctx->geo.p, ctx->sellStart.p, ctx->epad, ctx->adjStart.p, ctx->sellK.p,
-                               ctx->sellW.p, ctx->sord_cur));
+                               ctx->sellW.p, ctx->spos_cur));
         alens_status cs = set_chunk_table(ctx, coff);
         if (cs != ALENS_OK) return cs;
     }
@@ -1011,7 +1013,7 @@ static alens_status step_plain(alens_ctx *ctx, const double *f_ext, alens_solve_
     }
     TRY_K(launch_writeback(st, n, ctx->rsys.p, ctx->sysd.p, ctx->adjStart.p, ctx->sellStart.p, ctx->sellK.p,
                            ctx->sellW.p, ctx->epad, ctx->xg.p, ctx->mob.p, ctx->unc.p, ctx->F.p, ctx->U.p,
-                           ctx->gamma.p, M, nullptr, ctx->sord_cur));
+                           ctx->gamma.p, M, nullptr, ctx->spos_cur));
     if (ctx->cfg.warm_start >= 1) {
         TRY_HIP(ctx->pkey.ensure(Mc));
         TRY_HIP(ctx->pgam.ensure(Mc));
@@ -1100,6 +1102,7 @@ static alens_status plan_resident(alens_ctx *ctx) {
     TRY_HIP(ctx->slen.ensure((size_t)nsl + 1));
     TRY_HIP(ctx->sellStart.ensure((size_t)nsl + 1));
     TRY_HIP(ctx->sord.ensure(N));
+    TRY_HIP(ctx->spos.ensure(N));
     TRY_HIP(ctx->sellK.ensure(c.epad));
     TRY_HIP(ctx->sellW.ensure(6 * (size_t)c.epad));
     TRY_HIP(ctx->con_off.ensure((size_t)ns + 1));
@@ -1181,16 +1184,17 @@ static alens_status enqueue_resident(alens_ctx *ctx, bool fx, bool warm_prev) {
     TRY_K(launch_rows(st, rd, cd, c.M, c.M, ctx->cI.p, ctx->cJ.p, ctx->cImg.p, ctx->unc.p, ctx->delta0.p,
                       ctx->geo.p, ctx->posI.p, ctx->posJ.p, ctx->b.p, Mv));
     const int64_t nsl = ((int64_t)n + kSlice - 1) / kSlice;
-    ctx->sord_cur = nullptr;
+    ctx->sord_cur = ctx->spos_cur = nullptr;
     if (slot_order_on()) {
-        TRY_K(launch_slot_order(st, n, ctx->rowStart.p, ctx->jStart.p, nullptr, ctx->sord.p));
+        TRY_K(launch_slot_order(st, n, ctx->rowStart.p, ctx->jStart.p, nullptr, ctx->sord.p, ctx->spos.p));
         ctx->sord_cur = ctx->sord.p;
+        ctx->spos_cur = ctx->spos.p;
     }
     TRY_K(launch_slice_sizes(st, n, ctx->rowStart.p, ctx->jStart.p, nullptr, ctx->slen.p, ctx->sord_cur));
     TRY_K(exclusive_scan_u32(st, ctx->slen.p, ctx->sellStart.p, nsl, ctx->scan));
     TRY_K(launch_guard_slots(st, ctx->sellStart.p + nsl, c.epad, Mv));
     TRY_K(launch_adjacency(st, n, c.M, ctx->rowStart.p, ctx->jStart.p, ctx->jlist.p, nullptr, nullptr, 0u, ctx->geo.p,
-                           ctx->sellStart.p, c.epad, ctx->adjStart.p, ctx->sellK.p, ctx->sellW.p, ctx->sord_cur));
+                           ctx->sellStart.p, c.epad, ctx->adjStart.p, ctx->sellK.p, ctx->sellW.p, ctx->spos_cur));
     TRY_K(launch_chunk_table(st, ctx->nsys, n, ctx->sysd.p, ctx->rowStart.p, 0u, ctx->con_off.p, ctx->chunk_off.p,
                              ctx->nch_dev.p, ctx->chunk_sys.p, c.nch, Mv));
     // start point and solve (the solver reads the row and chunk counts from the device)
@@ -1227,7 +1231,7 @@ static alens_status enqueue_resident(alens_ctx *ctx, bool fx, bool warm_prev) {
     if (s != ALENS_OK) return s;
     TRY_K(launch_writeback(st, n, ctx->rsys.p, ctx->sysd.p, ctx->adjStart.p, ctx->sellStart.p, ctx->sellK.p,
                            ctx->sellW.p, c.epad, ctx->xg.p, ctx->mob.p, ctx->unc.p, ctx->F.p, ctx->U.p, ctx->gamma.p,
-                           c.M, Mv, ctx->sord_cur));
+                           c.M, Mv, ctx->spos_cur));
     if (ws >= 1)
         TRY_K(launch_make_keys(st, c.M, ctx->cI.p, ctx->cJ.p, ctx->gid.p, ctx->xg.p, ctx->pkey.p, ctx->pgam.p, Mv,
                                mprev));
@@ -1679,7 +1683,7 @@ alens_status alens_bcqp_solve(alens_ctx *ctx, int32_t n, int64_t m, const int32_
         }
     }
     ctx->epad = ep;
-    ctx->sord_cur = nullptr;  // rods in index order in this layout
+    ctx->sord_cur = ctx->spos_cur = nullptr;  // rods in index order in this layout
     TRY_HIP(ctx->adjStart.ensure(N + 1));
     TRY_HIP(ctx->sellStart.ensure(nsl + 1));
     TRY_HIP(ctx->sellK.ensure(sk.size()));

@@ -924,7 +924,8 @@ __device__ inline uint32_t rod_degree(const uint32_t *rowStart, const uint32_t *
 __global__ __launch_bounds__(kOrderWin) void k_slot_order(int32_t n, const uint32_t *__restrict__ rowStart,
                                                           const uint32_t *__restrict__ jStart,
                                                           const uint32_t *__restrict__ extStart,
-                                                          uint32_t *__restrict__ sord) {
+                                                          uint32_t *__restrict__ sord,
+                                                          uint32_t *__restrict__ spos) {
     __shared__ uint64_t s_key[kOrderWin];
     const int32_t base = blockIdx.x * kOrderWin;
     const int t = threadIdx.x;
@@ -945,13 +946,17 @@ __global__ __launch_bounds__(kOrderWin) void k_slot_order(int32_t n, const uint3
             __syncthreads();
         }
     }
-    if (i < n) sord[i] = (uint32_t)base + (uint32_t)(s_key[t] & 0xffffffffu);
+    if (i < n) {
+        const uint32_t r = (uint32_t)base + (uint32_t)(s_key[t] & 0xffffffffu);
+        sord[i] = r;         // position i holds rod r
+        spos[r] = (uint32_t)i;
+    }
 }
 
 HipErr launch_slot_order(hipStream_t st, int32_t n, const uint32_t *rowStart, const uint32_t *jStart,
-                         const uint32_t *extStart, uint32_t *sord) {
+                         const uint32_t *extStart, uint32_t *sord, uint32_t *spos) {
     if (n == 0) return hip_ok();
-    k_slot_order<<<nblk(n, kOrderWin), kOrderWin, 0, st>>>(n, rowStart, jStart, extStart, sord);
+    k_slot_order<<<nblk(n, kOrderWin), kOrderWin, 0, st>>>(n, rowStart, jStart, extStart, sord, spos);
     LAUNCH_CHECK();
     return hip_ok();
 }
@@ -1075,17 +1080,16 @@ __global__ void k_adjacency(int32_t n, uint32_t m, const uint32_t *__restrict__ 
                             const uint32_t *__restrict__ extStart, const uint32_t *__restrict__ extEnt, uint32_t mc,
                             const double *__restrict__ geo, const uint32_t *__restrict__ sellStart, uint64_t epad,
                             uint32_t *__restrict__ adjStart, uint32_t *__restrict__ sellK,
-                            double *__restrict__ sellW, const uint32_t *__restrict__ sord) {
-    const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t > n) return;
+                            double *__restrict__ sellW, const uint32_t *__restrict__ spos) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
     // past the slot capacity every rod gets degree 0 (the host sees sellStart[last] > epad)
     const bool fits = sellStart[(n + kSlice - 1) / kSlice] <= epad;
-    adjStart[t] = fits ? rowStart[t] + jStart[t] + (extStart ? extStart[t] : 0u) : 0u;
-    if (t == n || !fits) return;
-    // thread t fills slice position t (consecutive lanes write consecutive slots) with the entries of
-    // the rod placed there
-    const uint32_t p = (uint32_t)t;
-    const int32_t i = sord ? (int32_t)sord[p] : t;
+    adjStart[i] = fits ? rowStart[i] + jStart[i] + (extStart ? extStart[i] : 0u) : 0u;
+    if (i == n || !fits) return;
+    // rod-major: one thread per rod reads its own lists in order and writes its slice position's
+    // slots (measured faster than walking positions, whose rods' lists are scattered)
+    const uint32_t p = spos ? spos[i] : (uint32_t)i;
     const uint32_t j0 = jStart[i], j1 = jStart[i + 1];
     uint64_t idx = sellStart[p >> 6] + (p & (kSlice - 1));
     for (uint32_t t = j0; t < j1; ++t, idx += kSlice) {
@@ -1128,9 +1132,9 @@ __global__ void k_adjacency(int32_t n, uint32_t m, const uint32_t *__restrict__ 
 HipErr launch_adjacency(hipStream_t st, int32_t n, uint32_t m, const uint32_t *rowStart,
                         const uint32_t *jStart, const uint32_t *jlist, const uint32_t *extStart,
                         const uint32_t *extEnt, uint32_t mc, const double *geo, const uint32_t *sellStart,
-                        uint64_t epad, uint32_t *adjStart, uint32_t *sellK, double *sellW, const uint32_t *sord) {
+                        uint64_t epad, uint32_t *adjStart, uint32_t *sellK, double *sellW, const uint32_t *spos) {
     k_adjacency<<<nblk((int64_t)n + 1, 256), 256, 0, st>>>(n, m, rowStart, jStart, jlist, extStart, extEnt, mc, geo,
-                                                          sellStart, epad, adjStart, sellK, sellW, sord);
+                                                          sellStart, epad, adjStart, sellK, sellW, spos);
     LAUNCH_CHECK();
     return hip_ok();
 }

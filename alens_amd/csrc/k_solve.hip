@@ -855,15 +855,14 @@ __global__ void k_writeback(int32_t n, const int32_t *__restrict__ rsys, const S
                             const uint32_t *__restrict__ sellK, const double *__restrict__ sellW, uint64_t epad,
                             const double2 *__restrict__ xg,
                             const double *__restrict__ mob, const double *__restrict__ unc,
-                            double *__restrict__ F, double *__restrict__ U, const uint32_t *__restrict__ sord) {
-    // thread p takes the rod at slice position p: consecutive lanes read consecutive slots
-    const int32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const int32_t i = sord ? (int32_t)sord[p] : p;
+                            double *__restrict__ F, double *__restrict__ U, const uint32_t *__restrict__ spos) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
     const int mono = sys[rsys[i]].mono;
     double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const uint32_t deg = adjStart[i + 1] - adjStart[i];
-    size_t idx = (size_t)sellStart[p >> 6] + ((uint32_t)p & (kSlice - 1));
+    const uint32_t p = spos ? spos[i] : (uint32_t)i;
+    size_t idx = (size_t)sellStart[p >> 6] + (p & (kSlice - 1));
     for (uint32_t j = 0; j < deg; ++j, idx += kSlice) {
         const uint32_t k = sellK[idx] & 0x7fffffffu;
         const double v = xg[k].x;
@@ -902,10 +901,10 @@ __global__ void k_copy_gamma(const double2 *__restrict__ xg, double *__restrict_
 HipErr launch_writeback(hipStream_t st, int32_t n, const int32_t *rsys, const SysDev *sys, const uint32_t *adjStart,
                         const uint32_t *sellStart, const uint32_t *sellK, const double *sellW, uint64_t epad,
                         const double2 *xg, const double *mob, const double *unc, double *F, double *U,
-                        double *gamma, uint32_t m, const uint32_t *md, const uint32_t *sord) {
+                        double *gamma, uint32_t m, const uint32_t *md, const uint32_t *spos) {
     if (n > 0) {
         k_writeback<<<nblk(n, 256), 256, 0, st>>>(n, rsys, sys, adjStart, sellStart, sellK, sellW, epad, xg, mob, unc,
-                                                  F, U, sord);
+                                                  F, U, spos);
         LAUNCH_CHECK();
     }
     if (m > 0) {
