@@ -608,8 +608,9 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     const uint32_t mrows = dcount(a.m, a.m_dev);
     // 256-lane groups (two elements per lane) unless 128-lane groups save a round of chunks
     const int G = kGrpB / 2;
-    const bool wide =
-        (nch + gridDim.x * G - 1) / (gridDim.x * G) == (nch + gridDim.x * kGrpB - 1) / (gridDim.x * kGrpB);
+    // (the split kernel keeps to 256-lane groups: the 128-lane path would not fit its register budget)
+    const bool wide = SPLIT || (nch + gridDim.x * G - 1) / (gridDim.x * G) ==
+                                   (nch + gridDim.x * kGrpB - 1) / (gridDim.x * kGrpB);
     const int gb = wide ? G : kGrpB;
     if (threadIdx.x < kChunkRounds * gb) {
         const int r = threadIdx.x / gb, g = threadIdx.x % gb;
@@ -646,7 +647,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     // start point: x = P(x0), g = A x + b
     phase_A<A_PROJ, T, AG, SPLIT>(a, s_alpha, s_state, rm0, split, s_help, s_prod);
     if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
-    if (wide)
+    if (wide || SPLIT)
         phase_B<B_INIT, T, 256>(a, s_alpha, s_state, sh, s_cm, nch);
     else
         phase_B<B_INIT, T, 128>(a, s_alpha, s_state, sh, s_cm, nch);
@@ -705,7 +706,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
         // alpha0 = g^T g / g^T A g
         phase_A<A_GRAD, T, AG, SPLIT>(a, s_alpha, s_state, rm0, split, s_help, s_prod);
         if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
-        if (wide)
+        if (wide || SPLIT)
             phase_B<B_ALPHA0, T, 256>(a, s_alpha, s_state, sh, s_cm, nch);
         else
             phase_B<B_ALPHA0, T, 128>(a, s_alpha, s_state, sh, s_cm, nch);
@@ -725,7 +726,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
         for (int it = 1; it <= a.max_ite; ++it) {
             phase_A<A_PROJ, T, AG, SPLIT>(a, s_alpha, s_state, rm0, split, s_help, s_prod);
             if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
-            if (wide)
+            if (wide || SPLIT)
                 phase_B<B_ITER, T, 256>(a, s_alpha, s_state, sh, s_cm, nch);
             else
                 phase_B<B_ITER, T, 128>(a, s_alpha, s_state, sh, s_cm, nch);
@@ -791,9 +792,9 @@ static HipErr launch_bbpgd_k(hipStream_t st, SolveArgsT<T> a, int max_blocks, in
     return hip_ok();
 }
 
-// The heavy-rod split variant (FP32 only) when every lane holds at most one rod.  In FP64 the split
-// kernel exceeds the 128-register budget and spills; measured on the 200k-rod slab it was slower there
-// (50.0 against 48.0 us per iteration) and faster in FP32 (3.99 against 4.15 ms per solve).
+// The heavy-rod split variant (FP32 only) when every lane holds at most one rod.  Measured on the
+// 200k-rod slab: FP32 solve 4.21 -> 3.99 ms; in FP64 the split kernel sits at the 128-register budget
+// with spills and was slower (4.79 -> 4.92 ms).
 template <typename T, int AG>
 static HipErr launch_bbpgd(hipStream_t st, SolveArgsT<T> a, int max_blocks, int *grid_used) {
     if constexpr (sizeof(T) == 4) {
