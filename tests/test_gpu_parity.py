@@ -123,6 +123,38 @@ def test_dense_straining_example_bitexact():
     compare_full(rods, cfg, g, o)
 
 
+def test_flexible_filament_example_bitexact():
+    """The reference's FlexibleFilament input: 200 beads (length 1e-6 < diameter 0.02: spheres) in a
+    chain 0.0202 um apart, open 10 um box (Examples/FlexibleFilament/RunConfig.yaml).  Every
+    neighbouring pair is a contact (gap 2e-4 um < colBuf 0.02)."""
+    rods = systems.read_rod_file(os.path.join(GOLDEN, "flexible_filament_TubuleInitial.dat"))
+    cfg = dict(box_lo=(-5, -5, -5), box_hi=(5, 5, 5), pbc=(0, 0, 0), monolayer=0, col_buf=0.02, viscosity=1.0,
+               kT=0.00411, dt=1e-4, tol=1e-5, max_ite=1000000, seed=1234)
+    ctx, g = gpu_step(rods, cfg)
+    o = O.step(rods, O.cfg_from_dict(cfg))
+    assert o["I"].size == 199 and np.all(o["J"] - o["I"] == 1)
+    compare_full(rods, cfg, g, o)
+
+
+def test_active_3d_nematics_like_bitexact():
+    """Examples/Active3DNematics/RunConfig.yaml's system drawn here (its initial file is generated at
+    run time): 500 rods, L = 0.25 um, D = 0.025 um, periodic 0.7 um box, no Brownian motion -- dense
+    and overlapping at the start."""
+    rng = np.random.default_rng(500)
+    n = 500
+    z = rng.uniform(-1, 1, n)
+    ph = rng.uniform(0, 2 * np.pi, n)
+    d = np.stack([np.sqrt(1 - z * z) * np.cos(ph), np.sqrt(1 - z * z) * np.sin(ph), z], 1)
+    rods = dict(gid=np.arange(n, dtype=np.int32), center=rng.uniform(-0.35, 0.35, (n, 3)), dir=d,
+                length=np.full(n, 0.25), radius=np.full(n, 0.0125), immovable=None)
+    cfg = dict(box_lo=(-0.35, -0.35, -0.35), box_hi=(0.35, 0.35, 0.35), pbc=(1, 1, 1), monolayer=0, col_buf=0.025,
+               viscosity=0.01, kT=-0.00411, dt=1e-4, tol=1e-5, max_ite=100000, seed=1234)
+    ctx, g = gpu_step(rods, cfg)
+    o = O.step(rods, O.cfg_from_dict(cfg))
+    assert o["I"].size > 1000
+    compare_full(rods, cfg, g, o)
+
+
 def test_mix_motor_sliding_has_no_contacts():
     """Two parallel rods 0.07 um apart (gap 0.045 > colBuf 0.025): no constraint."""
     rods = systems.read_rod_file(os.path.join(GOLDEN, "mix_motor_sliding_TubuleInitial.dat"))

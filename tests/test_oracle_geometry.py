@@ -62,6 +62,8 @@ def test_cell_list_matches_brute_force(kind, n):
 @pytest.mark.parametrize("fname,cfg", [
     ("dense_monolayer_TubuleInitial.dat", dict(box_hi=(11, 11, 11), pbc=(1, 1, 0), monolayer=1)),
     ("dense_straining_TubuleInitial.dat", dict(box_hi=(3, 1, 1), pbc=(1, 0, 0))),
+    ("flexible_filament_TubuleInitial.dat", dict(box_lo=(-5, -5, -5), box_hi=(5, 5, 5), pbc=(0, 0, 0),
+                                                 col_buf=0.02)),
 ])
 def test_reference_inputs_cells_vs_brute(fname, cfg):
     rods = systems.read_rod_file(os.path.join(GOLDEN, fname))
