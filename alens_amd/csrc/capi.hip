@@ -1,5 +1,7 @@
 // capi.hip -- the C ABI of libalens.so (include/alens_capi.h): context lifecycle and the per-step
-// orchestration of the collision-constraint path on one GPU.
+// orchestration of the collision-constraint path on one GPU: plain steps (counts read back between
+// phases), graph mode (the whole step captured once and replayed with device-side counts), caller
+// constraint blocks, batched systems, the autotuner, the learned warm start and the BCQP entry point.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -1,7 +1,9 @@
 // k_collide.hip -- per-step collision collection on gfx950:
-//   per-rod mobility + U_nc (Brownian), AABB cell ranges, LSD radix sort of (cell, rod) entries,
-//   cell ranges, pair sweep (FP64 segment distance + cutoff), canonical ordering of the pairs,
-//   constraint rows and the rod-major adjacency used by the solver.
+//   per-rod mobility + U_nc (Brownian), the tube cell list (each rod enters the cells its inflated
+//   segment meets), LSD radix sort of (cell, rod) entries, cell ranges, pair sweep (FP64 segment
+//   distance + cutoff, emission in the cell of the closest-point midpoint), canonical ordering of the
+//   pairs, constraint rows, the degree order of the slices and the rod-major sliced adjacency used
+//   by the solver.
 // Reference roles: SylinderSystem::calcVelocityBrown / calcVelocityNonCon / collectPairCollision and
 // ConstraintCollector::buildConstraintMatrixVector, reached from rodSystem.runStep()
 // (SRC/TubuleSystem.cpp:223); block layout from SRC/TubuleSystem.cpp:719-736.

@@ -3,9 +3,12 @@
 A step is one pass of the hot path over the device-resident rod system: U_nc (mobility + Brownian),
 cell list + pair sweep + rows, persistent BBPGD to residual tol, write-back, Euler move.
 Default workload: the dense quasi-2D monolayer (200,000 rods, L = 1 um, 100 x 100 um slab, nematic
-von Mises kappa = 4), FP64.  Prints one JSON line.
+von Mises kappa = 4), FP64 plain BBPGD, each step one replay of a captured HIP graph.  Beside it: the
+chosen form (FP32 + FP64 refinement, learned warm start) with its accuracy and same-state iteration
+counts, and the FP64 oracle on the host cores for the same step.  Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload monolayer_slab] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload monolayer_slab|lognormal_3d|small_3d|replicas]
+                  [--precision 0|1] [--graph 1|0] [--no-cpu] [--no-form] [--no-autotune]
 """
 import argparse
 import json
