@@ -319,10 +319,14 @@ struct SplitMeta {
 // phase A: per rod, F = sum over incident constraints (ascending k) of v_k * w_e; U = M F
 // (rods of systems that have stopped are skipped).  Lane p % 64 of a wave walks the entries of the
 // rod at slice position p, so each step of the 64 lanes reads consecutive words.
+// rod metadata of a lane's passes 1 .. kPasses - 1 (pass 0 is kept in registers), fixed through a
+// solve: a pass of a stopped system costs one LDS read instead of five dependent global loads
+constexpr int kPasses = 6;
+
 template <int MODE, typename T, int kAG, bool SPLIT>
 __device__ inline void phase_A(const SolveArgsT<T> &a, const double *s_alpha, const int *s_state,
                                const RodMeta &rm0, bool split, const SplitMeta *s_help,
-                               T (*s_prod)[kSplitMax][6]) {
+                               T (*s_prod)[kSplitMax][6], const RodMeta (*s_rmeta)[kBlk]) {
     const int stride = gridDim.x * blockDim.x;
     const int i0 = work_block(a.xcd_map) * blockDim.x + threadIdx.x;
     if (SPLIT && split) {
@@ -372,8 +376,9 @@ __device__ inline void phase_A(const SolveArgsT<T> &a, const double *s_alpha, co
         }
         return;
     }
-    for (int p = i0; p < a.n; p += stride) {
-        const RodMeta rm = p == i0 ? rm0 : rod_meta(a, p);
+    int q = 0;
+    for (int p = i0; p < a.n; p += stride, ++q) {
+        const RodMeta rm = q == 0 ? rm0 : (q < kPasses ? s_rmeta[q - 1][threadIdx.x] : rod_meta(a, p));
         if (s_state[rm.sy] != S_ACTIVE) continue;
         T f[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
         walk_entries<MODE, T, kAG, false>(a, rm.idx, 0, rm.deg, (T)s_alpha[rm.sy], f, nullptr);
@@ -618,6 +623,15 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     }
     const int i0 = work_block(a.xcd_map) * blockDim.x + threadIdx.x;
     const RodMeta rm0 = i0 < a.n ? rod_meta(a, i0) : RodMeta{0u, 0u, 0, 0};
+    __shared__ RodMeta s_rmeta[kPasses - 1][kBlk];
+    {
+        const int stride = gridDim.x * blockDim.x;
+        for (int q = 1; q < kPasses; ++q) {
+            const int p = i0 + q * stride;
+            if (p >= a.n) break;
+            s_rmeta[q - 1][threadIdx.x] = rod_meta(a, p);
+        }
+    }
     // heavy-rod helpers (phase A), fixed through the solve
     __shared__ SplitMeta s_help[kSplitN];
     __shared__ T s_prod[kSplitN][kSplitMax][6];
@@ -645,7 +659,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     uint32_t epoch = 0;
 
     // start point: x = P(x0), g = A x + b
-    phase_A<A_PROJ, T, AG, SPLIT>(a, s_alpha, s_state, rm0, split, s_help, s_prod);
+    phase_A<A_PROJ, T, AG, SPLIT>(a, s_alpha, s_state, rm0, split, s_help, s_prod, s_rmeta);
     if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
     if (wide || SPLIT)
         phase_B<B_INIT, T, 256>(a, s_alpha, s_state, sh, s_cm, nch);
@@ -704,7 +718,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     __syncthreads();
     if (s_any) {
         // alpha0 = g^T g / g^T A g
-        phase_A<A_GRAD, T, AG, SPLIT>(a, s_alpha, s_state, rm0, split, s_help, s_prod);
+        phase_A<A_GRAD, T, AG, SPLIT>(a, s_alpha, s_state, rm0, split, s_help, s_prod, s_rmeta);
         if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
         if (wide || SPLIT)
             phase_B<B_ALPHA0, T, 256>(a, s_alpha, s_state, sh, s_cm, nch);
@@ -724,7 +738,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
         }
         __syncthreads();
         for (int it = 1; it <= a.max_ite; ++it) {
-            phase_A<A_PROJ, T, AG, SPLIT>(a, s_alpha, s_state, rm0, split, s_help, s_prod);
+            phase_A<A_PROJ, T, AG, SPLIT>(a, s_alpha, s_state, rm0, split, s_help, s_prod, s_rmeta);
             if (!grid_barrier(a.bar, epoch, &s_flag, bm)) return;
             if (wide || SPLIT)
                 phase_B<B_ITER, T, 256>(a, s_alpha, s_state, sh, s_cm, nch);
