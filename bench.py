@@ -236,12 +236,12 @@ def main():
         }
         # iterations on identical states (a plain FP64 driver moves the rods; each form keeps its own
         # previous-step impulses): plain FP64 against the FP32 form with each warm start
-        ev = W.evaluate(args.workload, n, 4, P, norm, seed=cfg["seed"], warmup=8,
+        ev = W.evaluate(args.workload, n, 4, P, norm, seed=cfg["seed"], warmup=40,
                         forms={"plain_fp64": (0, 0), "fp32_cold": (0, 1), "fp32_prev_gamma": (1, 1),
                                "fp32_network": (2, 1), "fp64_network": (2, 0)})
         pm = float(ev["plain_fp64"].mean())
         out["chosen_form"]["same_state_iterations"] = {
-            "states": "4 steps after 8 warm-up from the workload's seeded start",
+            "states": "4 steps after 40 warm-up from the workload's seeded start",
             **{k: float(ev[k].mean()) for k in ("plain_fp64", "fp32_cold", "fp32_prev_gamma", "fp32_network",
                                                 "fp64_network")},
             "fallbacks_fp32_network": int(ev["fallback_fp32_network"]),
