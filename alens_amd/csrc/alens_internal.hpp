@@ -283,7 +283,9 @@ constexpr int kUStride = 6;
 // grid-barrier words of the persistent solver (layout in k_solve.hip)
 constexpr int kBarWords = 32;
 constexpr int kBarGroups = 8;
-constexpr int kBarTotalWords = (2 + kBarGroups) * kBarWords;
+constexpr int kBarXcc = 16;  // XCC id slots (the hardware field is 4 bits wide)
+constexpr int kBarXccArr = 2 + kBarGroups, kBarXccSize = kBarXccArr + kBarXcc, kBarSetup = kBarXccSize + kBarXcc;
+constexpr int kBarTotalWords = (kBarSetup + 1) * kBarWords;
 
 // learned warm start (k_mlp.hip): 12 -> 64 -> 64 -> 64 -> 1; W1 stored 64 x 16 (columns 12..15 zero)
 constexpr int kMlpIn = 12;

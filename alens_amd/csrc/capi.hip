@@ -84,7 +84,7 @@ struct alens_ctx {
     bool have_xprev = false;
     DBuf<uint32_t> bar;
     // solver schedule (the autotuner's choices; environment variables override them unless tuning)
-    int32_t bar_mode = 1, xcd_map = 0, max_blocks = 0, entries = 0;
+    int32_t bar_mode = 3, xcd_map = 0, max_blocks = 0, entries = 0;
     bool tuning = false;
     // device-resident step (graph mode): capacities, device counts, the captured step
     bool capture = false;                  // enqueueing into a graph: no host waits
@@ -650,7 +650,7 @@ static alens_status run_bbpgd(alens_ctx *ctx, SolveArgsT<T> a, alens_solve_repor
     };
     const int max_blocks = env("ALENS_SOLVER_MAX_BLOCKS", ctx->max_blocks);
     a.bar_mode = env("ALENS_BARRIER_MODE", ctx->bar_mode);
-    if (a.bar_mode < 0 || a.bar_mode > 2) a.bar_mode = 0;
+    if (a.bar_mode < 0 || a.bar_mode > 3) a.bar_mode = 0;
     a.xcd_map = env("ALENS_XCD_MAP", ctx->xcd_map) != 0;
     a.split_ok = env("ALENS_SOLVER_SPLIT", 1) != 0;
     const int entries = env("ALENS_SOLVER_ENTRIES", ctx->entries);
@@ -1572,7 +1572,7 @@ alens_status alens_autotune(alens_ctx *ctx, alens_tune_report *rep_out) {
     alens_tune_report tr{};
     ctx->tuning = true;
     // coordinate descent from the defaults over the switches, one axis at a time
-    ctx->bar_mode = 1;
+    ctx->bar_mode = 3;
     ctx->xcd_map = 0;
     ctx->max_blocks = 0;
     ctx->entries = 0;
@@ -1587,7 +1587,7 @@ alens_status alens_autotune(alens_ctx *ctx, alens_tune_report *rep_out) {
     };
     std::vector<Axis> axes = {
         {&ctx->entries, {0, 2}},
-        {&ctx->bar_mode, {1, 0, 2}},
+        {&ctx->bar_mode, {3, 1, 0, 2}},
         {&ctx->xcd_map, {0, 1}},
         {&ctx->max_blocks, {0, (int32_t)std::max<int64_t>(8, grid * 3 / 4), (int32_t)std::max<int64_t>(8, grid / 2)}},
     };

@@ -188,7 +188,7 @@ alens_status alens_get_velocity(alens_ctx *ctx, double *u, double *u_nc, double 
 
 /* Solver autotuner (no counterpart in the reference, whose solver runs on Trilinos over MPI):
  * times cold solves of the last step's rows under candidate schedules - adjacency entries in flight
- * per rod step (largest, 2), grid-barrier variant (grouped, flat, grouped with longer polls),
+ * per rod step (largest, 2), grid-barrier variant (grouped by XCC, grouped, flat, grouped with longer polls),
  * XCD-contiguous work mapping, persistent grid size (all co-resident, 3/4, 1/2) - one axis at a time
  * from the defaults, keeps the fastest in the context for later steps, and reports the choice.
  * Results are bit-identical under every schedule; only the time changes.  Needs a step with
@@ -196,7 +196,8 @@ alens_status alens_get_velocity(alens_ctx *ctx, double *u, double *u_nc, double 
  * ALENS_BARRIER_MODE, ALENS_XCD_MAP and ALENS_SOLVER_MAX_BLOCKS override the choice outside tuning. */
 typedef struct {
     int32_t entries;     /* 0 = largest instantiated (3 in FP64, 4 in FP32), or 2 */
-    int32_t bar_mode;    /* 0 flat, 1 grouped, 2 grouped with longer polls */
+    int32_t bar_mode;    /* 0 flat, 1 grouped, 2 grouped with longer polls, 3 grouped by XCC with one
+                            L2 write-back per XCC */
     int32_t xcd_map;     /* 1 = blocks of one XCD take consecutive work */
     int32_t max_blocks;  /* 0 = all co-resident blocks */
     int32_t candidates;  /* schedules timed */
