@@ -541,3 +541,28 @@ def test_barrier_variants_bit_identical(monkeypatch, max_blocks):
             assert ra[0] == rb[0] and ra[0] > 10, (mode, s, ra[0], rb[0])
             assert_bits(ra[1], rb[1], f"gamma, barrier mode {mode}, step {s}")
             assert_bits(ra[2], rb[2], f"U, barrier mode {mode}, step {s}")
+
+
+def test_barrier_variants_bit_identical_replicas(monkeypatch):
+    """The same across barrier variants for 12 replica systems solved in one launch (systems stop at
+    different iterations while the others go on)."""
+    parts, cfg = systems.replicas(nsys=12, nmin=2000, nmax=12000, seed=21)
+    runs = {}
+    for mode in (1, 0, 3):
+        monkeypatch.setenv("ALENS_BARRIER_MODE", str(mode))
+        c = capi.Context(**cfg)
+        c.set_systems(parts)
+        c.sort_rods()
+        out = []
+        for s in range(3):
+            if s == 1:
+                c.set_graph(True)
+            c.step()
+            out.append(([r["iters"] for r in c.system_reports()], c.gamma().copy(), c.velocity()["u"].copy()))
+            c.euler()
+        runs[mode] = out
+    for mode in (0, 3):
+        for s, (ra, rb) in enumerate(zip(runs[1], runs[mode])):
+            assert ra[0] == rb[0] and len(set(ra[0])) > 1, (mode, s, ra[0], rb[0])
+            assert_bits(ra[1], rb[1], f"gamma, barrier mode {mode}, step {s}")
+            assert_bits(ra[2], rb[2], f"U, barrier mode {mode}, step {s}")
