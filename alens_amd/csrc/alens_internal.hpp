@@ -141,7 +141,8 @@ HipErr launch_step_tail(hipStream_t st, const uint32_t *entries, const uint32_t 
                         const double *out, const double *out32, int nrep, double *tail);
 HipErr launch_cell_ranges(hipStream_t st, const uint32_t *keys, int64_t E, uint32_t *cstart,
                           uint32_t *cend, const uint32_t *Ed = nullptr);
-HipErr launch_sweep(hipStream_t st, const RodsDev &r, const CfgDev &c, const uint32_t *keys, int64_t E, const uint32_t *cend, const uint32_t *vals, const int32_t *clo, const double *blo,
+HipErr launch_sweep(hipStream_t st, const RodsDev &r, const CfgDev &c, const uint32_t *keys, int64_t E,
+                    const uint32_t *cstart, const uint32_t *cend, const uint32_t *vals, const int32_t *clo, const double *blo,
                     const double *bhi, uint32_t *pair_count, uint32_t cap, uint32_t *pI, uint32_t *pJ,
                     uint32_t *pImg, uint32_t *cntI, uint32_t *cntJ, const uint32_t *Ed = nullptr);
 HipErr launch_morton_keys(hipStream_t st, const RodsDev &r, int32_t i0, int32_t n, uint32_t *keys,

@@ -866,7 +866,7 @@ static alens_status step_plain(alens_ctx *ctx, const double *f_ext, alens_solve_
         TRY_HIP(hipMemsetAsync(ctx->counter.p, 0, sizeof(uint32_t), st));
         TRY_HIP(hipMemsetAsync(ctx->cntI.p, 0, sizeof(uint32_t) * (n + 1), st));
         TRY_HIP(hipMemsetAsync(ctx->cntJ.p, 0, sizeof(uint32_t) * (n + 1), st));
-        TRY_K(launch_sweep(st, rd, cd, skeys, E, ctx->cend.p, svals, ctx->clo.p, ctx->blo.p, ctx->bhi.p,
+        TRY_K(launch_sweep(st, rd, cd, skeys, E, ctx->cstart.p, ctx->cend.p, svals, ctx->clo.p, ctx->blo.p, ctx->bhi.p,
                            ctx->counter.p, ctx->pcap, ctx->pI.p, ctx->pJ.p, ctx->pImg.p, ctx->cntI.p,
                            ctx->cntJ.p));
         TRY_HIP(hipMemcpyAsync(&M, ctx->counter.p, sizeof M, hipMemcpyDeviceToHost, st));
@@ -1175,7 +1175,7 @@ static alens_status enqueue_resident(alens_ctx *ctx, bool fx, bool warm_prev) {
         ctx->bar_clean = true;
     }
     TRY_K(launch_cell_ranges(st, skeys, c.E, ctx->cstart.p, ctx->cend.p, Ee));
-    TRY_K(launch_sweep(st, rd, cd, skeys, c.E, ctx->cend.p, svals, ctx->clo.p, ctx->blo.p, ctx->bhi.p,
+    TRY_K(launch_sweep(st, rd, cd, skeys, c.E, ctx->cstart.p, ctx->cend.p, svals, ctx->clo.p, ctx->blo.p, ctx->bhi.p,
                        ctx->counter.p, c.M, ctx->pI.p, ctx->pJ.p, ctx->pImg.p, ctx->cntI.p, ctx->cntJ.p, Ee));
     TRY_K(launch_guard_pairs(st, ctx->counter.p, c.M, Mv, ctx->cntI.p, ctx->cntJ.p, n));
     TRY_K(exclusive_scan_u32(st, ctx->cntI.p, ctx->rowStart.p, n, ctx->scan));

@@ -520,76 +520,99 @@ HipErr launch_cell_ranges(hipStream_t st, const uint32_t *keys, int64_t E, uint3
 // shared cell computes the same midpoint -- each (pair, image) is emitted exactly once.  Per-rod
 // counts of both sides feed the ordering.
 // ------------------------------------------------------------------------------------------------
-// One thread per sorted entry e (rod i in cell c): pairs with the later entries of the same cell.
+// One thread per sorted entry.  The k(k - 1) / 2 pairs of a cell of k entries are dealt round the
+// cell: entry r takes partners r + d (mod k) for d = 1 .. (k - 1) / 2, and for even k the entries
+// r < k / 2 also take r + k / 2, so every lane of the cell runs (k - 1) / 2 or k / 2 pairs instead of
+// k - 1 - r.  Each pair is still evaluated with its lower entry as rod i, by the same expressions.
 // Lanes of a wave run the same trip count (the wave maximum) so emission stays wave-aggregated.
 // Found pairs are staged in LDS (an LDS counter per block) and reserved in the global list with one
 // atomic per block at the end; a block that fills its stage reserves the rest one by one.
 constexpr int kSweepT = 256;
-constexpr int kStage = 1024;
-constexpr int kSweepS = kSweepT + 64;  // entries whose rod data a block stages: its own and 64 beyond
+constexpr int kStage = 512;
+constexpr int kSweepPre = 64;                          // entries staged before the block's own
+constexpr int kSweepS = kSweepPre + kSweepT + 64;      // and after them
 
-// Rod data of one sweep entry, from the block's LDS stage or, past it, from global memory.
-struct SweepRod {
+// Box data of one sweep entry (rod id, box range start, inflated box), from the block's LDS stage
+// or, outside it, from global memory.
+struct SweepBox {
     uint32_t id;
     int cl[3];
     double lo[3], hi[3];
 };
 
-__global__ __launch_bounds__(kSweepT) void k_sweep_entries(RodsDev r, CfgDev c, const uint32_t *__restrict__ keys,
-                                                       int64_t E, const uint32_t *__restrict__ cend,
-                                                       const uint32_t *__restrict__ vals,
-                                                       const int32_t *__restrict__ clo,
-                                                       const double *__restrict__ blo,
-                                                       const double *__restrict__ bhi,
-                                                       uint32_t *__restrict__ pair_count, uint32_t cap,
-                                                       uint32_t *__restrict__ pI, uint32_t *__restrict__ pJ,
-                                                       uint32_t *__restrict__ pImg,
-                                                       uint32_t *__restrict__ cntI,
-                                                       uint32_t *__restrict__ cntJ,
-                                                       const uint32_t *__restrict__ Ed) {
+// four waves per SIMD (at most 128 VGPRs): measured faster than three with more registers, and than
+// five with spills
+__global__ __launch_bounds__(kSweepT) __attribute__((amdgpu_waves_per_eu(4)))
+void k_sweep_entries(RodsDev r, CfgDev c, const uint32_t *__restrict__ keys, int64_t E,
+                     const uint32_t *__restrict__ cstart, const uint32_t *__restrict__ cend,
+                     const uint32_t *__restrict__ vals, const int32_t *__restrict__ clo,
+                     const double *__restrict__ blo, const double *__restrict__ bhi,
+                     uint32_t *__restrict__ pair_count, uint32_t cap, uint32_t *__restrict__ pI,
+                     uint32_t *__restrict__ pJ, uint32_t *__restrict__ pImg, uint32_t *__restrict__ cntI,
+                     uint32_t *__restrict__ cntJ, const uint32_t *__restrict__ Ed) {
     E = dcount((uint32_t)E, Ed);
     __shared__ uint32_t s_n, s_base;
     __shared__ uint32_t s_pi[kStage], s_pj[kStage], s_pm[kStage];
-    // the rods of entries e0 .. e0 + kSweepS, structure of arrays: box range start, inflated box,
-    // centre, direction, collision half-length and radius
+    // box data of entries e0 - kSweepPre .. e0 + kSweepT + 64, structure of arrays
     __shared__ uint32_t s_id[kSweepS];
     __shared__ int s_cl[3][kSweepS];
-    __shared__ double s_lo[3][kSweepS], s_hi[3][kSweepS], s_c[3][kSweepS], s_t[3][kSweepS], s_h[kSweepS],
-        s_r[kSweepS];
+    __shared__ double s_lo[3][kSweepS], s_hi[3][kSweepS];
     const int lane = threadIdx.x & 63;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const int64_t e0 = (int64_t)blockIdx.x * blockDim.x;
+    const int64_t es = e0 - kSweepPre;  // first staged entry
     const int64_t e = e0 + threadIdx.x;
     for (int q = threadIdx.x; q < kSweepS; q += blockDim.x) {
-        if (e0 + q >= E) break;
-        const uint32_t j = vals[e0 + q];
+        const int64_t eq = es + q;
+        if (eq < 0) continue;
+        if (eq >= E) break;
+        const uint32_t j = vals[eq];
         s_id[q] = j;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             s_cl[k][q] = clo[3 * j + k];
             s_lo[k][q] = blo[3 * j + k];
             s_hi[k][q] = bhi[3 * j + k];
-            s_c[k][q] = r.center[3 * j + k];
-            s_t[k][q] = r.dir[3 * j + k];
         }
-        s_h[q] = r.hcol[j];
-        s_r[q] = r.rcol[j];
     }
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
-    uint32_t cnt = 0, i = 0, s1 = 0;
-    int cc[3] = {0, 0, 0}, ui[3] = {0, 0, 0};
-    double bhi_i[3] = {0, 0, 0}, bli[3] = {0, 0, 0}, ci[3] = {0, 0, 0}, ti[3] = {0, 0, 0};
-    double hi_ = 0.0, ri = 0.0;
+    auto box_of = [&](int64_t ex) {
+        SweepBox w;
+        const int64_t q = ex - es;
+        if (q >= 0 && q < kSweepS) {
+            w.id = s_id[q];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                w.cl[k] = s_cl[k][q];
+                w.lo[k] = s_lo[k][q];
+                w.hi[k] = s_hi[k][q];
+            }
+        } else {
+            w.id = vals[ex];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                w.cl[k] = clo[3 * w.id + k];
+                w.lo[k] = blo[3 * w.id + k];
+                w.hi[k] = bhi[3 * w.id + k];
+            }
+        }
+        return w;
+    };
+    // this lane's entry: cell, place r in the cell of kc entries, pair count
+    uint32_t cnt = 0, cs = 0, kc = 1, rr = 0;
+    int cc[3] = {0, 0, 0};
+    SweepBox own{0u, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     int pbc[3] = {0, 0, 0}, nc[3] = {1, 1, 1};
     double len[3] = {0, 0, 0}, blo0[3] = {0, 0, 0}, hcell[3] = {1, 1, 1};
     if (e < E) {
         const uint32_t cell = keys[e];
-        s1 = cend[cell];
-        cnt = s1 - (uint32_t)e - 1u;
-        const int q = threadIdx.x;
-        i = s_id[q];
-        const SysDev &g = r.sys[r.rsys[i]];
+        cs = cstart[cell];
+        kc = cend[cell] - cs;
+        rr = (uint32_t)e - cs;
+        cnt = (kc - 1u) / 2u + ((kc % 2u == 0u && rr < kc / 2u) ? 1u : 0u);
+        own = box_of(e);
+        const SysDev &g = r.sys[r.rsys[own.id]];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             pbc[k] = g.pbc[k];
@@ -602,19 +625,6 @@ __global__ __launch_bounds__(kSweepT) void k_sweep_entries(RodsDev r, CfgDev c, 
         cc[0] = (int)(lc % (uint32_t)nc[0]);
         cc[1] = (int)((lc / (uint32_t)nc[0]) % (uint32_t)nc[1]);
         cc[2] = (int)(lc / ((uint32_t)nc[0] * (uint32_t)nc[1]));
-        if (cnt) {
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const int cli = s_cl[k][q];
-                ui[k] = pbc[k] ? cli + pmod_i(cc[k] - cli, nc[k]) : cc[k];
-                bli[k] = s_lo[k][q];
-                bhi_i[k] = s_hi[k][q];
-                ci[k] = s_c[k][q];
-                ti[k] = s_t[k][q];
-            }
-            hi_ = s_h[q];
-            ri = s_r[q];
-        }
     }
     uint32_t maxc = cnt;
 #pragma unroll
@@ -624,49 +634,40 @@ __global__ __launch_bounds__(kSweepT) void k_sweep_entries(RodsDev r, CfgDev c, 
     }
     for (uint32_t t = 0; t < maxc; ++t) {
         bool pass = false;
-        uint32_t j = 0;
+        uint32_t i = 0, j = 0;
         int m[3] = {0, 0, 0};
         if (t < cnt) {
-            const int64_t ej = e + 1 + t;
-            const int qj = (int)(ej - e0);
-            const bool staged = qj < kSweepS;
-            SweepRod w;
-            if (staged) {
-                w.id = s_id[qj];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    w.cl[k] = s_cl[k][qj];
-                    w.lo[k] = s_lo[k][qj];
-                    w.hi[k] = s_hi[k][qj];
-                }
-            } else {
-                w.id = vals[ej];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    w.cl[k] = clo[3 * w.id + k];
-                    w.lo[k] = blo[3 * w.id + k];
-                    w.hi[k] = bhi[3 * w.id + k];
-                }
-            }
-            j = w.id;
+            uint32_t r2 = rr + 1u + t;
+            if (r2 >= kc) r2 -= kc;
+            const int64_t ep = (int64_t)cs + r2;
+            const SweepBox pw = box_of(ep);
+            // rod i is the pair's lower entry
+            const bool lo_own = e < ep;
+            const SweepBox &wi = lo_own ? own : pw;
+            const SweepBox &wj = lo_own ? pw : own;
+            i = wi.id;
+            j = wj.id;
+            int ui[3];
             bool ok = true;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                const int uj = pbc[k] ? w.cl[k] + pmod_i(cc[k] - w.cl[k], nc[k]) : cc[k];
+                ui[k] = pbc[k] ? wi.cl[k] + pmod_i(cc[k] - wi.cl[k], nc[k]) : cc[k];
+                const int uj = pbc[k] ? wj.cl[k] + pmod_i(cc[k] - wj.cl[k], nc[k]) : cc[k];
                 m[k] = pbc[k] ? (ui[k] - uj) / nc[k] : 0;
                 const double sh = pbc[k] ? (double)m[k] * len[k] : 0.0;
-                ok = ok && (w.lo[k] + sh <= bhi_i[k]) && (bli[k] <= w.hi[k] + sh);
+                ok = ok && (wj.lo[k] + sh <= wi.hi[k]) && (wi.lo[k] <= wj.hi[k] + sh);
             }
             if (ok) {
-                double cj[3], tj[3], hj, rj;
+                double ci[3], ti[3], cj[3], tj[3];
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    const double ck = staged ? s_c[k][qj] : r.center[3 * j + k];
+                    ci[k] = r.center[3 * i + k];
+                    ti[k] = r.dir[3 * i + k];
+                    const double ck = r.center[3 * j + k];
                     cj[k] = pbc[k] ? ck + (double)m[k] * len[k] : ck;
-                    tj[k] = staged ? s_t[k][qj] : r.dir[3 * j + k];
+                    tj[k] = r.dir[3 * j + k];
                 }
-                hj = staged ? s_h[qj] : r.hcol[j];
-                rj = staged ? s_r[qj] : r.rcol[j];
+                const double hi_ = r.hcol[i], ri = r.rcol[i], hj = r.hcol[j], rj = r.rcol[j];
                 const SegRes sg = segseg(ci[0], ci[1], ci[2], ti[0], ti[1], ti[2], hi_, cj[0], cj[1], cj[2], tj[0], tj[1],
                                          tj[2], hj);
                 const double d0 = (sg.dist - ri) - rj;
@@ -723,12 +724,12 @@ __global__ __launch_bounds__(kSweepT) void k_sweep_entries(RodsDev r, CfgDev c, 
 }
 
 HipErr launch_sweep(hipStream_t st, const RodsDev &r, const CfgDev &c, const uint32_t *keys, int64_t E,
-                    const uint32_t *cend, const uint32_t *vals, const int32_t *clo, const double *blo,
-                    const double *bhi, uint32_t *pair_count, uint32_t cap, uint32_t *pI, uint32_t *pJ,
-                    uint32_t *pImg, uint32_t *cntI, uint32_t *cntJ, const uint32_t *Ed) {
+                    const uint32_t *cstart, const uint32_t *cend, const uint32_t *vals, const int32_t *clo,
+                    const double *blo, const double *bhi, uint32_t *pair_count, uint32_t cap, uint32_t *pI,
+                    uint32_t *pJ, uint32_t *pImg, uint32_t *cntI, uint32_t *cntJ, const uint32_t *Ed) {
     if (E == 0) return hip_ok();
-    k_sweep_entries<<<nblk(E, kSweepT), kSweepT, 0, st>>>(r, c, keys, E, cend, vals, clo, blo, bhi, pair_count, cap, pI,
-                                                  pJ, pImg, cntI, cntJ, Ed);
+    k_sweep_entries<<<nblk(E, kSweepT), kSweepT, 0, st>>>(r, c, keys, E, cstart, cend, vals, clo, blo, bhi, pair_count,
+                                                          cap, pI, pJ, pImg, cntI, cntJ, Ed);
     LAUNCH_CHECK();
     return hip_ok();
 }
