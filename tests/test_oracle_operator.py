@@ -168,6 +168,38 @@ def test_euler_keeps_unit_directions_and_wraps():
     assert np.allclose(t2, rods["dir"], atol=1e-14)
 
 
+def test_euler_rotation_and_translation_closed_form():
+    """The direction update is the rotation by theta = 2 atan(|w| dt / 2) about w (the unit quaternion
+    (1, w dt / 2) / norm), checked against an independent Rodrigues rotation; to first order it is
+    t + dt w x t (a reversed sense or a wrong angle fails); the centre moves by exactly v dt, up to a
+    whole box length."""
+    rods, cfg = systems.small_3d(n=300, box=3.3, seed=16)
+    rng = np.random.default_rng(17)
+    u = rng.normal(size=(300, 6)) * np.array([5.0, 5.0, 5.0, 400.0, 400.0, 400.0])
+    oc = O.cfg_from_dict(cfg)
+    dt = oc.dt
+    c, t = O.euler(rods, u.reshape(-1), oc)
+    w = u[:, 3:]
+    wn = np.linalg.norm(w, axis=1)
+    axis = w / wn[:, None]
+    th = 2.0 * np.arctan(0.5 * dt * wn)
+    t0 = rods["dir"]
+    cross = np.cross(axis, t0)
+    dot = np.sum(axis * t0, axis=1)
+    rod = (t0 * np.cos(th)[:, None] + cross * np.sin(th)[:, None]
+           + axis * (dot * (1.0 - np.cos(th)))[:, None])
+    assert np.max(np.abs(t - rod)) < 1e-13
+    # first order in dt: t + dt w x t
+    small = O.cfg_from_dict(dict(cfg, dt=1e-8))
+    _, ts = O.euler(rods, u.reshape(-1), small)
+    lin = t0 + 1e-8 * np.cross(w, t0)
+    assert np.max(np.abs(ts - lin)) < 1e-9 and np.max(np.abs(ts - t0)) > 1e-6
+    # translation: c0 + v dt, wrapped into the periodic box
+    L = 3.3
+    d = c - (rods["center"] + dt * u[:, :3])
+    assert np.max(np.abs(d / L - np.round(d / L))) < 1e-12
+
+
 def test_single_contact_closed_form():
     """One contact: gamma = max(0, -b / a) with a = dt (n.Mt_I n + rI.Mr_I rI + n.Mt_J n + rJ.Mr_J rJ)."""
     rng = np.random.default_rng(8)
