@@ -151,6 +151,13 @@ def test_rows_follow_reference_block_conventions():
     assert np.allclose(rw["rI"].reshape(-1, 3), np.cross(rw["posI"].reshape(-1, 3), n), atol=1e-15)
     assert np.all(rw["delta0"] < 0.025)
     assert np.array_equal(rw["b"], rw["delta0"])  # zero U_nc => b = delta0
+    # b = delta0 + dt D^T U_nc: the gap predicted after dt of unconstrained motion, with D^T the
+    # operator that the finite-difference kinematic test pins (a dropped dt or sign fails here)
+    unc = np.random.default_rng(13).normal(size=6 * 800)
+    rw2 = O.rows(rods, oc, I, J, img, unc)
+    pred = rw2["delta0"] + oc.dt * O.apply_DT(I, J, rw2, unc)
+    assert np.allclose(rw2["b"], pred, rtol=0, atol=1e-15)
+    assert np.max(np.abs(rw2["b"] - rw2["delta0"])) > 1e-5
 
 
 def test_small_periodic_box_is_rejected():
