@@ -1382,8 +1382,9 @@ alens_status alens_step_euler(alens_ctx *ctx) {
     if (!ctx->have_step) return fail(ctx, ALENS_E_STATE, "step_euler before step_constraints");
     TRY_HIP(hipSetDevice(ctx->dev));
     const CfgDev cd = make_cfgdev(ctx);
+    // enqueued only: every later call that reads rods or steps runs on the same stream, and the
+    // calls that copy to the host wait for it
     TRY_K(launch_euler(ctx->st, ctx->n, ctx->rsys.p, ctx->sysd.p, ctx->center.p, ctx->dir.p, ctx->U.p, cd));
-    TRY_HIP(hipStreamSynchronize(ctx->st));
     ctx->step += 1;
     ctx->have_step = false;
     return ALENS_OK;

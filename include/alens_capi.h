@@ -164,7 +164,8 @@ alens_status alens_get_graph_info(const alens_ctx *ctx, alens_graph_info *info);
 alens_status alens_step_constraints(alens_ctx *ctx, const double *f_ext, alens_solve_report *rep);
 
 /* Move rods with the last step's velocity (stepEuler(), SRC/TubuleSystem.cpp:55) and advance the
- * step counter. */
+ * step counter.  The update is enqueued on the context's stream and the call returns without waiting;
+ * later calls on the context are ordered after it (those that copy to the host wait for it). */
 alens_status alens_step_euler(alens_ctx *ctx);
 
 /* Constraints of the last step: the collision rows in canonical order (ascending rod index I,
