@@ -1,0 +1,59 @@
+"""The benchmark workloads at their full sizes (the 200,000-rod monolayer slab and the 1,000,000-rod
+lognormal 3-D box), first step from the seeded start, checked through properties that need no
+oracle solve at that size:
+- the pair set, periodic images and every row field (delta0, normal, posI x n, posJ x n, b) equal
+  the oracle's cell-list pairs and rows bit for bit (rows from the GPU's U_nc, which the smaller
+  parity tests pin to the oracle's);
+- the returned gamma satisfies the KKT conditions of the BCQP to the solver tolerance, with the
+  gradient A gamma + b recomputed from the oracle's D, M and D^T;
+- the contact impulses balance: the forces of F_con = D gamma sum to zero over the rods.
+"""
+import numpy as np
+import pytest
+
+from alens_amd import capi, systems
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float64).reshape(-1).view(np.uint64)
+
+
+@pytest.mark.parametrize("kind", ["monolayer_slab", "lognormal_3d"])
+def test_full_size_pairs_rows_kkt_and_balance(kind):
+    rods, cfg = getattr(systems, kind)()
+    n = rods["length"].size
+    ctx = capi.Context(**cfg)
+    ctx.set_rods(rods)
+    perm = ctx.sort_rods()
+    rods = {k: (v[perm] if isinstance(v, np.ndarray) else v) for k, v in rods.items()}
+    rep = ctx.step()
+    assert rep["converged"] == 1 and rep["res"] <= cfg["tol"], rep
+    g = ctx.constraint_arrays()
+    vel = ctx.velocity()
+    gam = ctx.gamma()
+    m = g["I"].size
+    print(f"{kind}: {n} rods, {m} constraints, {rep['iters']} iterations", flush=True)
+
+    oc = O.cfg_from_dict(cfg)
+    I, J, img = O.pairs(rods, oc)
+    assert np.array_equal(g["I"], I) and np.array_equal(g["J"], J), "pair set differs from the oracle's"
+    assert np.array_equal(g["img"], img), "periodic images differ"
+    rw = O.rows(rods, oc, I, J, img, vel["unc"])
+    for f in ("delta0", "nrm", "rI", "rJ", "b"):
+        assert np.array_equal(_bits(g[f]), _bits(rw[f])), f"{f} differs from the oracle's rows"
+
+    # KKT of min 1/2 x'Ax + b'x, x >= 0: x >= 0, A x + b >= -tol, |min(x, A x + b)| <= tol
+    _, U = O.apply_D_M(n, I, J, rw, vel["mob"], oc, gam)
+    grad = oc.dt * O.apply_DT(I, J, rw, U) + rw["b"]
+    slack = cfg["tol"] * (1 + 1e-9)
+    assert np.all(gam >= 0.0)
+    assert grad.min() >= -slack, grad.min()
+    assert np.max(np.abs(np.minimum(gam, grad))) <= slack
+    assert np.count_nonzero(gam) > m // 10
+
+    # Newton's third law: every contact puts opposite forces on its two rods
+    f = vel["fcon"].reshape(-1, 6)[:, :3]
+    assert np.all(np.abs(f.sum(0)) <= 1e-9 * np.abs(f).sum(0))
