@@ -57,3 +57,46 @@ def test_full_size_pairs_rows_kkt_and_balance(kind):
     # Newton's third law: every contact puts opposite forces on its two rods
     f = vel["fcon"].reshape(-1, 6)[:, :3]
     assert np.all(np.abs(f.sum(0)) <= 1e-9 * np.abs(f).sum(0))
+
+
+def test_full_size_replicas_per_system():
+    """The 64-replica batch (1.13M rods, boxes and slabs of 2k-50k rods) in one context: for every
+    system, its pairs and rows equal the oracle's for that system alone, its gamma satisfies the KKT
+    conditions, its forces balance, and its own report says converged."""
+    parts, cfg = systems.replicas()
+    ctx = capi.Context(**cfg)
+    offs = ctx.set_systems(parts)
+    perm = ctx.sort_rods()  # within each system
+    cat = {k: np.concatenate([np.asarray(r[k]).reshape(len(r["length"]), -1) for r, _ in parts])
+           for k in ("gid", "center", "dir", "length", "radius")}
+    cat = {k: (v[perm].reshape(-1) if v.shape[1] == 1 else v[perm]) for k, v in cat.items()}
+    ctx.step()
+    g = ctx.constraint_arrays()
+    vel = ctx.velocity()
+    gam = ctx.gamma()
+    sreps = ctx.system_reports()
+    assert len(sreps) == 64
+    checked = 0
+    for s, (_, box) in enumerate(parts):
+        lo, hi = int(offs[s]), int(offs[s + 1])
+        assert sreps[s]["converged"] == 1 and sreps[s]["res"] <= cfg["tol"], (s, sreps[s])
+        rods = {k: v[lo:hi] for k, v in cat.items()}
+        rods["immovable"] = None
+        oc = O.cfg_from_dict(dict(cfg, **box))
+        I, J, img = O.pairs(rods, oc)
+        sel = (g["I"] >= lo) & (g["I"] < hi)
+        assert np.array_equal(g["I"][sel] - lo, I) and np.array_equal(g["J"][sel] - lo, J), f"system {s} pairs"
+        assert np.array_equal(g["img"][sel], img), f"system {s} images"
+        rw = O.rows(rods, oc, I, J, img, vel["unc"][6 * lo:6 * hi])
+        for f in ("delta0", "nrm", "rI", "rJ", "b"):
+            assert np.array_equal(_bits(g[f][sel]), _bits(rw[f])), f"system {s}: {f}"
+        n = hi - lo
+        _, U = O.apply_D_M(n, I, J, rw, vel["mob"][6 * lo:6 * hi], oc, gam[sel])
+        grad = oc.dt * O.apply_DT(I, J, rw, U) + rw["b"]
+        slack = cfg["tol"] * (1 + 1e-9)
+        assert np.all(gam[sel] >= 0.0) and grad.min() >= -slack, s
+        assert np.max(np.abs(np.minimum(gam[sel], grad))) <= slack, s
+        f = vel["fcon"].reshape(-1, 6)[lo:hi, :3]
+        assert np.all(np.abs(f.sum(0)) <= 1e-9 * np.abs(f).sum(0) + 1e-300), s
+        checked += I.size
+    assert checked == g["I"].size
