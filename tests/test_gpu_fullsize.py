@@ -21,11 +21,14 @@ def _bits(a):
     return np.ascontiguousarray(a, np.float64).reshape(-1).view(np.uint64)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("kind", ["monolayer_slab", "lognormal_3d"])
-def test_full_size_pairs_rows_kkt_and_balance(kind):
+def test_full_size_pairs_rows_kkt_and_balance(kind, precision):
+    """precision 1 is the FP32 form: FP32 iterations, then the FP64 residual check and FP64 refinement,
+    so its gamma must meet the same FP64 KKT conditions."""
     rods, cfg = getattr(systems, kind)()
     n = rods["length"].size
-    ctx = capi.Context(**cfg)
+    ctx = capi.Context(**dict(cfg, precision=precision))
     ctx.set_rods(rods)
     perm = ctx.sort_rods()
     rods = {k: (v[perm] if isinstance(v, np.ndarray) else v) for k, v in rods.items()}
@@ -35,7 +38,8 @@ def test_full_size_pairs_rows_kkt_and_balance(kind):
     vel = ctx.velocity()
     gam = ctx.gamma()
     m = g["I"].size
-    print(f"{kind}: {n} rods, {m} constraints, {rep['iters']} iterations", flush=True)
+    print(f"{kind} precision {precision}: {n} rods, {m} constraints, {rep['iters']} FP64 and "
+          f"{rep['iters_fp32']} FP32 iterations", flush=True)
 
     oc = O.cfg_from_dict(cfg)
     I, J, img = O.pairs(rods, oc)
