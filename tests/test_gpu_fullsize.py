@@ -6,6 +6,8 @@ oracle solve at that size:
   parity tests pin to the oracle's);
 - the returned gamma satisfies the KKT conditions of the BCQP to the solver tolerance, with the
   gradient A gamma + b recomputed from the oracle's D, M and D^T;
+- the end-of-step separations delta0 + dt D^T U from the GPU's own write-back U are non-negative to
+  the tolerance, and zero where gamma > 0;
 - the contact impulses balance: the forces of F_con = D gamma sum to zero over the rods.
 """
 import numpy as np
@@ -57,6 +59,14 @@ def test_full_size_pairs_rows_kkt_and_balance(kind, precision):
     assert grad.min() >= -slack, grad.min()
     assert np.max(np.abs(np.minimum(gam, grad))) <= slack
     assert np.count_nonzero(gam) > m // 10
+
+    # the write-back: with the GPU's own U = U_nc + M D gamma, the separations at the end of the
+    # step, delta0 + dt D^T U, are the gradient above: non-negative to the tolerance, and zero (to
+    # the tolerance) where gamma > 0
+    sep = rw["delta0"] + oc.dt * O.apply_DT(I, J, rw, vel["u"])
+    assert np.max(np.abs(sep - grad)) <= 1e-9 * (1.0 + np.max(np.abs(grad)))
+    assert sep.min() >= -slack
+    assert np.max(np.abs(sep[gam > 0])) <= slack
 
     # Newton's third law: every contact puts opposite forces on its two rods
     f = vel["fcon"].reshape(-1, 6)[:, :3]
