@@ -99,25 +99,23 @@ def load(path=DEFAULT_MODEL):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=DEFAULT_MODEL)
-    ap.add_argument("--pairs", type=int, default=2_000_000, help="training pairs to collect")
+    ap.add_argument("--pairs", type=int, default=4_000_000, help="training pairs (a random subset)")
     ap.add_argument("--epochs", type=int, default=12)
     args = ap.parse_args()
     t0 = time.time()
+    # early and relaxed states: the small 3-D system, the dense slab right after its random start
+    # (steps 2-3) and near steady state (steps 40-47), and a 20k-rod slab over steps 10-39
+    runs = (("small_3d", 1000, 80, 11, 2), ("monolayer_slab", 200_000, 4, 1002, 2),
+            ("monolayer_slab", 200_000, 48, 1003, 40), ("monolayer_slab", 20_000, 40, 1004, 10))
     Xs, gs = [], []
-    # the small 3-D system (many short runs) and the dense monolayer
-    X1, g1 = collect("small_3d", 1000, 60, seed=11, skip=2)
-    Xs.append(X1)
-    gs.append(g1)
-    have = len(g1)
-    steps = 2
-    while have < args.pairs:
-        X2, g2 = collect("monolayer_slab", 200_000, steps + 2, seed=1000 + steps, skip=2)
-        Xs.append(X2)
-        gs.append(g2)
-        have += len(g2)
-        steps += 1
-    X = np.concatenate(Xs)[: args.pairs]
-    g = np.concatenate(gs)[: args.pairs]
+    for kind, n, steps, seed, skip in runs:
+        X, g = collect(kind, n, steps, seed=seed, skip=skip)
+        Xs.append(X)
+        gs.append(g)
+    X = np.concatenate(Xs)
+    g = np.concatenate(gs)
+    sel = np.random.default_rng(5).permutation(len(g))[: args.pairs]
+    X, g = X[sel], g[sel]
     print(f"collected {len(g)} pairs in {time.time() - t0:.1f} s", flush=True)
     P, norm, losses = train(X, g, epochs=args.epochs)
     print("loss per epoch", np.round(losses, 5).tolist(), flush=True)
