@@ -99,14 +99,16 @@ def load(path=DEFAULT_MODEL):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=DEFAULT_MODEL)
-    ap.add_argument("--pairs", type=int, default=4_000_000, help="training pairs (a random subset)")
+    ap.add_argument("--pairs", type=int, default=6_000_000, help="training pairs (a random subset)")
     ap.add_argument("--epochs", type=int, default=12)
     args = ap.parse_args()
     t0 = time.time()
     # early and relaxed states: the small 3-D system, the dense slab right after its random start
-    # (steps 2-3) and near steady state (steps 40-47), and a 20k-rod slab over steps 10-39
+    # (steps 2-3) and near steady state (steps 40-47), a 20k-rod slab over steps 10-39, and
+    # lognormal 3-D boxes of 200k rods (steps 40-47) and 1M rods (steps 40-43)
     runs = (("small_3d", 1000, 80, 11, 2), ("monolayer_slab", 200_000, 4, 1002, 2),
-            ("monolayer_slab", 200_000, 48, 1003, 40), ("monolayer_slab", 20_000, 40, 1004, 10))
+            ("monolayer_slab", 200_000, 48, 1003, 40), ("monolayer_slab", 20_000, 40, 1004, 10),
+            ("lognormal_3d", 200_000, 48, 1005, 40), ("lognormal_3d", 1_000_000, 44, 1006, 40))
     Xs, gs = [], []
     for kind, n, steps, seed, skip in runs:
         X, g = collect(kind, n, steps, seed=seed, skip=skip)

@@ -63,7 +63,7 @@ def test_gpu_training_fits_a_smooth_target():
 
 def test_learned_warm_start_reduces_iterations():
     """GPU training on the solver output of one monolayer run must lower the loss; the shipped network
-    (trained once on 4M contact pairs from early and relaxed states) is then evaluated on another run (same rod states fed to cold,
+    (trained once on 6M contact pairs from early and relaxed states of slabs and 3-D boxes) is then evaluated on another run (same rod states fed to cold,
     previous-gamma and network warm starts) and must beat the cold start.  Training with split-K
     atomics is not bit-reproducible, so the gain is asserted on the fixed shipped weights; the network
     trained here is evaluated and its ratio printed."""
