@@ -26,7 +26,7 @@ EXPORTS = [
     "alens_get_velocity", "alens_bcqp_solve", "alens_add_constraints", "alens_clear_constraints",
     "alens_get_con_stress", "alens_autotune", "alens_set_graph", "alens_get_graph_info",
     "alens_mlp_set", "alens_mlp_get", "alens_get_features", "alens_mlp_predict", "alens_mlp_gradient",
-    "alens_mlp_train",
+    "alens_mlp_train", "alens_copy_probe",
 ]
 MLP_PARAMS = 9473
 MLP_NORM = 25
@@ -158,12 +158,24 @@ def lib():
     L.alens_mlp_gradient.argtypes = [vp, ctypes.c_int64, _fp, _fp, _fp, _fp]
     L.alens_mlp_train.argtypes = [vp, ctypes.c_int64, _fp, _fp, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
                                   ctypes.c_uint64, _fp]
+    L.alens_copy_probe.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, _dp]
     for name in EXPORTS:
         fn = getattr(L, name)
         if name not in ("alens_destroy", "alens_last_error", "alens_version", "alens_get_step"):
             fn.restype = ctypes.c_int
     _lib = L
     return L
+
+
+def copy_probe(nbytes=1 << 30, reps=10, device=0):
+    """Read + write GB/s of a device-to-device copy kernel (alens_copy_probe): the practical HBM
+    ceiling beside the published peak.  One repetition of a known size also serves as the profiler
+    calibration kernel (k_copy_probe)."""
+    g = ctypes.c_double()
+    st = lib().alens_copy_probe(int(device), int(nbytes), int(reps), ctypes.byref(g))
+    if st != ALENS_OK:
+        raise AlensError(st, "alens_copy_probe failed")
+    return g.value
 
 
 def _p(a, ct):

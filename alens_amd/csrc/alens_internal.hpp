@@ -24,11 +24,29 @@ struct HipErr {
 };
 inline HipErr hip_ok() { return HipErr{}; }
 
-// growable device buffer (contents not preserved on growth)
+// growable device buffer (contents not preserved on growth); freed when it goes out of scope
 template <typename T>
 struct DBuf {
     T *p = nullptr;
     size_t cap = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : p(o.p), cap(o.cap) {
+        o.p = nullptr;
+        o.cap = 0;
+    }
+    DBuf &operator=(DBuf &&o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            cap = o.cap;
+            o.p = nullptr;
+            o.cap = 0;
+        }
+        return *this;
+    }
+    ~DBuf() { release(); }
     hipError_t ensure(size_t n) {
         if (n <= cap && p) return hipSuccess;
         if (p) {

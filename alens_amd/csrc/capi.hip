@@ -592,6 +592,7 @@ static alens_status parse_reports(alens_ctx *ctx, const double *o, uint32_t abor
         r.status = (int32_t)w[4];
         r.res0 = w[5];
         r.res = w[6];
+        r.n_con = (int64_t)w[7];
         if (per_sys) (*per_sys)[sy] = r;
         rep->iters = std::max(rep->iters, r.iters);
         rep->matvecs = std::max(rep->matvecs, r.matvecs);
@@ -624,10 +625,22 @@ static alens_status run_bbpgd(alens_ctx *ctx, SolveArgsT<T> a, alens_solve_repor
     TRY_HIP(ctx->part.ensure((size_t)kRedQ * std::max<int32_t>(ctx->nchunks_cur, 1)));
     TRY_HIP(ctx->out.ensure(8 * (size_t)ns + 8));
     TRY_HIP(ctx->out32.ensure(8 * (size_t)ns + 8));
-    if (ctx->capture && ctx->bar_clean)
+    if (ctx->capture && ctx->bar_clean) {
         ctx->bar_clean = false;  // the captured step zeroed them with its other counters
-    else
+    } else if (ctx->capture) {
+        // a later launch of the same captured step (the FP64 refinement after the FP32 solve): zero
+        // every word but the abort word, so an abort of the earlier launch reaches the step's tail
+        // (and makes this launch leave at once instead of refining a partial iterate)
+        ZeroJobs z{};
+        z.p[0] = ctx->bar.p;
+        z.n[0] = kBarWords;
+        z.p[1] = ctx->bar.p + kBarWords + 1;
+        z.n[1] = kBarTotalWords - kBarWords - 1;
+        z.count = 2;
+        TRY_K(launch_zero_many(ctx->st, z));
+    } else {
         TRY_K(launch_fill_u32(ctx->st, ctx->bar.p, 0u, kBarTotalWords));
+    }
     double *out = sizeof(T) == 8 ? ctx->out.p : ctx->out32.p;
     a.part = ctx->part.p;
     a.bar = ctx->bar.p;
@@ -1957,6 +1970,54 @@ alens_status alens_mlp_train(alens_ctx *ctx, int64_t m, const float *X, const fl
     for (auto *b : {&dX, &dT, &dXB, &dTB, &dG, &dM1, &dM2, &dL}) b->release();
     dIdx.release();
     return ALENS_OK;
+}
+
+}  // extern "C"
+
+// Streaming copy used by alens_copy_probe: each thread moves 16-byte words, grid-stride.
+__global__ void k_copy_probe(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t nvec) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvec; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+__global__ void k_fill_probe(uint4 *__restrict__ dst, size_t nvec) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvec; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = make_uint4((uint32_t)i, 1u, 2u, 3u);
+}
+
+extern "C" {
+
+alens_status alens_copy_probe(int32_t device, int64_t nbytes, int32_t reps, double *gbs) {
+    if (nbytes < 16 || reps < 1 || !gbs) return ALENS_E_ARG;
+    *gbs = 0.0;
+    const size_t nvec = (size_t)nbytes / 16;
+    if (hipSetDevice(device) != hipSuccess) return ALENS_E_HIP;
+    uint4 *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    alens_status rc = ALENS_E_HIP;
+    float ms = 0.f;
+    const unsigned grid = 8192, blk = 256;
+    if (hipMalloc((void **)&a, nvec * 16) != hipSuccess || hipMalloc((void **)&b, nvec * 16) != hipSuccess) {
+        rc = ALENS_E_OOM;
+        goto done;
+    }
+    if (hipStreamCreate(&st) != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        goto done;
+    k_fill_probe<<<grid, blk, 0, st>>>(a, nvec);
+    if (hipGetLastError() != hipSuccess || hipEventRecord(e0, st) != hipSuccess) goto done;
+    for (int32_t r = 0; r < reps; ++r) k_copy_probe<<<grid, blk, 0, st>>>(a, b, nvec);
+    if (hipGetLastError() != hipSuccess || hipEventRecord(e1, st) != hipSuccess) goto done;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) goto done;
+    *gbs = ms > 0.f ? 2.0 * (double)(nvec * 16) * reps / (ms * 1e-3) / 1e9 : 0.0;
+    rc = ALENS_OK;
+done:
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
 }
 
 alens_status alens_get_num_systems(const alens_ctx *ctx, int32_t *nsys) {

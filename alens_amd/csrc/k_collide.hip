@@ -314,10 +314,7 @@ HipErr ensure_radix_scratch(DBuf<uint32_t> &hist, ScanScratch &s, int64_t n);
 
 HipErr exclusive_scan_u32(hipStream_t st, const uint32_t *in, uint32_t *out, int64_t n,
                           ScanScratch &s) {
-    if (n == 0) {
-        ALENS_HIP_TRY(hipMemsetAsync(out, 0, sizeof(uint32_t), st));
-        return hip_ok();
-    }
+    if (n == 0) return launch_fill_u32(st, out, 0u, 1);  // a kernel, not a memset node, under capture
     return scan_level(st, in, out, n, s, 0);
 }
 

@@ -855,7 +855,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
             o[4] = st == S_CONVERGED ? 0.0 : (st == S_NONFINITE ? (double)ALENS_E_NONFINITE : (double)ALENS_E_NOCONV);
             o[5] = s_res0[q];
             o[6] = s_res[q];
-            o[7] = s_alpha[q];
+            o[7] = (double)(a.con_off[q + 1] - a.con_off[q]);  // the system's rows
         }
     }
 }

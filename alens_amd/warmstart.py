@@ -46,10 +46,12 @@ def train(X, gamma, epochs=12, batch=4096, lr=2e-3, seed=1):
     return P, norm, losses
 
 
-def evaluate(kind, n, steps, P, norm, seed=99, warmup=2, precision=0, forms=None):
+def evaluate(kind, n, steps, P, norm, seed=99, warmup=2, precision=0, forms=None, lead=None):
     """Same rod states fed to several contexts at every step (a plain FP64 driver moves the rods);
     returns each context's per-step BBPGD iterations (FP32 + FP64) and its warm-start fallbacks.
-    forms: name -> (warm_start, precision); default cold / previous gamma / network at `precision`."""
+    forms: name -> (warm_start, precision); default cold / previous gamma / network at `precision`.
+    lead: the forms start stepping this many steps before the measured ones (they need a previous
+    step for their warm-start keys); None = from the first step."""
     if forms is None:
         forms = {"cold": (0, precision), "prev": (1, precision), "mlp": (2, precision)}
     rods, cfg = systems.scaled(kind, n, seed=seed) if kind != "small_3d" else systems.small_3d(seed=seed)
@@ -67,10 +69,11 @@ def evaluate(kind, n, steps, P, norm, seed=99, warmup=2, precision=0, forms=None
     out = {k: [] for k in ctxs}
     for name in ctxs:
         out["fallback_" + name] = 0
+    first = 0 if lead is None else max(0, warmup - lead)
     for s in range(warmup + steps):
         c, t = driver.get_rods()
         state = dict(rods, center=c, dir=t)
-        for name, ctx in ctxs.items():
+        for name, ctx in (ctxs.items() if s >= first else ()):
             ctx.set_rods(state)
             ctx.set_step(s)
             r = ctx.step()

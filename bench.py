@@ -2,12 +2,14 @@
 
 A step is one pass of the hot path over the device-resident rod system: U_nc (mobility + Brownian),
 cell list + pair sweep + rows, persistent BBPGD to residual tol, write-back, Euler move.
-Default workload: the dense quasi-2D monolayer (200,000 rods, L = 1 um, 100 x 100 um slab, nematic
-von Mises kappa = 4), FP64 plain BBPGD, each step one replay of a captured HIP graph.  Beside it: the
-chosen form (FP32 + FP64 refinement, learned warm start) with its accuracy and same-state iteration
-counts, and the FP64 oracle on the host cores for the same step.  Prints one JSON line.
+Default workload: the 1,000,000-rod lognormal 3-D box (lengths lognormal(ln 1 um, 0.3) resampled
+above 3 um, periodic 25^3 um^3, isotropic), FP64 plain BBPGD, each step one replay of a captured HIP
+graph.  Beside it: the chosen form (FP32 + FP64 refinement, learned warm start) with its accuracy and
+same-state iteration counts, and the FP64 oracle on the host cores for the same step, with gamma and
+U compared bit for bit.  Prints one JSON line.  The library is driven through its C ABI only (no
+torch in the process, so profiled runs load one HIP runtime).
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload monolayer_slab|lognormal_3d|small_3d|replicas]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload lognormal_3d|monolayer_slab|small_3d|replicas]
                   [--precision 0|1] [--graph 1|0] [--no-cpu] [--no-form] [--no-autotune]
 """
 import argparse
@@ -23,6 +25,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+SPEEDUP_TARGET = 20.0  # one 1M-rod GPU step against the host-CPU restatement of the reference path
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "k_bbpgd_traffic.json")
 
 
 def bytes_per_matvec(m, n, w=8):
@@ -33,14 +37,60 @@ def bytes_per_matvec(m, n, w=8):
 
 
 def cpu_model():
+    """CPU model string plus family/model numbers from /proc/cpuinfo."""
+    info = {}
     try:
         with open("/proc/cpuinfo") as fh:
             for line in fh:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k in ("model name", "cpu family", "model", "stepping") and k not in info:
+                    info[k] = v.strip()
     except OSError:
         pass
-    return platform.processor() or "unknown"
+    name = info.get("model name") or platform.processor() or "unknown"
+    return f"{name} (family {info.get('cpu family', '?')}, model {info.get('model', '?')})"
+
+
+def cpu_topology():
+    """Logical CPUs this process may run on, and the physical cores behind them (distinct (package,
+    core) pairs from sysfs; equal to the logical count when SMT siblings are not both visible)."""
+    cpus = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in cpus:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+        try:
+            with open(base + "physical_package_id") as f1, open(base + "core_id") as f2:
+                cores.add((f1.read().strip(), f2.read().strip()))
+        except OSError:
+            cores.add(("?", str(c)))
+    return len(cpus), len(cores)
+
+
+def setup_cpu_threads(requested=0):
+    """OpenMP placement for the CPU leg, set before the oracle's OpenMP runtime loads: one thread per
+    physical core, spread over the cores (OMP_PROC_BIND=spread, OMP_PLACES=cores)."""
+    logical, physical = cpu_topology()
+    threads = requested or physical
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ["OMP_PROC_BIND"] = "spread"
+    os.environ["OMP_PLACES"] = "cores"
+    return threads, logical, physical
+
+
+def same_bits(a, b):
+    a = np.ascontiguousarray(a, np.float64).reshape(-1)
+    b = np.ascontiguousarray(b, np.float64).reshape(-1)
+    return a.size == b.size and np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+def load_traffic(workload):
+    """Measured solver traffic of a workload (rocprofv3 FETCH_SIZE + WRITE_SIZE passes), or None."""
+    if not os.path.exists(TRAFFIC_JSON):
+        return None
+    with open(TRAFFIC_JSON) as fh:
+        tr = json.load(fh)
+    return tr.get(workload)
 
 
 def main():
@@ -48,8 +98,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="monolayer_slab",
-                    choices=["monolayer_slab", "lognormal_3d", "small_3d", "replicas"])
+    ap.add_argument("--workload", default="lognormal_3d",
+                    choices=["lognormal_3d", "monolayer_slab", "small_3d", "replicas"])
     ap.add_argument("--precision", type=int, default=0, help="1: FP32 BBPGD + FP64 refinement for the timed pass")
     ap.add_argument("--n", type=int, default=0, help="rod count (default: the workload's own)")
     ap.add_argument("--cell-size", type=float, default=0.0,
@@ -68,7 +118,8 @@ def main():
     ap.add_argument("--rocprof-summary", nargs=3, metavar=("TRACE_DIR", "FETCH_DIR", "WRITE_DIR"),
                     help="summarise rocprofv3 output of a bench run made with the same --warmup / --steps "
                          "(--no-form --no-cpu --calib-copy-mib): prints markdown, writes --traffic-out")
-    ap.add_argument("--traffic-out", default=os.path.join(ROOT, "profiles", "k_bbpgd_traffic.json"))
+    ap.add_argument("--traffic-out", default=TRAFFIC_JSON,
+                    help="JSON of measured solver traffic, one record per workload (merged)")
     args = ap.parse_args()
     if args.rocprof_summary:
         return rocprof_summary(args)
@@ -80,16 +131,11 @@ def main():
     from alens_amd import capi, systems
 
     if args.calib_copy_mib:
-        import torch
-
-        nb = args.calib_copy_mib << 20
-        src = torch.empty(nb, dtype=torch.uint8, device="cuda").fill_(1)
-        dst = torch.empty_like(src)
-        torch.add(src, 1, out=dst)  # elementwise: reads nb bytes, writes nb bytes
-        torch.cuda.synchronize()
-        del src, dst
+        # one k_copy_probe launch of known traffic (reads and writes calib_copy_mib MiB) for scaling
+        # the profiler's FETCH_SIZE / WRITE_SIZE counters
+        capi.copy_probe(args.calib_copy_mib << 20, reps=1)
     # (skipped in profiled runs: its copy kernels would sit beside the calibration copy)
-    copy_gbs = None if args.calib_copy_mib else stream_copy_gbs()
+    copy_gbs = None if args.calib_copy_mib else capi.copy_probe(1 << 30, reps=10)
     if args.workload == "replicas":
         return bench_replicas(args, capi, systems, copy_gbs)
     gen = {"monolayer_slab": systems.monolayer_slab, "lognormal_3d": systems.lognormal_3d,
@@ -184,16 +230,14 @@ def main():
     }
     if args.graph and out["graph"]["replays"] == 0:
         print(f"graph mode did not replay: {out['graph']}", file=sys.stderr, flush=True)
-    # measured HBM traffic per launch: bytes per matvec from the committed rocprofv3 FETCH_SIZE /
-    # WRITE_SIZE passes on this workload, times this run's matvecs per launch
-    tp = os.path.join(ROOT, "profiles", "k_bbpgd_traffic.json")
-    if os.path.exists(tp) and not args.n and args.precision == 0:
-        with open(tp) as fh:
-            tr = json.load(fh)
-        if tr.get("workload", "").split(":")[0] == args.workload:
-            out["roofline"]["traffic"] = tr["traffic_bytes_per_matvec"] * float(mv.mean())
-            out["roofline"]["traffic_source"] = ("profiles/k_bbpgd_traffic.json: rocprofv3 FETCH_SIZE + WRITE_SIZE "
-                                                 f"per matvec ({tr['traffic_over_algorithmic']:.2f} x algorithmic)")
+    # measured HBM traffic per launch: the ratio of measured to algorithmic bytes from the committed
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE passes on this workload, times this run's algorithmic bytes
+    tr = load_traffic(args.workload) if (not args.n and args.precision == 0) else None
+    if tr:
+        out["roofline"]["traffic"] = tr["traffic_over_algorithmic"] * float(alg / K)
+        out["roofline"]["traffic_source"] = (
+            f"profiles/k_bbpgd_traffic.json[{args.workload}]: rocprofv3 FETCH_SIZE + WRITE_SIZE of the solver, "
+            f"{tr['traffic_over_algorithmic']:.2f} x its algorithmic bytes, times this run's algorithmic bytes")
 
     # chosen form on the same workload: FP32 iterations + FP64 residual/refinement, started from the
     # learned network (the shipped weights); plain BBPGD finishes every solve
@@ -238,10 +282,11 @@ def main():
         # previous-step impulses): plain FP64 against the FP32 form with each warm start
         ev = W.evaluate(args.workload, n, 4, P, norm, seed=cfg["seed"], warmup=40,
                         forms={"plain_fp64": (0, 0), "fp32_cold": (0, 1), "fp32_prev_gamma": (1, 1),
-                               "fp32_network": (2, 1), "fp64_network": (2, 0)})
+                               "fp32_network": (2, 1), "fp64_network": (2, 0)}, lead=2)
         pm = float(ev["plain_fp64"].mean())
         out["chosen_form"]["same_state_iterations"] = {
-            "states": "4 steps after 40 warm-up from the workload's seeded start",
+            "states": "4 steps after 40 warm-up steps of a plain FP64 driver from the workload's seeded start "
+                      "(each form also steps the 2 states before them, for its previous-step keys)",
             **{k: float(ev[k].mean()) for k in ("plain_fp64", "fp32_cold", "fp32_prev_gamma", "fp32_network",
                                                 "fp64_network")},
             "fallbacks_fp32_network": int(ev["fallback_fp32_network"]),
@@ -253,6 +298,7 @@ def main():
         c, t = fctx.get_rods()
         st = fctx.step_count
         fctx.close()
+        del fctx
         sols = {}
         for name, kw in (("fp32_form", dict(precision=1)), ("plain_fp64", {}),
                          ("tight_fp64", dict(tol=1e-11, max_ite=2_000_000))):
@@ -275,9 +321,9 @@ def main():
 
     if not args.no_cpu:
         # the same step on both sides: from the post-benchmark state, one more GPU step (timed alone)
-        # and the FP64 oracle step on the host cores; the two are bit-identical, iterations included
+        # and the FP64 oracle step on the host cores; gamma and U are compared bit for bit
+        threads, logical, physical = setup_cpu_threads(args.cpu_threads)
         from oracle import oracle as O
-        threads = args.cpu_threads or len(os.sched_getaffinity(0))
         O.set_threads(threads)
         c, t = ctx.get_rods()
         state = dict(rods, center=c, dir=t)
@@ -286,22 +332,30 @@ def main():
         g0 = time.perf_counter()
         grep = ctx.step()
         tg = time.perf_counter() - g0
+        g_gpu, u_gpu = ctx.gamma(), ctx.velocity()["u"]
         ts = time.perf_counter()
         res = O.step(state, oc)
         tc = time.perf_counter() - ts
-        same = res["report"]["iters"] == grep["iters"] and res["I"].size == grep["n_con"]
+        same = (res["report"]["iters"] == grep["iters"] and res["I"].size == grep["n_con"]
+                and same_bits(g_gpu, res["gamma"]) and same_bits(u_gpu, res["u"]))
         out["cpu_baseline"] = {
             "value": n / tc,
             "unit": "rod-steps/s",
             "cores": threads,
             "kind": "port",
-            "sample": f"1 full step (FP64 oracle, OpenMP) from the post-benchmark state: {res['I'].size} constraints, "
-                      f"{res['report']['iters']} BBPGD iterations, {tc:.3f} s; the GPU took {tg * 1e3:.2f} ms for the "
-                      f"same step ({grep['iters']} iterations{', bit-identical' if same else ', MISMATCH'}); "
-                      f"CPU {cpu_model()}",
+            "sample": f"1 full step (FP64 oracle, OpenMP, {threads} threads, OMP_PROC_BIND=spread OMP_PLACES=cores; "
+                      f"{logical} logical CPUs = {physical} physical cores visible) from the post-benchmark state: "
+                      f"{res['I'].size} constraints, {res['report']['iters']} BBPGD iterations, {tc:.3f} s; the GPU "
+                      f"took {tg * 1e3:.2f} ms for the same step ({grep['iters']} iterations; gamma and U "
+                      f"{'bit-identical' if same else 'MISMATCH'}); CPU {cpu_model()}",
+            "same_step_bit_identical": bool(same),
+            "logical_cpus": logical,
+            "physical_cores": physical,
         }
         out["gpu_over_cpu"] = value / (n / tc)
         out["gpu_over_cpu_same_step"] = tc / tg
+        out["gpu_over_cpu_target"] = SPEEDUP_TARGET
+    ctx.close()
     print(json.dumps(out), flush=True)
 
 
@@ -343,8 +397,8 @@ def rocprof_summary(args):
     for key, d, cname in (("fetch", fdir, "FETCH_SIZE"), ("write", wdir, "WRITE_SIZE")):
         rows = [r for r in _csv(os.path.join(d, "run_counter_collection.csv")) if r["Counter_Name"] == cname]
         rows.sort(key=lambda r: int(r["Dispatch_Id"]))
-        # calibration: the second elementwise kernel is the --calib-copy-mib add (reads N, writes N bytes)
-        calib = [r for r in rows if "elementwise" in r["Kernel_Name"]][1]
+        # calibration: the --calib-copy-mib launch of k_copy_probe (reads N, writes N bytes)
+        calib = [r for r in rows if "k_copy_probe" in r["Kernel_Name"]][0]
         true_bytes = float(args.calib_copy_mib) * (1 << 20)
         factor = true_bytes / (float(calib["Counter_Value"]) * 1024.0)
         solves = [r for r in rows if "k_bbpgd" in r["Kernel_Name"]][-args.steps:]
@@ -356,7 +410,8 @@ def rocprof_summary(args):
     mv = js["matvecs_per_step"]
     alg = [bytes_per_matvec(int(m), int(js["config"]["rods"])) for m in js["constraints_per_timed_step"]]
     per_mv = [(f + w) / v for f, w, v in zip(out["fetch"]["bytes"], out["write"]["bytes"], mv)]
-    res = dict(kernel="k_bbpgd<double>", workload=js["config"]["workload"],
+    res = dict(kernel="k_bbpgd<double>", workload=js["config"]["workload"], rods=int(js["config"]["rods"]),
+               constraints_per_timed_step=js["constraints_per_timed_step"], matvecs_per_timed_step=mv,
                fetch_factor=out["fetch"]["factor"], write_factor=out["write"]["factor"],
                traffic_bytes_per_matvec=float(np.mean(per_mv)),
                algorithmic_bytes_per_matvec=float(np.mean(alg)),
@@ -369,40 +424,22 @@ def rocprof_summary(args):
     print(f"| measured bytes per matvec (fetch + write, scaled) | {res['traffic_bytes_per_matvec'] / 1e6:.1f} MB |")
     print(f"| algorithmic bytes per matvec (200 M + 144 N) | {res['algorithmic_bytes_per_matvec'] / 1e6:.1f} MB |")
     print(f"| measured / algorithmic | {res['traffic_over_algorithmic']:.2f} |")
+    allw = {}
+    if os.path.exists(args.traffic_out):
+        with open(args.traffic_out) as fh:
+            allw = json.load(fh)
+    allw[res["workload"].split(":")[0]] = res
     with open(args.traffic_out, "w") as fh:
-        json.dump(res, fh, indent=1)
+        json.dump(allw, fh, indent=1)
 
 
-def stream_copy_gbs(nbytes=1 << 30, reps=10):
-    """Device-to-device copy bandwidth on this GPU (read + write bytes per second, GB/s): the
-    practical HBM ceiling beside the published 8 TB/s."""
-    try:
-        import torch
-    except ImportError:
-        return None
-    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
-    dst = torch.empty_like(src)
-    dst.copy_(src)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record()
-    for _ in range(reps):
-        dst.copy_(src)
-    e1.record()
-    torch.cuda.synchronize()
-    gbs = 2.0 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
-    del src, dst
-    torch.cuda.empty_cache()
-    return gbs
-
-
-def _run_replicas(args, capi, parts, cfg, precision, warm_start, P=None, norm=None):
+def _run_replicas(args, capi, parts, cfg, precision, warm_start, P=None, norm=None, keep=False):
     ctx = capi.Context(**dict(cfg, precision=precision, warm_start=warm_start))
     if warm_start == 2:
         ctx.mlp_set(P, norm)
     offs = ctx.set_systems(parts)
     ctx.set_graph(bool(args.graph))
-    ctx.sort_rods()
+    perm = ctx.sort_rods()
     n = int(offs[-1])
     for w in range(args.warmup):
         ctx.step()
@@ -423,11 +460,24 @@ def _run_replicas(args, capi, parts, cfg, precision, warm_start, P=None, norm=No
     el = time.perf_counter() - t0
     it = np.array([[r["iters"] + r["iters_fp32"] for r in sr] for sr in sreps])  # steps x replicas
     hist, edges = np.histogram(it.mean(0), bins=[0, 25, 50, 100, 200, 400, 800, 1600, 1e9])
-    res = dict(n=n, elapsed=el, reps=reps, tune=tune, graph=ctx.graph_info() if args.graph else None,
+    # algorithmic solver bytes: every system moves its own rows and rods once per matvec
+    nsys_rods = np.diff(offs)
+    w = 4 if precision else 8
+    alg = 0.0
+    for sr in sreps:
+        for q, r in enumerate(sr):
+            alg += r["matvecs"] * bytes_per_matvec(r["n_con"], int(nsys_rods[q]))
+            if precision:
+                alg += r["iters_fp32"] * bytes_per_matvec(r["n_con"], int(nsys_rods[q]), w=4)
+    res = dict(n=n, elapsed=el, reps=reps, sreps=sreps, tune=tune, graph=ctx.graph_info() if args.graph else None,
                iters_mean_per_replica=float(it.mean()),
                hist={"edges": [float(e) for e in edges[:-1]], "counts": hist.tolist()},
-               fallbacks=int(sum(r["fallback_used"] for sr in sreps for r in sr)))
-    ctx.close()
+               fallbacks=int(sum(r["fallback_used"] for sr in sreps for r in sr)),
+               alg_bytes=alg, t_solve_s=float(sum(r["t_solve_ms"] for r in reps)) * 1e-3, offs=offs, perm=perm)
+    if keep:
+        res["ctx"] = ctx
+    else:
+        ctx.close()
     return res
 
 
@@ -435,11 +485,13 @@ def bench_replicas(args, capi, systems, copy_gbs=None):
     """64 independent replicas (rod counts log-uniform in [2k, 50k], 3-D boxes and slabs) stepped
     together: one collection pass and one persistent solver launch per step for all of them (one
     graph replay per step in graph mode).  Plain FP64 from a cold start gives `value`; the chosen
-    form (FP32 + FP64 refinement, learned warm start) runs the same workload beside it."""
+    form (FP32 + FP64 refinement, learned warm start) runs the same workload beside it, and the FP64
+    oracle takes the same step of every replica on the host cores."""
     parts, cfg = systems.replicas()
     K = args.steps
-    a = _run_replicas(args, capi, parts, cfg, args.precision, 0)
+    a = _run_replicas(args, capi, parts, cfg, args.precision, 0, keep=True)
     n, reps = a["n"], a["reps"]
+    achieved = a["alg_bytes"] / a["t_solve_s"] / 1e9 if a["t_solve_s"] > 0 else 0.0
     out = {
         "metric": "rod-steps/sec at BBPGD residual 1e-5 (64 replicas in one launch per step)",
         "value": n * K / a["elapsed"], "unit": "rod-steps/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
@@ -455,8 +507,58 @@ def bench_replicas(args, capi, systems, copy_gbs=None):
         "t_collect_ms_mean": float(np.mean([r["t_collect_ms"] for r in reps])),
         "solver_schedule": a["tune"],
         "graph": a["graph"],
-        "stream_copy_gbs": copy_gbs,
+        "roofline": {
+            "bound": "hbm", "kernel": "k_bbpgd64 (persistent BBPGD over all replicas)",
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None, "algorithmic_bytes_per_launch": a["alg_bytes"] / K,
+            "note": "algorithmic bytes summed per system: matvecs_s x (200 M_s + 144 N_s)",
+            "stream_copy_gbs": copy_gbs,
+            "frac_of_stream_copy": achieved / copy_gbs if copy_gbs else None,
+        },
     }
+    tr = load_traffic("replicas")
+    if tr and args.precision == 0:
+        out["roofline"]["traffic"] = tr["traffic_over_algorithmic"] * a["alg_bytes"] / K
+        out["roofline"]["traffic_source"] = "profiles/k_bbpgd_traffic.json[replicas], scaled by this run's algorithmic bytes"
+    if not args.no_cpu:
+        ctx = a["ctx"]
+        threads, logical, physical = setup_cpu_threads(args.cpu_threads)
+        from oracle import oracle as O
+        O.set_threads(threads)
+        offs, perm = a["offs"], a["perm"]
+        cat = {k: np.concatenate([np.asarray(r[k]).reshape(len(r["length"]), -1) for r, _ in parts])[perm]
+               for k in ("gid", "length", "radius")}
+        c, t = ctx.get_rods()
+        st = ctx.step_count
+        g0 = time.perf_counter()
+        grep = ctx.step()
+        tg = time.perf_counter() - g0
+        g = ctx.constraint_arrays()
+        gam, u = ctx.gamma(), ctx.velocity()["u"]
+        tc, same, ncon = 0.0, True, 0
+        for s_, (_, box) in enumerate(parts):
+            lo, hi = int(offs[s_]), int(offs[s_ + 1])
+            rods = dict(gid=cat["gid"][lo:hi, 0], length=cat["length"][lo:hi, 0], radius=cat["radius"][lo:hi, 0],
+                        center=c[lo:hi], dir=t[lo:hi], immovable=None)
+            oc = O.cfg_from_dict(dict(cfg, **box))
+            oc.step = st
+            ts = time.perf_counter()
+            res = O.step(rods, oc)
+            tc += time.perf_counter() - ts
+            sel = (g["I"] >= lo) & (g["I"] < hi)
+            same &= bool(same_bits(gam[sel], res["gamma"]) and same_bits(u[6 * lo:6 * hi], res["u"]))
+            ncon += res["I"].size
+        out["cpu_baseline"] = {
+            "value": n / tc, "unit": "rod-steps/s", "cores": threads, "kind": "port",
+            "sample": f"1 full step of all {len(parts)} replicas one after another (FP64 oracle, OpenMP, {threads} "
+                      f"threads, OMP_PROC_BIND=spread OMP_PLACES=cores; {logical} logical CPUs = {physical} physical "
+                      f"cores) from the post-benchmark state: {ncon} constraints, {tc:.3f} s; the GPU took "
+                      f"{tg * 1e3:.2f} ms for the same step (gamma and U {'bit-identical' if same else 'MISMATCH'} "
+                      f"per replica); CPU {cpu_model()}",
+            "same_step_bit_identical": bool(same),
+        }
+        out["gpu_over_cpu"] = out["value"] / (n / tc)
+    a["ctx"].close()
     if not args.no_form:
         from alens_amd import warmstart as W
 

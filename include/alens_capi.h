@@ -111,7 +111,8 @@ alens_status alens_set_systems(alens_ctx *ctx, int32_t nsys, const int32_t *rod_
                                const int32_t *gid, const double *center, const double *dir, const double *length,
                                const double *radius, const uint8_t *immovable);
 alens_status alens_get_num_systems(const alens_ctx *ctx, int32_t *nsys);
-/* per-system solve reports of the last step (nsys entries; iters_fp32 filled for precision = 1) */
+/* per-system solve reports of the last step (nsys entries; n_con = the system's rows; iters_fp32 filled
+ * for precision = 1) */
 alens_status alens_get_system_reports(alens_ctx *ctx, alens_solve_report *reps);
 
 /* Reorder the rods on the device by the Morton code of their centre's cell so that neighbours sit
@@ -237,6 +238,12 @@ alens_status alens_mlp_gradient(alens_ctx *ctx, int64_t m, const float *X, const
  * the weights when no network is loaded.  batch is a multiple of 16; loss_per_epoch may be NULL. */
 alens_status alens_mlp_train(alens_ctx *ctx, int64_t m, const float *X, const float *gamma, int32_t epochs,
                              int32_t batch, float lr, uint64_t seed, float *loss_per_epoch);
+
+/* Device copy probe (no counterpart in the reference; a measurement aid for benchmark scripts): on
+ * HIP device `device`, one kernel (k_copy_probe) copies nbytes between two fresh buffers, reps times;
+ * *gbs = (read + written bytes) / time of those launches, in GB/s.  The buffers are freed on return.
+ * A single launch of known traffic also calibrates profiler byte counters. */
+alens_status alens_copy_probe(int32_t device, int64_t nbytes, int32_t reps, double *gbs);
 
 #ifdef __cplusplus
 }
