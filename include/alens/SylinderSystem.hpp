@@ -3,31 +3,33 @@
 // The reference calls into SimToolbox's SylinderSystem / ConstraintCollector / BCQPSolver from
 // TubuleSystem (SRC/TubuleSystem.cpp:79, 120, 223, 55, 697-742).  These header-only classes keep those
 // names and call shapes so that code written against them moves onto the GPU engine unchanged:
-//   SylinderSystem::initialize / prepareStep / runStep / stepEuler / getConstraintPoolNonConst /
-//   getStepCount, the per-thread ConstraintBlockPool (pool.size() == thread count, one queue per
-//   thread), ConstraintBlock's 16-argument constructor and setStress, and BCQPSolver::solveBBPGD;
-//   rod files (TubuleInitial.dat / SylinderAscii_*.dat), writeResult, reinitialize and getSnapID.
+//   SylinderSystem::initialize(configFile, posFile, argc, argv) / reinitialize(configFile, restartFile,
+//   argc, argv, eulerStep) / runConfig / prepareStep / runStep / calcConStress / stepEuler /
+//   getConstraintPoolNonConst / getStepCount / getSnapID, the per-thread ConstraintBlockPool
+//   (pool.size() == thread count, one queue per thread), ConstraintBlock's 16-argument constructor and
+//   setStress, and BCQPSolver::solveBBPGD; RunConfig.yaml (RunConfig.hpp), rod files
+//   (TubuleInitial.dat / SylinderAscii_*.dat), constraint-block snapshots (ConBlock_*.pvtp,
+//   ConBlockVTK.hpp) and the restart record.
 // Errors surface as alens::Error exceptions carrying the alens_status and message.
 #pragma once
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <random>
 #include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "../alens_capi.h"
+#include "ConBlockVTK.hpp"
+#include "RunConfig.hpp"
 
 namespace alens {
-
-struct Error : std::runtime_error {
-    alens_status status;
-    Error(alens_status s, const std::string &m) : std::runtime_error(m), status(s) {}
-};
 
 inline void check(const alens_ctx *ctx, alens_status s, const char *what) {
     if (s != ALENS_OK) throw Error(s, std::string(what) + ": " + alens_last_error(ctx));
@@ -146,18 +148,127 @@ inline void writeRodFile(const std::string &path, const Rods &r, double time) {
     std::fclose(f);
 }
 
+inline bool fileExists(const std::string &path) {
+    std::ifstream f(path);
+    return (bool)f;
+}
+
+// Rods drawn from the run configuration when no rod file exists (what the reference's initialize
+// does without its posFile): sylinderNumber rods, centres uniform in initBox (with initCircularX, in
+// the disc of the box's y-z cross-section), direction initOrient, or uniform on the sphere when any
+// component lies outside [-1, 1] (in the x-y plane for a monolayer, whose rods sit at the box's
+// mid-height), length sylinderLength or lognormal with that mean and sylinderLengthSigma, radius
+// sylinderDiameter / 2.  The random stream (std::mt19937_64 seeded with rngSeed) is this engine's own.
+inline Rods initialRodsFromConfig(const RunConfig &rc) {
+    Rods r;
+    std::mt19937_64 rng((uint64_t)rc.rngSeed);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    const double *lo = rc.initBoxLow, *hi = rc.initBoxHigh;
+    bool fixedDir = true;
+    for (int k = 0; k < 3; ++k) fixedDir = fixedDir && rc.initOrient[k] >= -1.0 && rc.initOrient[k] <= 1.0;
+    const double sig = rc.sylinderLengthSigma;
+    std::lognormal_distribution<double> LN(std::log(rc.sylinderLength > 0 ? rc.sylinderLength : 1.0) - 0.5 * sig * sig,
+                                           sig > 0 ? sig : 1.0);
+    for (int i = 0; i < rc.sylinderNumber; ++i) {
+        double c[3];
+        for (int k = 0; k < 3; ++k) c[k] = lo[k] + (hi[k] - lo[k]) * U(rng);
+        if (rc.initCircularX) {
+            const double yc = 0.5 * (lo[1] + hi[1]), zc = 0.5 * (lo[2] + hi[2]);
+            const double R = 0.5 * std::min(hi[1] - lo[1], hi[2] - lo[2]);
+            const double rr = R * std::sqrt(U(rng)), th = 6.283185307179586 * U(rng);
+            c[1] = yc + rr * std::cos(th);
+            c[2] = zc + rr * std::sin(th);
+        }
+        if (rc.monolayer) c[2] = 0.5 * (lo[2] + hi[2]);
+        double t[3];
+        if (fixedDir) {
+            for (int k = 0; k < 3; ++k) t[k] = rc.initOrient[k];
+        } else if (rc.monolayer) {
+            const double th = 6.283185307179586 * U(rng);
+            t[0] = std::cos(th);
+            t[1] = std::sin(th);
+            t[2] = 0.0;
+        } else {
+            const double z = 2.0 * U(rng) - 1.0, ph = 6.283185307179586 * U(rng), s = std::sqrt(1.0 - z * z);
+            t[0] = s * std::cos(ph);
+            t[1] = s * std::sin(ph);
+            t[2] = z;
+        }
+        const double tn = std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+        if (!(tn > 0)) throw Error(ALENS_E_ARG, "RunConfig: initOrient is the zero vector");
+        r.gid.push_back(i);
+        r.center.insert(r.center.end(), {c[0], c[1], c[2]});
+        r.direction.insert(r.direction.end(), {t[0] / tn, t[1] / tn, t[2] / tn});
+        r.length.push_back(sig > 0 ? LN(rng) : rc.sylinderLength);
+        r.radius.push_back(0.5 * rc.sylinderDiameter);
+    }
+    return r;
+}
+
+// Engine settings that are not keys of RunConfig.yaml: precision form, warm start, device and
+// cell-list edge (alens_config fields of the same names).
+struct EngineOptions {
+    int32_t precision = 0, warm_start = 0, device = 0;
+    double cell_size = 0.0;
+};
+
 class SylinderSystem {
   public:
+    // The reference constructs SylinderSystem empty and calls initialize(configFile, posFile, ...);
+    // nThreads is the size of the constraint pool (one queue per thread).
+    explicit SylinderSystem(int nThreads = 1, EngineOptions opt = EngineOptions())
+        : opt_(opt), pool_((size_t)(nThreads > 0 ? nThreads : 1)) {}
     SylinderSystem(const alens_config &cfg, int nThreads = 1) : pool_((size_t)(nThreads > 0 ? nThreads : 1)) {
-        alens_status s = alens_create(&cfg, &ctx_);
-        if (s != ALENS_OK) throw Error(s, "alens_create failed");
+        create(cfg);
     }
-    ~SylinderSystem() { alens_destroy(ctx_); }
+    ~SylinderSystem() {
+        if (ctx_) alens_destroy(ctx_);
+    }
     SylinderSystem(const SylinderSystem &) = delete;
     SylinderSystem &operator=(const SylinderSystem &) = delete;
 
+    // rodSystem.runConfig: the configuration read by initialize / reinitialize (SRC/TubuleSystem.cpp:112-113)
+    RunConfig runConfig;
+
+    // initialize(configFile, posFile, argc, argv) (SRC/TubuleSystem.cpp:79): read RunConfig.yaml, create
+    // the engine, and load the rods of posFile, or draw them from the configuration when that file
+    // does not exist; sylinderFixed makes every rod immovable.  argc / argv are accepted for the
+    // reference's signature (it forwards them to its MPI / Trilinos set-up) and not used.
+    void initialize(const std::string &configFile, const std::string &posFile, int argc = 0, char **argv = nullptr) {
+        (void)argc;
+        (void)argv;
+        runConfig = readRunConfig(configFile);
+        create(runConfig.toEngineConfig(engineBase()));
+        Rods r = fileExists(posFile) ? readRodFile(posFile) : initialRodsFromConfig(runConfig);
+        if (runConfig.sylinderFixed) r.immovable.assign((size_t)r.size(), 1);
+        initialize(r);
+    }
+    // reinitialize(configFile, restartFile, argc, argv, eulerStep) (SRC/TubuleSystem.cpp:38): read the
+    // configuration, then resume from the restart record.  The snapshot holds the positions after the
+    // last Euler move (writeResult runs after stepEuler here), so the next stepEuler before any
+    // runStep has nothing left to apply and returns at once; eulerStep is accepted for the signature.
+    void reinitialize(const std::string &configFile, const std::string &restartFile, int argc, char **argv,
+                      bool eulerStep) {
+        (void)argc;
+        (void)argv;
+        (void)eulerStep;
+        runConfig = readRunConfig(configFile);
+        create(runConfig.toEngineConfig(engineBase()));
+        reinitialize(restartFile);
+        if (runConfig.sylinderFixed) {
+            rods_.immovable.assign((size_t)n_, 1);
+            std::vector<double> c, t;
+            getPositions(c, t);
+            check(ctx_, alens_set_rods(ctx_, n_, rods_.gid.data(), c.data(), t.data(), rods_.length.data(),
+                                       rods_.radius.data(), rods_.immovable.data()),
+                  "set_rods");
+        }
+    }
+
     // initialize(): load rods; spatialSort reorders them the way FDPS keeps its particle arrays
     void initialize(const Rods &r, bool spatialSort = true) {
+        if (!ctx_) throw Error(ALENS_E_STATE, "SylinderSystem: no engine configuration (construct with an "
+                                              "alens_config or call initialize(configFile, posFile))");
         n_ = r.size();
         check(ctx_, alens_set_rods(ctx_, n_, r.gid.data(), r.center.data(), r.direction.data(), r.length.data(),
                                    r.radius.data(), r.immovable.empty() ? nullptr : r.immovable.data()),
@@ -177,17 +288,26 @@ class SylinderSystem {
             rods_ = std::move(s);
         }
     }
-    // initialize(posFile): rods from a rod file (SRC/TubuleSystem.cpp:79 passes "TubuleInitial.dat")
-    void initialize(const std::string &posFile, bool spatialSort = true) { initialize(readRodFile(posFile), spatialSort); }
+    // rods from a rod file only (the engine already configured)
+    void initializeFromFile(const std::string &posFile, bool spatialSort = true) {
+        initialize(readRodFile(posFile), spatialSort);
+    }
 
-    // writeResult(): SylinderAscii_<snapID>.dat with the current rods (17 significant digits) and the
-    // restart record TimeStepInfo.txt ("key value" lines: snapID, stepCount, time, snapFile) in folder;
-    // call it after stepEuler, so the file holds the positions the next step starts from.
+    // writeResult(): SylinderAscii_<snapID>.dat with the current rods (17 significant digits), the
+    // constraint blocks of the last step with their impulses as ConBlock_r0_<snapID>.vtp and
+    // ConBlock_<snapID>.pvtp (when a step has run since the last Euler move, i.e. when called between
+    // runStep and stepEuler; otherwise the blocks of the step before are kept from that call), and the
+    // restart record TimeStepInfo.txt ("key value" lines: snapID, stepCount, time, snapFile) in folder.
+    // For a restart that continues the trajectory, call it after stepEuler, so the rod file holds the
+    // positions the next step starts from.
     void writeResult(const std::string &folder, double time) {
         Rods r = rods_;
         getPositions(r.center, r.direction);
         const std::string name = "SylinderAscii_" + std::to_string(snapID_) + ".dat";
         writeRodFile(folder + "/" + name, r, time);
+        if (stepped_) lastBlocks_ = getCollisionBlocksC();
+        if (!vtk::writeConBlocks(folder, snapID_, lastBlocks_))
+            throw Error(ALENS_E_ARG, "cannot write ConBlock files in " + folder);
         std::ofstream info(folder + "/TimeStepInfo.txt");
         info << "snapID " << snapID_ << "\nstepCount " << (getStepCount() - 1) << "\ntime ";
         char buf[40];
@@ -222,6 +342,7 @@ class SylinderSystem {
     // runStep(): Brownian + non-constraint velocity, collision collection, BBPGD over the collision
     // rows and the pool's blocks (queues concatenated in thread order), write-back
     void runStep(const double *forceTorque = nullptr) {
+        requireEngine();
         check(ctx_, alens_clear_constraints(ctx_), "clear_constraints");
         std::vector<alens_con_block> raw;
         for (const auto &q : pool_)
@@ -229,12 +350,21 @@ class SylinderSystem {
         if (!raw.empty()) check(ctx_, alens_add_constraints(ctx_, (int64_t)raw.size(), raw.data()), "add_constraints");
         alens_status s = alens_step_constraints(ctx_, forceTorque, &report_);
         if (s != ALENS_OK && s != ALENS_E_NOCONV) check(ctx_, s, "step_constraints");
+        stepped_ = true;
     }
     // calcConStress(): gamma-weighted stress sums of the last step (collision rows, pool rows)
     void calcConStress(double collision[9], double pool[9]) {
         check(ctx_, alens_get_con_stress(ctx_, collision, pool), "get_con_stress");
     }
-    void stepEuler() { check(ctx_, alens_step_euler(ctx_), "step_euler"); }
+    // stepEuler(): move the rods with the last step's velocities (SRC/TubuleSystem.cpp:55).  Right
+    // after (re)initialize, before any runStep, there is no velocity to apply and it returns at once.
+    void stepEuler() {
+        requireEngine();
+        if (!stepped_) return;
+        lastBlocks_ = getCollisionBlocksC();
+        check(ctx_, alens_step_euler(ctx_), "step_euler");
+        stepped_ = false;
+    }
 
     ConstraintBlockPool &getConstraintPoolNonConst() { return pool_; }
     int64_t getStepCount() const { return alens_get_step(ctx_); }
@@ -242,11 +372,15 @@ class SylinderSystem {
     alens_ctx *handle() { return ctx_; }
 
     // this step's constraints with their impulses: collision rows in canonical order, then the pool's
-    std::vector<ConstraintBlock> getCollisionBlocks() {
+    std::vector<alens_con_block> getCollisionBlocksC() {
         int64_t m = 0;
         check(ctx_, alens_get_num_constraints(ctx_, &m), "get_num_constraints");
         std::vector<alens_con_block> raw((size_t)m);
-        check(ctx_, alens_get_constraints(ctx_, m, raw.data(), &m), "get_constraints");
+        if (m > 0) check(ctx_, alens_get_constraints(ctx_, m, raw.data(), &m), "get_constraints");
+        return raw;
+    }
+    std::vector<ConstraintBlock> getCollisionBlocks() {
+        const std::vector<alens_con_block> raw = getCollisionBlocksC();
         std::vector<ConstraintBlock> out;
         out.reserve(raw.size());
         for (const auto &c : raw) out.push_back(ConstraintBlock::fromC(c));
@@ -265,7 +399,35 @@ class SylinderSystem {
     }
 
   private:
+    void create(const alens_config &cfg) {
+        if (ctx_) {
+            alens_destroy(ctx_);
+            ctx_ = nullptr;
+        }
+        alens_status s = alens_create(&cfg, &ctx_);
+        if (s != ALENS_OK) {
+            ctx_ = nullptr;
+            throw Error(s, "alens_create failed (invalid configuration or no HIP device)");
+        }
+        stepped_ = false;
+        lastBlocks_.clear();
+    }
+    alens_config engineBase() const {
+        alens_config c{};
+        c.precision = opt_.precision;
+        c.warm_start = opt_.warm_start;
+        c.device = opt_.device;
+        c.cell_size = opt_.cell_size;
+        return c;
+    }
+    void requireEngine() const {
+        if (!ctx_) throw Error(ALENS_E_STATE, "SylinderSystem: not initialized");
+    }
+
     alens_ctx *ctx_ = nullptr;
+    EngineOptions opt_;
+    bool stepped_ = false;                       // a runStep since the last Euler move
+    std::vector<alens_con_block> lastBlocks_;    // blocks of the last step (for writeResult after stepEuler)
     int32_t n_ = 0;
     int snapID_ = 0;
     Rods rods_;  // per-rod data in the context's (sorted) order; positions live on the device

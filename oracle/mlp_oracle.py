@@ -1,4 +1,5 @@
-"""FP64 NumPy oracle of the learned warm-start network (12 -> 64 -> 64 -> 64 -> 1, SiLU, softplus).
+"""FP64 NumPy oracle of the learned warm-start network (12 -> 64 -> 64 -> 64 -> 1, SiLU, softplus)
+and of its per-constraint inputs (features, match_previous).
 
 TEST INFRASTRUCTURE ONLY (imported by tests/ as the checker of libalens.so's MFMA kernels).
 
@@ -15,6 +16,49 @@ import numpy as np
 PW1, PB1, PW2, PB2, PW3, PB3, PW4, PB4 = 0, 1024, 1088, 5184, 5248, 9344, 9408, 9472
 NPARAM = 9473
 NIN = 12
+
+
+def match_previous(I, J, I_prev, J_prev, gamma_prev):
+    """Previous-step impulse of each row's rod pair (I, J), 0 where the pair had no row last step: the
+    warm-start key is the pair of rod indices (rods keep their order between steps)."""
+    prev = {(int(a), int(b)): float(g) for a, b, g in zip(I_prev, J_prev, gamma_prev)}
+    return np.array([prev.get((int(a), int(b)), 0.0) for a, b in zip(I, J)], np.float64)
+
+
+def features(I, J, delta0, nrm, posI, posJ, dirs, length, unc, xprev, dt, wide=False):
+    """Per-constraint network inputs (16 float32 per row, 12 used), in the expression order of the
+    device kernel: gap; normal (3); contact arms along the rods, s_I / (L_I / 2) and s_J / (L_J / 2)
+    with s = arm . direction; dt times the relative velocity of the two contact points under U_nc,
+    (v_I + w_I x p_I) - (v_J + w_J x p_J) (3); L_I; L_J; previous-step gamma (0 when None).  Lengths
+    and directions are the rods' own (SRC/TubuleBind.hpp:54-67: centre +- (L/2) direction).
+    wide=True returns the FP64 values before the float32 rounding."""
+    I = np.asarray(I, np.int64)
+    J = np.asarray(J, np.int64)
+    m = I.size
+    n = np.asarray(nrm, np.float64).reshape(m, 3)
+    pi = np.asarray(posI, np.float64).reshape(m, 3)
+    pj = np.asarray(posJ, np.float64).reshape(m, 3)
+    t = np.asarray(dirs, np.float64).reshape(-1, 3)
+    L = np.asarray(length, np.float64)
+    u = np.asarray(unc, np.float64).reshape(-1, 6)
+    ti, tj, Li, Lj, ui, uj = t[I], t[J], L[I], L[J], u[I], u[J]
+    si = (pi[:, 0] * ti[:, 0] + pi[:, 1] * ti[:, 1]) + pi[:, 2] * ti[:, 2]
+    sj = (pj[:, 0] * tj[:, 0] + pj[:, 1] * tj[:, 1]) + pj[:, 2] * tj[:, 2]
+    vr = np.empty((m, 3))
+    vr[:, 0] = (ui[:, 0] + (ui[:, 4] * pi[:, 2] - ui[:, 5] * pi[:, 1])) - (uj[:, 0] + (uj[:, 4] * pj[:, 2] - uj[:, 5] * pj[:, 1]))
+    vr[:, 1] = (ui[:, 1] + (ui[:, 5] * pi[:, 0] - ui[:, 3] * pi[:, 2])) - (uj[:, 1] + (uj[:, 5] * pj[:, 0] - uj[:, 3] * pj[:, 2]))
+    vr[:, 2] = (ui[:, 2] + (ui[:, 3] * pi[:, 1] - ui[:, 4] * pi[:, 0])) - (uj[:, 2] + (uj[:, 3] * pj[:, 1] - uj[:, 4] * pj[:, 0]))
+    X = np.zeros((m, 16), np.float64)
+    X[:, 0] = np.asarray(delta0, np.float64)
+    X[:, 1:4] = n
+    with np.errstate(divide="ignore", invalid="ignore"):
+        X[:, 4] = np.where(Li > 0.0, si / (0.5 * Li), 0.0)
+        X[:, 5] = np.where(Lj > 0.0, sj / (0.5 * Lj), 0.0)
+    X[:, 6:9] = vr * dt
+    X[:, 9] = Li
+    X[:, 10] = Lj
+    X[:, 11] = 0.0 if xprev is None else np.asarray(xprev, np.float64)
+    return X if wide else X.astype(np.float32)
 
 
 def bf16(x):

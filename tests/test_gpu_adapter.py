@@ -36,3 +36,50 @@ def test_cpp_adapter_restart(tmp_path):
     assert v["iters_a"] == v["iters_b"] and float(v["rel"]) <= 1e-8, p.stdout
     assert v["snap"] == "1" and v["step"] == "3", p.stdout
     assert (tmp_path / "SylinderAscii_0.dat").exists() and (tmp_path / "TimeStepInfo.txt").exists()
+
+
+def test_cpp_adapter_from_run_config(tmp_path):
+    """initialize(RunConfig.yaml, TubuleInitial.dat) on the reference's DenseMonoLayer example (9,700
+    rods, monolayer, periodic in x and y, tol 1e-6, dt 1e-5): two steps through the adapter, then
+    writeResult.  The same two steps through the Python binding, configured by alens_amd.runconfig
+    from the same file, give the same rows and bit-identical impulses, and the ConBlock snapshot the
+    adapter wrote holds them (read back with the independent VTK reader)."""
+    import os
+
+    import numpy as np
+
+    from alens_amd import capi, runconfig, systems
+    from tests.test_conblock_vtk import read_vtp
+
+    golden = os.path.join(os.path.dirname(__file__), "golden")
+    yml = os.path.join(golden, "runconfig", "DenseMonoLayer_RunConfig.yaml")
+    dat = os.path.join(golden, "dense_monolayer_TubuleInitial.dat")
+    with tempfile.TemporaryDirectory() as d:
+        exe = build_adapter_driver(d, "adapter_config")
+        p = subprocess.run([exe, yml, dat, str(tmp_path), "2"], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    tok = p.stdout.split()
+    v = dict(zip(tok[0::2], tok[1::2]))
+    assert v["rods"] == "9700" and v["converged"] == "1" and v["steps"] == "2", p.stdout
+    assert float(v["dt"]) == 1e-5 and float(v["tol"]) == 1e-6 and v["monolayer"] == "1", p.stdout
+    gam_cpp = np.fromfile(str(tmp_path / "gamma.bin"), np.float64)
+
+    cfg = runconfig.load(yml)
+    rods = systems.read_rod_file(dat)
+    ctx = capi.Context(**cfg)
+    ctx.set_rods(rods)
+    perm = ctx.sort_rods()
+    for _ in range(2):
+        ctx.step()
+        g = ctx.constraint_arrays()
+        gam = ctx.gamma()
+        ctx.euler()
+    ctx.close()
+    assert gam.size == gam_cpp.size == int(v["M"]) > 1000
+    assert np.array_equal(gam.view(np.uint64), gam_cpp.view(np.uint64))
+    snap = read_vtp(str(tmp_path / "ConBlock_r0_0.vtp"))
+    assert (tmp_path / "ConBlock_0.pvtp").exists() and (tmp_path / "SylinderAscii_0.dat").exists()
+    assert np.array_equal(snap["gamma"].view(np.uint64), gam.view(np.uint64))
+    gid = rods["gid"][perm]
+    assert np.array_equal(snap["gid"].reshape(-1, 2), np.stack([gid[g["I"]], gid[g["J"]]], 1))
+    assert np.array_equal(snap["delta0"], g["delta0"])

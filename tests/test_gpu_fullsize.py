@@ -114,3 +114,91 @@ def test_full_size_replicas_per_system():
         assert np.all(np.abs(f.sum(0)) <= 1e-9 * np.abs(f).sum(0) + 1e-300), s
         checked += I.size
     assert checked == g["I"].size
+
+
+def _relaxed(kind, relax, **over):
+    """A context on the full-size workload after `relax` graph-mode steps from the seeded start, and
+    the rod arrays in the context's (spatial) order."""
+    rods, cfg = getattr(systems, kind)()
+    cfg = dict(cfg, **over)
+    ctx = capi.Context(**cfg)
+    ctx.set_rods(rods)
+    perm = ctx.sort_rods()
+    rods = {k: (v[perm] if isinstance(v, np.ndarray) else v) for k, v in rods.items()}
+    ctx.set_graph(True)
+    for _ in range(relax):
+        ctx.step()
+        ctx.euler()
+    return ctx, rods, cfg
+
+
+@pytest.mark.parametrize("kind,relax", [("monolayer_slab", 20), ("lognormal_3d", 8)])
+def test_full_size_relaxed_step_bitexact(kind, relax):
+    """At a relaxed state of the full-size workloads (after graph-mode steps from the seeded start),
+    one more GPU step (a graph replay) and the FP64 oracle step from the same state agree bit for
+    bit: pair set, iteration count, gamma, F_con and U."""
+    ctx, rods, cfg = _relaxed(kind, relax)
+    c, t = ctx.get_rods()
+    st = ctx.step_count
+    rep = ctx.step()
+    assert ctx.graph_info()["replays"] > 0
+    g = ctx.constraint_arrays()
+    gam = ctx.gamma()
+    vel = ctx.velocity()
+    oc = O.cfg_from_dict(cfg)
+    oc.step = st
+    ref = O.step(dict(rods, center=c, dir=t), oc)
+    print(f"{kind} step {st}: {g['I'].size} constraints, GPU {rep['iters']} / oracle {ref['report']['iters']} "
+          f"iterations", flush=True)
+    assert np.array_equal(g["I"], ref["I"]) and np.array_equal(g["J"], ref["J"])
+    assert rep["iters"] == ref["report"]["iters"] and rep["converged"] == 1
+    assert np.array_equal(_bits(gam), _bits(ref["gamma"])), "gamma differs"
+    assert np.array_equal(_bits(vel["fcon"]), _bits(ref["fcon"])), "F_con differs"
+    assert np.array_equal(_bits(vel["u"]), _bits(ref["u"])), "U differs"
+    ctx.close()
+
+
+def test_monolayer_500_steps_graph():
+    """The 200,000-rod monolayer slab as its configuration states it: 500 FP64 steps, each one
+    replay of the captured step graph.  Every step: converged to tol, the KKT conditions of the
+    returned gamma (gradient from the oracle's D, M, D^T on the step's own rows) and zero net contact
+    force.  On 10 sampled steps (every 50th), teacher-forced from the GPU's state before the step:
+    pair set, iteration count, gamma and U bit-identical to the FP64 oracle's step."""
+    ctx, rods, cfg = _relaxed("monolayer_slab", 0)
+    oc = O.cfg_from_dict(cfg)
+    n = rods["length"].size
+    slack = cfg["tol"] * (1 + 1e-9)
+    sampled = 0
+    for s in range(500):
+        check = s > 0 and s % 50 == 0 or s == 499
+        if check:
+            c, t = ctx.get_rods()
+            st = ctx.step_count
+        rep = ctx.step()
+        assert rep["converged"] == 1 and rep["res"] <= cfg["tol"], (s, rep)
+        g = ctx.constraint_arrays()
+        gam = ctx.gamma()
+        vel = ctx.velocity()
+        rw = dict(nrm=g["nrm"].reshape(-1), rI=g["rI"].reshape(-1), rJ=g["rJ"].reshape(-1))
+        _, U = O.apply_D_M(n, g["I"], g["J"], rw, vel["mob"], oc, gam)
+        grad = oc.dt * O.apply_DT(g["I"], g["J"], rw, U) + g["b"]
+        assert np.all(gam >= 0.0) and grad.min() >= -slack, s
+        assert np.max(np.abs(np.minimum(gam, grad))) <= slack, s
+        f = vel["fcon"].reshape(-1, 6)[:, :3]
+        assert np.all(np.abs(f.sum(0)) <= 1e-9 * np.abs(f).sum(0) + 1e-300), s
+        if check:
+            ocs = O.cfg_from_dict(cfg)
+            ocs.step = st
+            ref = O.step(dict(rods, center=c, dir=t), ocs)
+            assert np.array_equal(g["I"], ref["I"]) and np.array_equal(g["J"], ref["J"]), s
+            assert rep["iters"] == ref["report"]["iters"], s
+            assert np.array_equal(_bits(gam), _bits(ref["gamma"])), s
+            assert np.array_equal(_bits(vel["u"]), _bits(ref["u"])), s
+            sampled += 1
+            print(f"step {s}: {g['I'].size} constraints, {rep['iters']} iterations, bit-identical to the oracle",
+                  flush=True)
+        ctx.euler()
+    assert sampled == 10
+    info = ctx.graph_info()
+    assert info["replays"] >= 490 and info["capture_failures"] == 0, info
+    ctx.close()

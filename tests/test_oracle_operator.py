@@ -125,8 +125,61 @@ def test_mobility_limits():
     assert np.all(mob2[[0, 3], 3:] == 0)
 
 
+def _perrin_rot(a, b, mu):
+    """Rotational drag of a prolate spheroid (semi-axes a > b) about an equatorial axis, the exact
+    Stokes result (Perrin 1934): 32 pi mu (a^4 - b^4) / (3 ((2 a^2 - b^2) S - 2 a)),
+    S = 2 ln((a + c) / b) / c, c = sqrt(a^2 - b^2).  Tends to 8 pi mu a^3 for a sphere and to
+    pi mu L^3 / (3 (ln(L / b) - 1/2)) for L = 2a >> b."""
+    c = np.sqrt(a * a - b * b)
+    S = 2.0 / c * np.log((a + c) / b)
+    return 32.0 * np.pi * mu * (a ** 4 - b ** 4) / (3.0 * ((2.0 * a * a - b * b) * S - 2.0 * a))
+
+
+def test_rod_rotational_drag():
+    """1/zeta_r of a rod (rotation about an axis perpendicular to it), pinned three ways that a
+    factor-2 or a wrong power of L would fail:
+    - the slender-body limit zeta_r -> pi mu L^3 / (3 ln(L/r)) (Batchelor 1970; Doi & Edwards'
+      D_r = 3 kT ln(L/d) / (pi mu L^3)), within 10% at L/r = 8e4 and converging like 1/ln(L/r);
+    - exact L^3 scaling at fixed aspect ratio, and 1/mu scaling;
+    - resistive-force theory: a line of perpendicular drag zeta_perp / L per unit length turning
+      about its centre has zeta_r = zeta_perp L^2 / 12, at every aspect ratio;
+    - the exact rotational drag of the inscribed prolate spheroid (Perrin 1934), which the
+      spherocylinder's slender-body value approaches from below, 0.82 -> 0.93 for L/r = 80 -> 1e6."""
+    mu = 0.01
+    oc = O.make_cfg(viscosity=mu)
+    r = 0.0125
+    L = np.array([1.0, 10.0, 1000.0, 1e4, 1e8, 1e14, 10.0 * 1.25e-4 / r])
+    rad = np.array([r, r, r, r, r, r, 1.25e-4])
+    n = L.size
+    rods = dict(gid=np.arange(n, dtype=np.int32), center=np.zeros((n, 3)), dir=np.tile([1.0, 0, 0], (n, 1)),
+                length=L, radius=rad)
+    mob = O.mobility(rods, oc).reshape(-1, 6)
+    zr = 1.0 / mob[:, 5]
+    zperp = 1.0 / mob[:, 3]
+    lr = np.log(L / rad)
+    sb = np.pi * mu * L ** 3 / (3.0 * lr)
+    rel = zr / sb
+    assert abs(rel[2] - 1.0) < 0.1  # L/r = 8e4, the translational test's bar
+    assert np.all(np.abs(rel[1:6] - 1.0) <= 0.6 / lr[1:6]) and rel[5] > rel[4] > rel[3] > rel[2] > rel[1]
+    # L/r = 800 twice (L = 10, r = 0.0125 and L = 0.1, r = 1.25e-4): ratio (10 / 0.1)^3
+    assert np.isclose(zr[1] / zr[6], (L[1] / L[6]) ** 3, rtol=1e-13)
+    mob_mu = O.mobility(rods, O.make_cfg(viscosity=3 * mu)).reshape(-1, 6)
+    assert np.allclose(mob_mu[:, 5] * 3.0, mob[:, 5], rtol=1e-14)
+    assert np.allclose(zr, zperp * L ** 2 / 12.0, rtol=1e-13)
+    per = np.array([_perrin_rot(0.5 * L[k], rad[k], mu) for k in (0, 1, 2, 3)])
+    q = zr[:4] / per
+    assert np.all((q > 0.75) & (q < 1.0)) and q[3] > q[2] > q[1] > q[0], q
+    # the sphere branch (L < 2r): the exact sphere value, which Perrin's formula also gives as b -> a
+    sph = dict(rods, length=np.array([0.02]), radius=np.array([r]), gid=np.zeros(1, np.int32),
+               center=np.zeros((1, 3)), dir=np.array([[1.0, 0, 0]]))
+    zs = 1.0 / O.mobility(sph, oc).reshape(-1, 6)[0, 5]
+    assert np.isclose(zs, _perrin_rot(r * (1 + 1e-6), r, mu), rtol=1e-4)
+
+
 def test_brownian_covariance():
-    """U_B has zero mean and covariance 2 kT M / dt (per rod 6x6 block), checked over many rods."""
+    """Fluctuation-dissipation: U_B has zero mean and covariance 2 kT M / dt (per rod 6x6 block), checked
+    over many rods.  This pins the noise amplitude and its frame relative to M; M itself (1/zeta_par,
+    1/zeta_perp, 1/zeta_r) is pinned against closed forms by the mobility tests above."""
     n = 40000
     rng = np.random.default_rng(5)
     t = rng.normal(size=(n, 3))
