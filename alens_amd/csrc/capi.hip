@@ -548,7 +548,72 @@ int64_t alens_get_step(const alens_ctx *ctx) { return ctx ? ctx->step : -1; }
 
 }  // extern "C"
 
-// Launches one persistent BBPGD solve on the context's stream and reads its report words.
+// Rod-major adjacency of arbitrary rows (I, J need not be ordered), built on the host in the sliced
+// layout the solver reads (see launch_adjacency): each rod's entries in ascending constraint index,
+// the I side with (n, r_i), the J side with (-n, -r_j) and the top bit set on its index; geometry
+// component-major (geo[q * m + k]).
+struct HostAdjacency {
+    std::vector<uint32_t> start, sstart, sk;
+    std::vector<double> sw, geo;
+    uint64_t ep = 0;
+    // padded slot count only (no fill), for a capacity check before anything is copied
+    static uint64_t slots(int32_t n, int64_t m, const int32_t *I, const int32_t *J, std::vector<uint32_t> &deg) {
+        deg.assign((size_t)n, 0);
+        for (int64_t k = 0; k < m; ++k) {
+            deg[I[k]]++;
+            deg[J[k]]++;
+        }
+        uint64_t ep = 0;
+        for (size_t sl = 0; sl * kSlice < (size_t)n; ++sl) {
+            uint32_t dmax = 0;
+            for (size_t i = sl * kSlice; i < std::min((sl + 1) * kSlice, (size_t)n); ++i) dmax = std::max(dmax, deg[i]);
+            ep += (uint64_t)dmax * kSlice;
+        }
+        return ep;
+    }
+    void build(int32_t n, int64_t m, const int32_t *I, const int32_t *J, const double *nrm, const double *r_i,
+               const double *r_j) {
+        const size_t M = (size_t)m;
+        start.assign((size_t)n + 1, 0);
+        geo.assign(9 * M, 0.0);
+        for (int64_t k = 0; k < m; ++k) {
+            start[I[k] + 1]++;
+            start[J[k] + 1]++;
+            for (int a = 0; a < 3; ++a) {
+                geo[a * M + k] = nrm[3 * k + a];
+                geo[(3 + a) * M + k] = r_i[3 * k + a];
+                geo[(6 + a) * M + k] = r_j[3 * k + a];
+            }
+        }
+        for (int32_t i = 0; i < n; ++i) start[i + 1] += start[i];
+        const size_t nsl = ((size_t)n + kSlice - 1) / kSlice;
+        sstart.assign(nsl + 1, 0);
+        for (size_t sl = 0; sl < nsl; ++sl) {
+            uint32_t dmax = 0;
+            for (size_t i = sl * kSlice; i < std::min((sl + 1) * kSlice, (size_t)n); ++i)
+                dmax = std::max(dmax, start[i + 1] - start[i]);
+            sstart[sl + 1] = sstart[sl] + dmax * (uint32_t)kSlice;
+        }
+        ep = sstart[nsl];
+        sk.assign(std::max<uint64_t>(ep, 1), 0);
+        sw.assign(6 * std::max<uint64_t>(ep, 1), 0.0);
+        std::vector<uint32_t> fill((size_t)n, 0);
+        // entries go in ascending k per rod; k runs in order, so each rod's slots fill left to right
+        for (int64_t k = 0; k < m; ++k) {
+            for (int side = 0; side < 2; ++side) {
+                const int32_t i = side == 0 ? I[k] : J[k];
+                const size_t idx = sstart[(size_t)i / kSlice] + (size_t)fill[i]++ * kSlice + (size_t)i % kSlice;
+                sk[idx] = side == 0 ? (uint32_t)k : ((uint32_t)k | 0x80000000u);
+                const double *r = side == 0 ? r_i : r_j;
+                for (int a = 0; a < 3; ++a) {
+                    sw[a * ep + idx] = side == 0 ? nrm[3 * k + a] : -nrm[3 * k + a];
+                    sw[(3 + a) * ep + idx] = side == 0 ? r[3 * k + a] : -r[3 * k + a];
+                }
+            }
+        }
+    }
+};
+
 // Per-system constraint offsets -> reduction-chunk offsets and the chunk -> system table.
 static alens_status set_chunk_table(alens_ctx *ctx, const std::vector<uint32_t> &coff) {
     const int32_t ns = (int32_t)coff.size() - 1;
@@ -605,15 +670,28 @@ static alens_status parse_reports(alens_ctx *ctx, const double *o, uint32_t abor
     return ALENS_OK;
 }
 
-static alens_status read_reports(alens_ctx *ctx, const double *out_dev, alens_solve_report *rep,
+static alens_status read_reports(alens_ctx *ctx, const double *out_dev, const uint32_t *bar, alens_solve_report *rep,
                                  std::vector<alens_solve_report> *per_sys) {
     std::vector<double> o(8 * (size_t)ctx->nsys_cur);
     uint32_t abort_word = 0;
     TRY_HIP(hipMemcpyAsync(o.data(), out_dev, sizeof(double) * o.size(), hipMemcpyDeviceToHost, ctx->st));
-    TRY_HIP(hipMemcpyAsync(&abort_word, ctx->bar.p + kBarWords, sizeof abort_word, hipMemcpyDeviceToHost, ctx->st));
+    TRY_HIP(hipMemcpyAsync(&abort_word, bar + kBarWords, sizeof abort_word, hipMemcpyDeviceToHost, ctx->st));
     TRY_HIP(hipStreamSynchronize(ctx->st));
     return parse_reports(ctx, o.data(), abort_word, rep, per_sys);
 }
+
+// The solver's working memory: chunk partials, per-system report words, barrier words and the chunk
+// table (the context's buffers, or a caller's memory for the allocation-free entry point).
+struct SolveMem {
+    double *part, *out;
+    uint32_t *bar;
+    const uint32_t *con_off, *chunk_off;
+    const int32_t *chunk_sys;
+};
+
+template <typename T>
+static alens_status run_bbpgd_on(alens_ctx *ctx, SolveArgsT<T> a, const SolveMem &mem, alens_solve_report *rep,
+                                 float *ms_solve, std::vector<alens_solve_report> *per_sys);
 
 // Launches one persistent BBPGD solve on the context's stream.  Outside graph capture it waits for
 // it, times it and reads the report; while a step is being captured it only enqueues (the FP64
@@ -625,6 +703,15 @@ static alens_status run_bbpgd(alens_ctx *ctx, SolveArgsT<T> a, alens_solve_repor
     TRY_HIP(ctx->part.ensure((size_t)kRedQ * std::max<int32_t>(ctx->nchunks_cur, 1)));
     TRY_HIP(ctx->out.ensure(8 * (size_t)ns + 8));
     TRY_HIP(ctx->out32.ensure(8 * (size_t)ns + 8));
+    const SolveMem mem{ctx->part.p, sizeof(T) == 8 ? ctx->out.p : ctx->out32.p, ctx->bar.p,
+                       ctx->con_off.p, ctx->chunk_off.p, ctx->chunk_sys.p};
+    return run_bbpgd_on<T>(ctx, a, mem, rep, ms_solve, per_sys);
+}
+
+template <typename T>
+static alens_status run_bbpgd_on(alens_ctx *ctx, SolveArgsT<T> a, const SolveMem &mem, alens_solve_report *rep,
+                                 float *ms_solve, std::vector<alens_solve_report> *per_sys) {
+    const int32_t ns = ctx->nsys_cur;
     if (ctx->capture && ctx->bar_clean) {
         ctx->bar_clean = false;  // the captured step zeroed them with its other counters
     } else if (ctx->capture) {
@@ -632,22 +719,22 @@ static alens_status run_bbpgd(alens_ctx *ctx, SolveArgsT<T> a, alens_solve_repor
         // every word but the abort word, so an abort of the earlier launch reaches the step's tail
         // (and makes this launch leave at once instead of refining a partial iterate)
         ZeroJobs z{};
-        z.p[0] = ctx->bar.p;
+        z.p[0] = mem.bar;
         z.n[0] = kBarWords;
-        z.p[1] = ctx->bar.p + kBarWords + 1;
+        z.p[1] = mem.bar + kBarWords + 1;
         z.n[1] = kBarTotalWords - kBarWords - 1;
         z.count = 2;
         TRY_K(launch_zero_many(ctx->st, z));
     } else {
-        TRY_K(launch_fill_u32(ctx->st, ctx->bar.p, 0u, kBarTotalWords));
+        TRY_K(launch_fill_u32(ctx->st, mem.bar, 0u, kBarTotalWords));
     }
-    double *out = sizeof(T) == 8 ? ctx->out.p : ctx->out32.p;
-    a.part = ctx->part.p;
-    a.bar = ctx->bar.p;
+    double *out = mem.out;
+    a.part = mem.part;
+    a.bar = mem.bar;
     a.out = out;
-    a.con_off = ctx->con_off.p;
-    a.chunk_off = ctx->chunk_off.p;
-    a.chunk_sys = ctx->chunk_sys.p;
+    a.con_off = mem.con_off;
+    a.chunk_off = mem.chunk_off;
+    a.chunk_sys = mem.chunk_sys;
     a.nsys = ns;
     a.dt = ctx->cfg.dt;
     a.tol = ctx->cfg.tol;
@@ -685,7 +772,7 @@ static alens_status run_bbpgd(alens_ctx *ctx, SolveArgsT<T> a, alens_solve_repor
         return ALENS_OK;
     }
     TRY_HIP(hipEventRecord(ctx->ev[3], ctx->st));
-    alens_status s = read_reports(ctx, out, rep, per_sys);
+    alens_status s = read_reports(ctx, out, mem.bar, rep, per_sys);
     if (s != ALENS_OK) return s;
     TRY_HIP(hipEventElapsedTime(ms_solve, ctx->ev[2], ctx->ev[3]));
     return ALENS_OK;
@@ -1662,42 +1749,12 @@ alens_status alens_bcqp_solve(alens_ctx *ctx, int32_t n, int64_t m, const int32_
     // rod-major adjacency in ascending constraint index, built on the host for arbitrary rows, in the
     // sliced layout the solver reads (see launch_adjacency); geometry component-major
     const size_t M = (size_t)m, N = (size_t)std::max(n, 1);
-    std::vector<uint32_t> start((size_t)n + 1, 0);
-    std::vector<double> geo(9 * M);
-    for (int64_t k = 0; k < m; ++k) {
-        start[I[k] + 1]++;
-        start[J[k] + 1]++;
-        for (int a = 0; a < 3; ++a) {
-            geo[a * M + k] = nrm[3 * k + a];
-            geo[(3 + a) * M + k] = r_i[3 * k + a];
-            geo[(6 + a) * M + k] = r_j[3 * k + a];
-        }
-    }
-    for (int32_t i = 0; i < n; ++i) start[i + 1] += start[i];
-    const size_t nsl = ((size_t)n + kSlice - 1) / kSlice;
-    std::vector<uint32_t> sstart(nsl + 1, 0);
-    for (size_t sl = 0; sl < nsl; ++sl) {
-        uint32_t dmax = 0;
-        for (size_t i = sl * kSlice; i < std::min((sl + 1) * kSlice, (size_t)n); ++i)
-            dmax = std::max(dmax, start[i + 1] - start[i]);
-        sstart[sl + 1] = sstart[sl] + dmax * (uint32_t)kSlice;
-    }
-    const uint64_t ep = sstart[nsl];
-    std::vector<uint32_t> sk(std::max<uint64_t>(ep, 1), 0), fill((size_t)n, 0);
-    std::vector<double> sw(6 * std::max<uint64_t>(ep, 1), 0.0);
-    // entries go in ascending k per rod; k runs in order, so each rod's slots fill left to right
-    for (int64_t k = 0; k < m; ++k) {
-        for (int side = 0; side < 2; ++side) {
-            const int32_t i = side == 0 ? I[k] : J[k];
-            const size_t idx = sstart[(size_t)i / kSlice] + (size_t)fill[i]++ * kSlice + (size_t)i % kSlice;
-            sk[idx] = side == 0 ? (uint32_t)k : ((uint32_t)k | 0x80000000u);
-            const double *r = side == 0 ? r_i : r_j;
-            for (int a = 0; a < 3; ++a) {
-                sw[a * ep + idx] = side == 0 ? nrm[3 * k + a] : -nrm[3 * k + a];
-                sw[(3 + a) * ep + idx] = side == 0 ? r[3 * k + a] : -r[3 * k + a];
-            }
-        }
-    }
+    HostAdjacency h;
+    h.build(n, m, I, J, nrm, r_i, r_j);
+    const size_t nsl = h.sstart.size() - 1;
+    const uint64_t ep = h.ep;
+    std::vector<uint32_t> &start = h.start, &sstart = h.sstart, &sk = h.sk;
+    std::vector<double> &sw = h.sw, &geo = h.geo;
     ctx->epad = ep;
     ctx->sord_cur = ctx->spos_cur = nullptr;  // rods in index order in this layout
     TRY_HIP(ctx->adjStart.ensure(N + 1));
