@@ -26,7 +26,7 @@ EXPORTS = [
     "alens_get_velocity", "alens_bcqp_solve", "alens_add_constraints", "alens_clear_constraints",
     "alens_get_con_stress", "alens_autotune", "alens_set_graph", "alens_get_graph_info",
     "alens_mlp_set", "alens_mlp_get", "alens_get_features", "alens_mlp_predict", "alens_mlp_gradient",
-    "alens_mlp_train", "alens_copy_probe",
+    "alens_mlp_train", "alens_copy_probe", "alens_bcqp_solve_static",
 ]
 MLP_PARAMS = 9473
 MLP_NORM = 25
@@ -159,6 +159,7 @@ def lib():
     L.alens_mlp_train.argtypes = [vp, ctypes.c_int64, _fp, _fp, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
                                   ctypes.c_uint64, _fp]
     L.alens_copy_probe.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, _dp]
+    L.alens_bcqp_solve_static.argtypes = L.alens_bcqp_solve.argtypes
     for name in EXPORTS:
         fn = getattr(L, name)
         if name not in ("alens_destroy", "alens_last_error", "alens_version", "alens_get_step"):
@@ -409,7 +410,8 @@ class Context:
                                              _p(f, ctypes.c_double), _p(mob, ctypes.c_double)))
         return dict(u=u, unc=unc, fcon=f, mob=mob)
 
-    def bcqp_solve(self, n, I, J, nrm, rI, rJ, b, mob, x0=None, allow=(ALENS_E_NOCONV,)):
+    def bcqp_solve(self, n, I, J, nrm, rI, rJ, b, mob, x0=None, allow=(ALENS_E_NOCONV,), static=False):
+        """alens_bcqp_solve, or with static=True its allocation-free variant alens_bcqp_solve_static."""
         I = np.ascontiguousarray(I, np.int32)
         J = np.ascontiguousarray(J, np.int32)
         nrm = np.ascontiguousarray(nrm, np.float64).reshape(-1)
@@ -421,7 +423,8 @@ class Context:
         m = I.size
         g = np.zeros(max(m, 1))
         rep = Report()
-        st = lib().alens_bcqp_solve(self._h, int(n), m, _p(I, ctypes.c_int32), _p(J, ctypes.c_int32),
+        fn = lib().alens_bcqp_solve_static if static else lib().alens_bcqp_solve
+        st = fn(self._h, int(n), m, _p(I, ctypes.c_int32), _p(J, ctypes.c_int32),
                                     _p(nrm, ctypes.c_double), _p(rI, ctypes.c_double), _p(rJ, ctypes.c_double),
                                     _p(b, ctypes.c_double), _p(mob, ctypes.c_double), _p(x0, ctypes.c_double),
                                     _p(g, ctypes.c_double), ctypes.byref(rep))

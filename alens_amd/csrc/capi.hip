@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -1805,6 +1806,197 @@ alens_status alens_bcqp_solve(alens_ctx *ctx, int32_t n, int64_t m, const int32_
     rep.t_solve_ms = ms;
     rep.t_step_ms = ms;
     ctx->have_step = false;  // the step buffers now hold this solve's rows
+    if (rep_out) *rep_out = rep;
+    if (rep.status == ALENS_E_NONFINITE) return fail(ctx, ALENS_E_NONFINITE, "non-finite BBPGD residual");
+    if (rep.status == ALENS_E_NOCONV) return fail(ctx, ALENS_E_NOCONV, "BBPGD did not reach tol within max_ite");
+    return ALENS_OK;
+}
+
+}  // extern "C"
+
+// ---- allocation-free BCQP solve: every device buffer lives in one static device array ----
+// Sizes: rows, rods and padded adjacency slots of one call.  scratch holds, each part 256-byte
+// aligned: adjStart (n + 1 u32), sellStart (n / 64 + 2 u32), sellK (S u32), sellW (6 S f64), geo (9 M f64),
+// cI, cJ (M i32), b, x0, gamma (M f64), xg (M x 2 f64), Us and mob (6 N f64), rsys (N i32), smono,
+// con_off, chunk_off (2 u32 each), chunk_sys (chunks i32), part (8 x chunks f64), bar (barrier words),
+// out (16 f64).
+constexpr size_t kStaticMaxRows = size_t(1) << 17;
+constexpr size_t kStaticMaxRods = size_t(1) << 16;
+constexpr size_t kStaticMaxSlots = 4 * kStaticMaxRows;
+constexpr size_t kStaticMaxChunks = (kStaticMaxRows + kRedChunk - 1) / kRedChunk;
+static constexpr size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
+constexpr size_t kScratchBytes =
+    al256(4 * (kStaticMaxRods + 1)) + al256(4 * (kStaticMaxRods / kSlice + 2)) + al256(4 * kStaticMaxSlots) +
+    al256(48 * kStaticMaxSlots) + al256(72 * kStaticMaxRows) + 2 * al256(4 * kStaticMaxRows) +
+    3 * al256(8 * kStaticMaxRows) + al256(16 * kStaticMaxRows) + 2 * al256(48 * kStaticMaxRods) +
+    al256(4 * kStaticMaxRods) + 4 * al256(8) + al256(4 * kStaticMaxChunks) + al256(8 * kRedQ * kStaticMaxChunks) +
+    al256(4 * kBarTotalWords) + al256(8 * 16);
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+// The device address of `scratch` on each device, fetched once with that device current (the array's
+// name is only a host-side stand-in; kernels and copies get this pointer).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+hipError_t scratch_on(int dev, unsigned char **out) {
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[dev] = (unsigned char *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
+// carves consecutive 256-byte aligned parts out of the scratch array
+struct Carve {
+    unsigned char *base;
+    size_t off = 0;
+    template <typename T>
+    T *take(size_t count) {
+        T *p = (T *)(base + off);
+        off += al256(sizeof(T) * count);
+        return p;
+    }
+};
+}  // namespace
+
+extern "C" {
+
+alens_status alens_bcqp_solve_static(alens_ctx *ctx, int32_t n, int64_t m, const int32_t *I, const int32_t *J,
+                                     const double *nrm, const double *r_i, const double *r_j, const double *b,
+                                     const double *mob, const double *x0, double *gamma_out,
+                                     alens_solve_report *rep_out) {
+    if (!ctx) return ALENS_E_ARG;
+    if (n < 0 || m < 0 || (m > 0 && (!I || !J || !nrm || !r_i || !r_j || !b || !mob || !gamma_out)))
+        return fail(ctx, ALENS_E_ARG, "bcqp_solve_static: bad arguments");
+    if ((size_t)m > kStaticMaxRows || (size_t)n > kStaticMaxRods)
+        return fail(ctx, ALENS_E_ARG,
+                    "bcqp_solve_static: at most " + std::to_string(kStaticMaxRows) + " rows and " +
+                        std::to_string(kStaticMaxRods) + " rods (use alens_bcqp_solve for larger problems)");
+    alens_solve_report rep{};
+    if (m == 0) {
+        rep.converged = 1;
+        if (rep_out) *rep_out = rep;
+        return ALENS_OK;
+    }
+    for (int64_t k = 0; k < m; ++k)
+        if (I[k] < 0 || I[k] >= n || J[k] < 0 || J[k] >= n || I[k] == J[k])
+            return fail(ctx, ALENS_E_ARG, "bcqp_solve_static: rod index out of range");
+    std::vector<uint32_t> deg;
+    if (HostAdjacency::slots(n, m, I, J, deg) > kStaticMaxSlots)
+        return fail(ctx, ALENS_E_ARG,
+                    "bcqp_solve_static: the rows' padded adjacency needs more than " + std::to_string(kStaticMaxSlots) +
+                        " slots (a few rods with very many rows); use alens_bcqp_solve");
+    TRY_HIP(hipSetDevice(ctx->dev));
+    unsigned char *base = nullptr;
+    TRY_HIP(scratch_on(ctx->dev, &base));
+    HostAdjacency h;
+    h.build(n, m, I, J, nrm, r_i, r_j);
+    const size_t M = (size_t)m, N = (size_t)n, ep = h.ep, nsl = h.sstart.size() - 1;
+    const uint32_t nch = (uint32_t)((M + kRedChunk - 1) / kRedChunk);
+    Carve cv{base};
+    uint32_t *adjStart = cv.take<uint32_t>(kStaticMaxRods + 1);
+    uint32_t *sellStart = cv.take<uint32_t>(kStaticMaxRods / kSlice + 2);
+    uint32_t *sellK = cv.take<uint32_t>(kStaticMaxSlots);
+    double *sellW = cv.take<double>(6 * kStaticMaxSlots);
+    double *geo = cv.take<double>(9 * kStaticMaxRows);
+    int32_t *cI = cv.take<int32_t>(kStaticMaxRows);
+    int32_t *cJ = cv.take<int32_t>(kStaticMaxRows);
+    double *bd = cv.take<double>(kStaticMaxRows);
+    double *x0d = cv.take<double>(kStaticMaxRows);
+    double *gam = cv.take<double>(kStaticMaxRows);
+    double2 *xg = cv.take<double2>(kStaticMaxRows);
+    double *Us = cv.take<double>(6 * kStaticMaxRods);
+    double *mobd = cv.take<double>(6 * kStaticMaxRods);
+    int32_t *rsys = cv.take<int32_t>(kStaticMaxRods);
+    int32_t *smono = cv.take<int32_t>(2);
+    uint32_t *con_off = cv.take<uint32_t>(2);
+    uint32_t *chunk_off = cv.take<uint32_t>(2);
+    cv.take<uint32_t>(2);
+    int32_t *chunk_sys = cv.take<int32_t>(kStaticMaxChunks);
+    double *part = cv.take<double>((size_t)kRedQ * kStaticMaxChunks);
+    uint32_t *bar = cv.take<uint32_t>(kBarTotalWords);
+    double *out = cv.take<double>(16);
+    if (cv.off > kScratchBytes) return fail(ctx, ALENS_E_ARG, "bcqp_solve_static: scratch layout overflow");
+    hipStream_t st = ctx->st;
+    // the previous call's contents stay in the array: every part read below is written first
+    const uint32_t coff[2] = {0u, (uint32_t)M}, choff[2] = {0u, nch};
+    TRY_HIP(hipMemcpyAsync(adjStart, h.start.data(), sizeof(uint32_t) * (N + 1), hipMemcpyHostToDevice, st));
+    TRY_HIP(hipMemcpyAsync(sellStart, h.sstart.data(), sizeof(uint32_t) * (nsl + 1), hipMemcpyHostToDevice, st));
+    TRY_HIP(hipMemcpyAsync(sellK, h.sk.data(), sizeof(uint32_t) * ep, hipMemcpyHostToDevice, st));
+    for (int c = 0; c < 6; ++c)  // component stride ep (the padded slot count of this call)
+        TRY_HIP(hipMemcpyAsync(sellW + c * ep, h.sw.data() + c * ep, sizeof(double) * ep, hipMemcpyHostToDevice, st));
+    TRY_HIP(hipMemcpyAsync(geo, h.geo.data(), sizeof(double) * 9 * M, hipMemcpyHostToDevice, st));
+    TRY_HIP(hipMemcpyAsync(cI, I, sizeof(int32_t) * M, hipMemcpyHostToDevice, st));
+    TRY_HIP(hipMemcpyAsync(cJ, J, sizeof(int32_t) * M, hipMemcpyHostToDevice, st));
+    TRY_HIP(hipMemcpyAsync(bd, b, sizeof(double) * M, hipMemcpyHostToDevice, st));
+    TRY_HIP(hipMemcpyAsync(mobd, mob, sizeof(double) * 6 * N, hipMemcpyHostToDevice, st));
+    if (x0) TRY_HIP(hipMemcpyAsync(x0d, x0, sizeof(double) * M, hipMemcpyHostToDevice, st));
+    TRY_HIP(hipMemcpyAsync(con_off, coff, sizeof coff, hipMemcpyHostToDevice, st));
+    TRY_HIP(hipMemcpyAsync(chunk_off, choff, sizeof choff, hipMemcpyHostToDevice, st));
+    ZeroJobs z{};
+    z.p[0] = (uint32_t *)rsys;
+    z.n[0] = (int64_t)N;
+    z.p[1] = (uint32_t *)smono;
+    z.n[1] = 1;
+    z.p[2] = (uint32_t *)chunk_sys;
+    z.n[2] = nch;
+    z.count = 3;
+    TRY_K(launch_zero_many(st, z));  // one system: every rod and chunk in system 0, no monolayer
+    TRY_K(launch_init_xg(st, xg, x0 ? x0d : nullptr, (uint32_t)M));
+    SolveArgsT<double> a{};
+    a.n = n;
+    a.m = (uint32_t)M;
+    a.adjStart = adjStart;
+    a.sellStart = sellStart;
+    a.sord = nullptr;  // rods in index order
+    a.sellK = sellK;
+    a.sellW = sellW;
+    a.epad = ep;
+    a.mob = mobd;
+    a.cI = cI;
+    a.cJ = cJ;
+    a.geo = geo;
+    a.b = bd;
+    a.xg = xg;
+    a.U = Us;
+    a.rsys = rsys;
+    a.smono = smono;
+    a.warm = x0 ? 1 : 0;
+    // the solver reads the context's system and chunk counts; this call's are one system, nch chunks
+    const int32_t ns_keep = ctx->nsys_cur, nch_keep = ctx->nchunks_cur;
+    const bool cap_keep = ctx->capture;
+    const uint32_t *md_keep = ctx->cur_mdev;
+    const int32_t *nd_keep = ctx->cur_nchdev;
+    ctx->nsys_cur = 1;
+    ctx->nchunks_cur = (int32_t)nch;
+    ctx->capture = false;
+    ctx->cur_mdev = nullptr;
+    ctx->cur_nchdev = nullptr;
+    float ms = 0.f;
+    const SolveMem mem{part, out, bar, con_off, chunk_off, chunk_sys};
+    alens_status s = run_bbpgd_on<double>(ctx, a, mem, &rep, &ms, nullptr);
+    ctx->nsys_cur = ns_keep;
+    ctx->nchunks_cur = nch_keep;
+    ctx->capture = cap_keep;
+    ctx->cur_mdev = md_keep;
+    ctx->cur_nchdev = nd_keep;
+    if (s != ALENS_OK) return s;
+    TRY_K(launch_writeback(st, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, xg, nullptr, nullptr,
+                           nullptr, nullptr, gam, (uint32_t)M));
+    TRY_HIP(hipMemcpyAsync(gamma_out, gam, sizeof(double) * M, hipMemcpyDeviceToHost, st));
+    TRY_HIP(hipStreamSynchronize(st));
+    rep.n_con = m;
+    rep.t_solve_ms = ms;
+    rep.t_step_ms = ms;
     if (rep_out) *rep_out = rep;
     if (rep.status == ALENS_E_NONFINITE) return fail(ctx, ALENS_E_NONFINITE, "non-finite BBPGD residual");
     if (rep.status == ALENS_E_NOCONV) return fail(ctx, ALENS_E_NOCONV, "BBPGD did not reach tol within max_ite");

@@ -216,6 +216,21 @@ alens_status alens_bcqp_solve(alens_ctx *ctx, int32_t n, int64_t m, const int32_
                               const double *r_j, const double *b, const double *mob,
                               const double *x0, double *gamma_out, alens_solve_report *rep);
 
+/* Allocation-free variant of alens_bcqp_solve (no counterpart in the reference): same arguments and
+ * results, bit-identical gamma and report, but every device buffer it uses (staged rows, sliced
+ * adjacency, iterate, velocities, partial sums, barrier and report words) lives in one static device
+ * array of the library, so the call makes no hipMalloc / hipFree.  It is for callers that cannot
+ * allocate device memory: serving loops with a fixed footprint, harnesses that forbid allocation,
+ * code that must not grow the device heap.  Limits: m <= 131072 rows, n <= 65536 rods and at most
+ * 524288 padded adjacency slots (64 rods per slice times the slice's largest degree); larger problems
+ * are refused with ALENS_E_ARG.  The array (about 50 MB) is held on every GPU the library runs on until
+ * the process ends.  All calls on one GPU share it: calls that overlap on one GPU (from several
+ * threads or contexts) race and must be serialised by the caller.  The call waits for its result. */
+alens_status alens_bcqp_solve_static(alens_ctx *ctx, int32_t n, int64_t m, const int32_t *I,
+                                     const int32_t *J, const double *nrm, const double *r_i,
+                                     const double *r_j, const double *b, const double *mob,
+                                     const double *x0, double *gamma_out, alens_solve_report *rep);
+
 /* ---- learned warm start (no counterpart in the reference: the reference starts its BBPGD from the
  * constraint blocks' gamma guess, SRC/TubuleSystem.cpp:722,728) ----
  * Network 12 -> 64 -> 64 -> 64 -> 1, SiLU hidden layers, softplus output, BF16 MFMA with FP32
