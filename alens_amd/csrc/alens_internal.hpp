@@ -251,8 +251,7 @@ struct SolveArgsT {
     const T *lb;                 // NULL = all zero
     const T *invk;               // NULL = no compliance; else A += diag(invk)
     typename Vec2<T>::type *xg;  // (x, g) per constraint; x holds the start point on entry
-    typename Vec2<T>::type *hh;  // per constraint, scratch: the halves of (D^T U)_k that rods I and J
-                                 // contribute (phase A writes them, phase B adds them)
+    T *U;                        // kUStride n scratch (rod velocities, padded rows)
     double *part;                // kRedQ * nchunks partials
     uint32_t *bar;               // barrier words (zeroed before launch)
     double *out;                 // per-system report words (8 per system)
@@ -298,7 +297,8 @@ HipErr launch_make_keys(hipStream_t st, uint32_t m, const int32_t *cI, const int
                         const uint32_t *md = nullptr, uint32_t *mprev_d = nullptr);
 
 constexpr int kRedQ = 8;  // partial-sum slots per chunk
-
+// values per solver velocity row (padding rows to 8 measured slower than the compact 6)
+constexpr int kUStride = 6;
 // grid-barrier words of the persistent solver (layout in k_solve.hip)
 constexpr int kBarWords = 32;
 constexpr int kBarGroups = 8;

@@ -49,8 +49,7 @@ struct alens_ctx {
     DBuf<double> center, dir, length, radius, hcol, rcol;
     DBuf<uint8_t> immov;
     bool has_immov = false;
-    DBuf<double> mob, unc, U, F, fext;
-    DBuf<double2> hh;                      // solver scratch: the rods' halves of D^T U per constraint
+    DBuf<double> mob, unc, U, F, Us, fext;
 
     DBuf<int32_t> clo, chi;
     DBuf<double> blo, bhi;
@@ -77,8 +76,7 @@ struct alens_ctx {
     DBuf<float> lb32, invk32;
     uint32_t mcol = 0, mext = 0;  // collision rows, caller rows of the last step
     // precision form: FP32 copies of the operator data and the FP32 iterate
-    DBuf<float> geo32, sellW32, mob32, b32;
-    DBuf<float2> hh32;
+    DBuf<float> geo32, sellW32, mob32, b32, U32;
     DBuf<float2> xg32;
     // learned warm start: FP32 master parameters, normalisation, features of the step
     DBuf<float> mlpP, mlpNorm, feat;
@@ -263,7 +261,7 @@ void alens_destroy(alens_ctx *ctx) {
     if (ctx->st) (void)hipStreamSynchronize(ctx->st);
     auto rel = [](auto &b) { b.release(); };
     rel(ctx->gid); rel(ctx->center); rel(ctx->dir); rel(ctx->length); rel(ctx->radius); rel(ctx->hcol);
-    rel(ctx->rcol); rel(ctx->immov); rel(ctx->mob); rel(ctx->unc); rel(ctx->U); rel(ctx->F); rel(ctx->hh);
+    rel(ctx->rcol); rel(ctx->immov); rel(ctx->mob); rel(ctx->unc); rel(ctx->U); rel(ctx->F); rel(ctx->Us);
     rel(ctx->fext); rel(ctx->clo); rel(ctx->chi); rel(ctx->blo); rel(ctx->bhi); rel(ctx->ecnt);
     rel(ctx->eoff); rel(ctx->keys); rel(ctx->vals); rel(ctx->keys2); rel(ctx->vals2); rel(ctx->cstart);
     rel(ctx->cend); rel(ctx->hist);
@@ -277,7 +275,7 @@ void alens_destroy(alens_ctx *ctx) {
     if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
     rel(ctx->extStart); rel(ctx->extEnt); rel(ctx->extNrm); rel(ctx->extX0); rel(ctx->lb); rel(ctx->invk);
-    rel(ctx->lb32); rel(ctx->invk32); rel(ctx->geo32); rel(ctx->sellW32); rel(ctx->mob32); rel(ctx->b32); rel(ctx->hh32); rel(ctx->xg32);
+    rel(ctx->lb32); rel(ctx->invk32); rel(ctx->geo32); rel(ctx->sellW32); rel(ctx->mob32); rel(ctx->b32); rel(ctx->U32); rel(ctx->xg32);
     rel(ctx->mlpP); rel(ctx->mlpNorm); rel(ctx->feat); rel(ctx->xprev);
     rel(ctx->sysd); rel(ctx->rsys); rel(ctx->smono); rel(ctx->chunk_sys); rel(ctx->con_off); rel(ctx->chunk_off);
     for (auto &e : ctx->ev)
@@ -383,6 +381,7 @@ alens_status alens_set_systems(alens_ctx *ctx, int32_t nsys, const int32_t *rod_
     TRY_HIP(ctx->rcol.ensure(N));
     TRY_HIP(ctx->rsys.ensure(N));
     for (auto *bf : {&ctx->mob, &ctx->unc, &ctx->U, &ctx->F, &ctx->fext}) TRY_HIP(bf->ensure(6 * N));
+    TRY_HIP(ctx->Us.ensure(kUStride * N));
     TRY_HIP(ctx->clo.ensure(3 * N));
     TRY_HIP(ctx->chi.ensure(3 * N));
     TRY_HIP(ctx->blo.ensure(3 * N));
@@ -802,8 +801,7 @@ static alens_status run_solver(alens_ctx *ctx, int32_t n, uint32_t m, const doub
     a.lb = ctx->mext ? ctx->lb.p : nullptr;
     a.invk = ctx->mext ? ctx->invk.p : nullptr;
     a.xg = ctx->xg.p;
-    TRY_HIP(ctx->hh.ensure(std::max<size_t>(ctx->gstride, 1)));
-    a.hh = ctx->hh.p;
+    a.U = ctx->Us.p;
     a.rsys = rsys;
     a.smono = smono;
     a.warm = warm ? 1 : 0;
@@ -822,7 +820,7 @@ static alens_status run_solver_mixed(alens_ctx *ctx, int32_t n, uint32_t m, bool
     TRY_HIP(ctx->b32.ensure(M));
     TRY_HIP(ctx->xg32.ensure(M));
     TRY_HIP(ctx->mob32.ensure(6 * N));
-    TRY_HIP(ctx->hh32.ensure(M));
+    TRY_HIP(ctx->U32.ensure(kUStride * N));
     TRY_K(launch_to_f32(st, ctx->geo.p, ctx->geo32.p, 9 * (size_t)m));
     TRY_K(launch_to_f32(st, ctx->sellW.p, ctx->sellW32.p, 6 * (size_t)ctx->epad));
     TRY_K(launch_to_f32(st, ctx->b.p, ctx->b32.p, (size_t)m));
@@ -851,7 +849,7 @@ static alens_status run_solver_mixed(alens_ctx *ctx, int32_t n, uint32_t m, bool
     a.lb = ctx->mext ? ctx->lb32.p : nullptr;
     a.invk = ctx->mext ? ctx->invk32.p : nullptr;
     a.xg = ctx->xg32.p;
-    a.hh = ctx->hh32.p;
+    a.U = ctx->U32.p;
     a.rsys = ctx->rsys.p;
     a.smono = ctx->smono.p;
     a.warm = warm ? 1 : 0;
@@ -1202,7 +1200,6 @@ static alens_status plan_resident(alens_ctx *ctx) {
     TRY_HIP(ctx->posI.ensure(3 * (size_t)c.M));
     TRY_HIP(ctx->posJ.ensure(3 * (size_t)c.M));
     TRY_HIP(ctx->xg.ensure(c.M));
-    TRY_HIP(ctx->hh.ensure(c.M));
     TRY_HIP(ctx->xprev.ensure(c.M));
     TRY_HIP(grow_keep(ctx->pkey, c.M));
     TRY_HIP(ctx->slen.ensure((size_t)nsl + 1));
@@ -1234,7 +1231,7 @@ static alens_status plan_resident(alens_ctx *ctx) {
         TRY_HIP(ctx->b32.ensure(c.M));
         TRY_HIP(ctx->xg32.ensure(c.M));
         TRY_HIP(ctx->mob32.ensure(6 * N));
-        TRY_HIP(ctx->hh32.ensure(c.M));
+        TRY_HIP(ctx->U32.ensure(kUStride * N));
     }
     ctx->caps = c;
     return ALENS_OK;
@@ -1771,7 +1768,7 @@ alens_status alens_bcqp_solve(alens_ctx *ctx, int32_t n, int64_t m, const int32_
     TRY_HIP(ctx->b.ensure(M));
     TRY_HIP(ctx->xg.ensure(M));
     TRY_HIP(ctx->gamma.ensure(M));
-    TRY_HIP(ctx->hh.ensure(M));
+    TRY_HIP(ctx->Us.ensure(kUStride * N));
     TRY_HIP(ctx->fext.ensure(6 * N));
     TRY_HIP(ctx->bar.ensure(kBarTotalWords));
     TRY_HIP(ctx->out.ensure(16));
@@ -1820,7 +1817,7 @@ alens_status alens_bcqp_solve(alens_ctx *ctx, int32_t n, int64_t m, const int32_
 // ---- allocation-free BCQP solve: every device buffer lives in one static device array ----
 // Sizes: rows, rods and padded adjacency slots of one call.  scratch holds, each part 256-byte
 // aligned: adjStart (n + 1 u32), sellStart (n / 64 + 2 u32), sellK (S u32), sellW (6 S f64), geo (9 M f64),
-// cI, cJ (M i32), b, x0, gamma (M f64), xg and hh (M x 2 f64), mob (6 N f64), rsys (N i32), smono,
+// cI, cJ (M i32), b, x0, gamma (M f64), xg (M x 2 f64), Us and mob (6 N f64), rsys (N i32), smono,
 // con_off, chunk_off (2 u32 each), chunk_sys (chunks i32), part (8 x chunks f64), bar (barrier words),
 // out (16 f64).
 constexpr size_t kStaticMaxRows = size_t(1) << 17;
@@ -1831,7 +1828,7 @@ static constexpr size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
 constexpr size_t kScratchBytes =
     al256(4 * (kStaticMaxRods + 1)) + al256(4 * (kStaticMaxRods / kSlice + 2)) + al256(4 * kStaticMaxSlots) +
     al256(48 * kStaticMaxSlots) + al256(72 * kStaticMaxRows) + 2 * al256(4 * kStaticMaxRows) +
-    3 * al256(8 * kStaticMaxRows) + 2 * al256(16 * kStaticMaxRows) + al256(48 * kStaticMaxRods) +
+    3 * al256(8 * kStaticMaxRows) + al256(16 * kStaticMaxRows) + 2 * al256(48 * kStaticMaxRods) +
     al256(4 * kStaticMaxRods) + 4 * al256(8) + al256(4 * kStaticMaxChunks) + al256(8 * kRedQ * kStaticMaxChunks) +
     al256(4 * kBarTotalWords) + al256(8 * 16);
 
@@ -1917,7 +1914,7 @@ alens_status alens_bcqp_solve_static(alens_ctx *ctx, int32_t n, int64_t m, const
     double *x0d = cv.take<double>(kStaticMaxRows);
     double *gam = cv.take<double>(kStaticMaxRows);
     double2 *xg = cv.take<double2>(kStaticMaxRows);
-    double2 *hh = cv.take<double2>(kStaticMaxRows);
+    double *Us = cv.take<double>(6 * kStaticMaxRods);
     double *mobd = cv.take<double>(6 * kStaticMaxRods);
     int32_t *rsys = cv.take<int32_t>(kStaticMaxRods);
     int32_t *smono = cv.take<int32_t>(2);
@@ -1970,7 +1967,7 @@ alens_status alens_bcqp_solve_static(alens_ctx *ctx, int32_t n, int64_t m, const
     a.geo = geo;
     a.b = bd;
     a.xg = xg;
-    a.hh = hh;
+    a.U = Us;
     a.rsys = rsys;
     a.smono = smono;
     a.warm = x0 ? 1 : 0;

@@ -359,14 +359,9 @@ __device__ inline void walk_entries(const SolveArgsT<T> &a, size_t idx0, uint32_
     }
 }
 
-// The rod's halves of D^T U: with U = M F of the rod, each adjacency entry (constraint k, weights w =
-// (n, r) on the I side, (-n, -r_J) on the J side) gets h = (w0 u0 + w1 u1 + w2 u2) + (w3 u3 + w4 u4 +
-// w5 u5), stored to hh[k].x (I side) or hh[k].y (J side).  Phase B adds them: hh[k].x + hh[k].y equals
-// (n.v_I + r_I.w_I) - (n.v_J + r_J.w_J), the row of D^T U in dt_row's order, to the bit (negating
-// every weight negates each product and sum exactly, and a - b is a + (-b)).  Phase B thus reads two
-// words per constraint instead of the constraint geometry and both rods' velocities.
-template <typename T, int kAG>
-__device__ inline void rod_halves(const SolveArgsT<T> &a, const RodMeta &rm, const T (&f)[6]) {
+// U = M F of a rod
+template <typename T>
+__device__ inline void store_U(const SolveArgsT<T> &a, const RodMeta &rm, const T (&f)[6]) {
     using V2 = typename Vec2<T>::type;
     const size_t i = (size_t)rm.rod;
     const V2 *m2 = reinterpret_cast<const V2 *>(a.mob + 6 * i);
@@ -374,28 +369,10 @@ __device__ inline void rod_halves(const SolveArgsT<T> &a, const RodMeta &rm, con
     const T mb[6] = {m01.x, m01.y, m23.x, m23.y, m45.x, m45.y};
     T u[6];
     apply_M(mb, f, a.smono[rm.sy], u);
-    const size_t ep = a.epad;
-    T *h = reinterpret_cast<T *>(a.hh);
-    size_t idx = rm.idx;
-    for (uint32_t j = 0; j < rm.deg; j += kAG, idx += kAG * kSlice) {
-        uint32_t kk[kAG];
-        T w[kAG][6];
-#pragma unroll
-        for (int g = 0; g < kAG; ++g) {
-            const bool on = j + g < rm.deg;
-            const size_t ig = idx + g * kSlice;
-            kk[g] = on ? a.sellK[ig] : 0u;
-#pragma unroll
-            for (int c = 0; c < 6; ++c) w[g][c] = on ? a.sellW[c * ep + ig] : T(0);
-        }
-#pragma unroll
-        for (int g = 0; g < kAG; ++g) {
-            if (j + g >= rm.deg) break;
-            const T t = w[g][0] * u[0] + w[g][1] * u[1] + w[g][2] * u[2];
-            const T r = w[g][3] * u[3] + w[g][4] * u[4] + w[g][5] * u[5];
-            h[2 * (size_t)(kk[g] & 0x7fffffffu) + (kk[g] >> 31)] = t + r;
-        }
-    }
+    V2 *U2 = reinterpret_cast<V2 *>(a.U + kUStride * i);
+    U2[0] = mk2<T>(u[0], u[1]);
+    U2[1] = mk2<T>(u[2], u[3]);
+    U2[2] = mk2<T>(u[4], u[5]);
 }
 
 // Split of heavy rods (one slice position per lane): in a block, the rod at position t < kSplitN whose
@@ -410,8 +387,8 @@ struct SplitMeta {
     int sy;
 };
 
-// phase A: per rod, F = sum over incident constraints (ascending k) of v_k * w_e; U = M F and the
-// rod's halves of D^T U per incident constraint (rods of systems that have stopped are skipped).  Lane p % 64 of a wave walks the entries of the
+// phase A: per rod, F = sum over incident constraints (ascending k) of v_k * w_e; U = M F
+// (rods of systems that have stopped are skipped).  Lane p % 64 of a wave walks the entries of the
 // rod at slice position p, so each step of the 64 lanes reads consecutive words.
 // rod metadata of a lane's passes 1 .. kPasses - 1 (pass 0 is kept in registers), fixed through a
 // solve: a pass of a stopped system costs one LDS read instead of five dependent global loads
@@ -451,7 +428,7 @@ __device__ inline void phase_A(const SolveArgsT<T> &a, const double *s_alpha, co
 #pragma unroll
                 for (int c = 0; c < 6; ++c) s_prod[t][kSplitMax - 1][c] = f[c];
             } else {
-                rod_halves<T, kAG>(a, rm0, f);
+                store_U(a, rm0, f);
             }
         }
         __syncthreads();
@@ -466,7 +443,7 @@ __device__ inline void phase_A(const SolveArgsT<T> &a, const double *s_alpha, co
                 for (int c = 0; c < 6; ++c) f[c] = f[c] + s_prod[t][q][c];
             }
             walk_entries<MODE, T, kAG, false>(a, rm0.idx, jh, rm0.deg, (T)s_alpha[rm0.sy], f, nullptr);
-            rod_halves<T, kAG>(a, rm0, f);
+            store_U(a, rm0, f);
         }
         return;
     }
@@ -476,10 +453,31 @@ __device__ inline void phase_A(const SolveArgsT<T> &a, const double *s_alpha, co
         if (s_state[rm.sy] != S_ACTIVE) continue;
         T f[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
         walk_entries<MODE, T, kAG, false>(a, rm.idx, 0, rm.deg, (T)s_alpha[rm.sy], f, nullptr);
-        rod_halves<T, kAG>(a, rm, f);
+        store_U(a, rm, f);
     }
 }
 
+
+template <typename T>
+__device__ inline T du_row(const SolveArgsT<T> &a, uint32_t k) {
+    using V2 = typename Vec2<T>::type;
+    T gk[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) gk[q] = a.geo[(size_t)q * a.m + k];
+    const size_t I = (size_t)a.cI[k], J = (size_t)a.cJ[k];
+    const V2 *UI = reinterpret_cast<const V2 *>(a.U + kUStride * I);
+    const V2 *UJ = reinterpret_cast<const V2 *>(a.U + kUStride * J);
+    T uI[6], uJ[6];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const V2 p = UI[q], r = UJ[q];
+        uI[2 * q] = p.x;
+        uI[2 * q + 1] = p.y;
+        uJ[2 * q] = r.x;
+        uJ[2 * q + 1] = r.y;
+    }
+    return dt_row(gk, uI, uJ);
+}
 
 enum : int { B_INIT = 0, B_ALPHA0 = 1, B_ITER = 2 };
 
@@ -591,10 +589,9 @@ __device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, co
                     ok[q] = kk[q] < cm.k1;
                     if (ok[q]) {
                         const typename Vec2<T>::type xg = a.xg[kk[q]];
-                        const typename Vec2<T>::type hv = a.hh[kk[q]];
                         xo[q] = xg.x;
                         go[q] = xg.y;
-                        du[q] = hv.x + hv.y;  // (D^T U)_k from the two rods' halves
+                        du[q] = du_row<T>(a, kk[q]);
                     }
                 }
             }
