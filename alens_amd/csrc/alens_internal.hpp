@@ -314,7 +314,9 @@ constexpr int kMlpParams = 9473;
 constexpr int kMlpNorm = 2 * kMlpIn + 1;  // mean[12], 1/std[12], gamma scale
 struct MlpScratch {
     float *XT, *Z1, *H1, *Z2, *H2, *Z3, *H3, *dO, *dZ1, *dZ2, *dZ3, *loss, *lossv;
+    float *kpart;  // K-slice partial products of the weight-gradient GEMMs (kMlpSplitFloats)
 };
+constexpr size_t kMlpSplitFloats = 256 * 64 * 64;  // at most 256 K-slices of a 64 x 64 product
 HipErr launch_features(hipStream_t st, uint32_t m, uint32_t gs, const int32_t *cI, const int32_t *cJ, const double *delta0,
                        const double *geo, const double *posI, const double *posJ, const double *dir,
                        const double *length, const double *unc, const double2 *xprev, double dt, float *X,
@@ -323,8 +325,9 @@ HipErr launch_mlp_infer_f32(hipStream_t st, uint32_t m, const float *X, const fl
                             float *out);
 HipErr launch_mlp_infer_xg(hipStream_t st, uint32_t m, const float *X, const float *P, const float *norm,
                            double2 *xg, const uint32_t *md = nullptr);
+// part: K-slice partials (ksplit > 1), summed in order into C; NULL = one slice
 HipErr launch_gemm_bf16(hipStream_t st, int M, int N, int K, const float *A, int lda, int tA, const float *B,
-                        int ldb, int tB, float *C, int ldc, int ksplit);
+                        int ldb, int tB, float *C, int ldc, int ksplit, float *part);
 HipErr launch_gather_batch(hipStream_t st, int B, const int32_t *idx, const float *X, const float *t, float *XB,
                            float *tB);
 HipErr mlp_gradient(hipStream_t st, int B, const float *X, const float *t, const float *P, const float *norm,

@@ -2069,7 +2069,7 @@ alens_status alens_mlp_predict(alens_ctx *ctx, int64_t m, const float *X, float 
 
 namespace {
 struct TrainBufs {
-    DBuf<float> XT, Z1, H1, Z2, H2, Z3, H3, dO, dZ1, dZ2, dZ3, loss, lossv;
+    DBuf<float> XT, Z1, H1, Z2, H2, Z3, H3, dO, dZ1, dZ2, dZ3, loss, lossv, kpart;
     hipError_t ensure(int B) {
         const size_t nb = (size_t)64 * B;
         hipError_t e;
@@ -2078,13 +2078,14 @@ struct TrainBufs {
             if ((e = b->ensure(nb)) != hipSuccess) return e;
         if ((e = dO.ensure((size_t)B)) != hipSuccess) return e;
         if ((e = lossv.ensure((size_t)B)) != hipSuccess) return e;
+        if ((e = kpart.ensure(kMlpSplitFloats)) != hipSuccess) return e;
         return loss.ensure(4);
     }
     MlpScratch view() {
-        return MlpScratch{XT.p, Z1.p, H1.p, Z2.p, H2.p, Z3.p, H3.p, dO.p, dZ1.p, dZ2.p, dZ3.p, loss.p, lossv.p};
+        return MlpScratch{XT.p, Z1.p, H1.p, Z2.p, H2.p, Z3.p, H3.p, dO.p, dZ1.p, dZ2.p, dZ3.p, loss.p, lossv.p, kpart.p};
     }
     void release() {
-        for (auto *b : {&XT, &Z1, &H1, &Z2, &H2, &Z3, &H3, &dO, &dZ1, &dZ2, &dZ3, &loss, &lossv}) b->release();
+        for (auto *b : {&XT, &Z1, &H1, &Z2, &H2, &Z3, &H3, &dO, &dZ1, &dZ2, &dZ3, &loss, &lossv, &kpart}) b->release();
     }
 };
 }  // namespace

@@ -6,10 +6,12 @@
 // gamma0 = gscale * output.  Parameters live in one FP32 master array (layout kP*); the MFMA operands
 // are the BF16 roundings of the master values.
 //
-// MFMA: v_mfma_f32_16x16x16_bf16 (builtin ..._16x16x16bf16_1k).  Lane l holds A[l % 16][4 (l / 16) + i],
-// B[4 (l / 16) + i][l % 16] and D[4 (l / 16) + i][l % 16], i = 0..3.  Constraints run along N, so a
-// layer's output tile (features along rows 4 (l / 16) + i, constraint l % 16) is already laid out as
-// the next layer's B operand.
+// MFMA, inference: v_mfma_f32_16x16x16_bf16 (builtin ..._16x16x16bf16_1k).  Lane l holds
+// A[l % 16][4 (l / 16) + i], B[4 (l / 16) + i][l % 16] and D[4 (l / 16) + i][l % 16], i = 0..3.
+// Constraints run along N, so a layer's output tile (features along rows 4 (l / 16) + i, constraint
+// l % 16) is already laid out as the next layer's B operand and stays in registers from layer to layer.
+// MFMA, training GEMMs: gfx950's v_mfma_f32_16x16x32_bf16 where K is a multiple of 32 (lane l holds
+// A[l % 16][8 (l / 16) + i], B[8 (l / 16) + i][l % 16], i = 0..7, and D as above), else 16x16x16.
 #include "alens_dev.hpp"
 #include "alens_internal.hpp"
 
@@ -24,6 +26,7 @@ namespace alens {
 static inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ inline short f2bf(float f) {
@@ -37,6 +40,10 @@ __device__ inline short f2bf(float f) {
 __device__ inline f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
+__device__ inline f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ inline __bf16 f2bf16(float f) { return __builtin_bit_cast(__bf16, f2bf(f)); }
 
 __device__ inline float silu(float z) { return z / (1.0f + __expf(-z)); }
 __device__ inline float silu_grad(float z) {
@@ -207,14 +214,15 @@ HipErr launch_mlp_infer_xg(hipStream_t st, uint32_t m, const float *X, const flo
 }
 
 // ------------------------------------------------------------------------------------------------
-// training: generic BF16 MFMA GEMM  C (M x N) (+)= op(A) (M x K) * op(B) (K x N), FP32 accumulate.
+// training: generic BF16 MFMA GEMM  C (M x N) = op(A) (M x K) * op(B) (K x N), FP32 accumulate.
 // op(A)(m, k) = tA ? A[k * lda + m] : A[m * lda + k];  op(B)(k, n) = tB ? B[n * ldb + k] : B[k * ldb + n].
-// One wave per 16 x 16 tile of C and K-slice; atomic accumulation when the K range is split.
-// M, N, K are multiples of 16.
+// One wave per 16 x 16 tile of C and K-slice.  When the K range is split, slice y writes its tile to
+// part[y] and k_sum_slices adds the slices in order (the result does not depend on scheduling).
+// M, N, K are multiples of 16; K slices of a multiple of 32 use the 16x16x32 MFMA.
 // ------------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_gemm_bf16(int M, int N, int K, const float *__restrict__ A, int lda, int tA,
                                                   const float *__restrict__ B, int ldb, int tB, float *__restrict__ C,
-                                                  int ldc, int kslice, int atomic) {
+                                                  int ldc, int kslice, float *__restrict__ part) {
     const int lane = threadIdx.x;
     const int r16 = lane & 15, q = lane >> 4;
     const int tilesN = N / 16;
@@ -224,7 +232,18 @@ __global__ __launch_bounds__(64) void k_gemm_bf16(int M, int N, int K, const flo
     if (k1 > K) k1 = K;
     const int m = 16 * tm + r16, n = 16 * tn + r16;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k = k0; k < k1; k += 16) {
+    int k = k0;
+    for (; k + 32 <= k1; k += 32) {
+        bf16x8 a, b;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int kk = k + 8 * q + i;
+            a[i] = f2bf16(tA ? A[(size_t)kk * lda + m] : A[(size_t)m * lda + kk]);
+            b[i] = f2bf16(tB ? B[(size_t)n * ldb + kk] : B[(size_t)kk * ldb + n]);
+        }
+        acc = mfma32(a, b, acc);
+    }
+    for (; k < k1; k += 16) {
         bf16x4 a, b;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -234,27 +253,43 @@ __global__ __launch_bounds__(64) void k_gemm_bf16(int M, int N, int K, const flo
         }
         acc = mfma16(a, b, acc);
     }
+    float *dst = part ? part + (size_t)blockIdx.y * M * N : C;
+    const int ld = part ? N : ldc;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int row = 16 * tm + 4 * q + i;
-        float *dst = C + (size_t)row * ldc + n;
-        if (atomic)
-            atomicAdd(dst, acc[i]);
-        else
-            *dst = acc[i];
-    }
+    for (int i = 0; i < 4; ++i) dst[(size_t)(16 * tm + 4 * q + i) * ld + n] = acc[i];
+}
+
+// C[r][c] = part[0][r][c] + part[1][r][c] + ... (slices in order)
+__global__ void k_sum_slices(int M, int N, int ks, const float *__restrict__ part, float *__restrict__ C, int ldc) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= M * N) return;
+    float s = 0.0f;
+    for (int y = 0; y < ks; ++y) s += part[(size_t)y * M * N + idx];
+    C[(size_t)(idx / N) * ldc + idx % N] = s;
+}
+
+// K-slices of a split GEMM: a multiple of 32 at least 32 (the 16x16x32 MFMA), at most kMaxSplit slices
+constexpr int kMaxSplit = 256;
+static int gemm_kslice(int K, int ksplit) {
+    if (ksplit <= 1) return K;
+    int kslice = ((K / ksplit + 31) / 32) * 32;
+    if (kslice < 32) kslice = 32;
+    if ((K + kslice - 1) / kslice > kMaxSplit) kslice = ((K / kMaxSplit + 31) / 32 + 1) * 32;
+    return kslice;
 }
 
 HipErr launch_gemm_bf16(hipStream_t st, int M, int N, int K, const float *A, int lda, int tA, const float *B,
-                        int ldb, int tB, float *C, int ldc, int ksplit) {
+                        int ldb, int tB, float *C, int ldc, int ksplit, float *part) {
     if (M % 16 || N % 16 || K % 16) return HipErr{hipErrorInvalidValue, "gemm dims", __FILE__, __LINE__};
-    int kslice = K;
-    if (ksplit > 1) kslice = ((K / ksplit + 15) / 16) * 16;
-    if (kslice < 16) kslice = 16;
+    const int kslice = part ? gemm_kslice(K, ksplit) : K;
     const int ks = (K + kslice - 1) / kslice;
     dim3 grid((unsigned)((M / 16) * (N / 16)), (unsigned)ks);
-    k_gemm_bf16<<<grid, 64, 0, st>>>(M, N, K, A, lda, tA, B, ldb, tB, C, ldc, kslice, ks > 1 ? 1 : 0);
+    k_gemm_bf16<<<grid, 64, 0, st>>>(M, N, K, A, lda, tA, B, ldb, tB, C, ldc, kslice, ks > 1 ? part : nullptr);
     LAUNCH_CHECK();
+    if (ks > 1) {
+        k_sum_slices<<<nblk((int64_t)M * N, 256), 256, 0, st>>>(M, N, ks, part, C, ldc);
+        LAUNCH_CHECK();
+    }
     return hip_ok();
 }
 
@@ -345,15 +380,15 @@ static HipErr fwd_bwd(hipStream_t st, int B, const float *X, const float *t, con
     LAUNCH_CHECK();
     HipErr e;
     // forward (feature-major activations: row = unit, column = sample)
-    e = launch_gemm_bf16(st, 64, B, 16, P + kPW1, 16, 0, s.XT, B, 0, s.Z1, B, 1);
+    e = launch_gemm_bf16(st, 64, B, 16, P + kPW1, 16, 0, s.XT, B, 0, s.Z1, B, 1, nullptr);
     if (!e.ok()) return e;
     k_bias_silu<<<nblk(nb, 256), 256, 0, st>>>(64, B, s.Z1, P + kPB1, s.H1);
     LAUNCH_CHECK();
-    e = launch_gemm_bf16(st, 64, B, 64, P + kPW2, 64, 0, s.H1, B, 0, s.Z2, B, 1);
+    e = launch_gemm_bf16(st, 64, B, 64, P + kPW2, 64, 0, s.H1, B, 0, s.Z2, B, 1, nullptr);
     if (!e.ok()) return e;
     k_bias_silu<<<nblk(nb, 256), 256, 0, st>>>(64, B, s.Z2, P + kPB2, s.H2);
     LAUNCH_CHECK();
-    e = launch_gemm_bf16(st, 64, B, 64, P + kPW3, 64, 0, s.H2, B, 0, s.Z3, B, 1);
+    e = launch_gemm_bf16(st, 64, B, 64, P + kPW3, 64, 0, s.H2, B, 0, s.Z3, B, 1, nullptr);
     if (!e.ok()) return e;
     k_bias_silu<<<nblk(nb, 256), 256, 0, st>>>(64, B, s.Z3, P + kPB3, s.H3);
     LAUNCH_CHECK();
@@ -369,23 +404,23 @@ static HipErr fwd_bwd(hipStream_t st, int B, const float *X, const float *t, con
     k_head_back<<<nblk(nb, 256), 256, 0, st>>>(B, P, s.dO, s.Z3, s.dZ3);
     LAUNCH_CHECK();
     const int ks = B / 1024 > 1 ? B / 1024 : 1;
-    e = launch_gemm_bf16(st, 64, 64, B, s.dZ3, B, 0, s.H2, B, 1, grad + kPW3, 64, ks);
+    e = launch_gemm_bf16(st, 64, 64, B, s.dZ3, B, 0, s.H2, B, 1, grad + kPW3, 64, ks, s.kpart);
     if (!e.ok()) return e;
     k_row_dot<<<64, 256, 0, st>>>(B, s.dZ3, nullptr, grad + kPB3);
     LAUNCH_CHECK();
-    e = launch_gemm_bf16(st, 64, B, 64, P + kPW3, 64, 1, s.dZ3, B, 0, s.dZ2, B, 1);
+    e = launch_gemm_bf16(st, 64, B, 64, P + kPW3, 64, 1, s.dZ3, B, 0, s.dZ2, B, 1, nullptr);
     if (!e.ok()) return e;
     k_silu_back<<<nblk(nb, 256), 256, 0, st>>>(nb, s.Z2, s.dZ2);
     LAUNCH_CHECK();
-    e = launch_gemm_bf16(st, 64, 64, B, s.dZ2, B, 0, s.H1, B, 1, grad + kPW2, 64, ks);
+    e = launch_gemm_bf16(st, 64, 64, B, s.dZ2, B, 0, s.H1, B, 1, grad + kPW2, 64, ks, s.kpart);
     if (!e.ok()) return e;
     k_row_dot<<<64, 256, 0, st>>>(B, s.dZ2, nullptr, grad + kPB2);
     LAUNCH_CHECK();
-    e = launch_gemm_bf16(st, 64, B, 64, P + kPW2, 64, 1, s.dZ2, B, 0, s.dZ1, B, 1);
+    e = launch_gemm_bf16(st, 64, B, 64, P + kPW2, 64, 1, s.dZ2, B, 0, s.dZ1, B, 1, nullptr);
     if (!e.ok()) return e;
     k_silu_back<<<nblk(nb, 256), 256, 0, st>>>(nb, s.Z1, s.dZ1);
     LAUNCH_CHECK();
-    e = launch_gemm_bf16(st, 64, 16, B, s.dZ1, B, 0, s.XT, B, 1, grad + kPW1, 16, ks);
+    e = launch_gemm_bf16(st, 64, 16, B, s.dZ1, B, 0, s.XT, B, 1, grad + kPW1, 16, ks, s.kpart);
     if (!e.ok()) return e;
     k_row_dot<<<64, 256, 0, st>>>(B, s.dZ1, nullptr, grad + kPB1);
     LAUNCH_CHECK();

@@ -61,12 +61,29 @@ def test_gpu_training_fits_a_smooth_target():
     assert np.allclose(pred, MO.predict(P, norm, X[:4096], bf=True), rtol=5e-4, atol=5e-4 * gam.max())
 
 
+def test_gpu_training_is_reproducible():
+    """The weight-gradient GEMMs split K into slices whose partial products are added in a fixed
+    order, so two trainings from the same seed give the same parameters to the bit."""
+    rng = np.random.default_rng(8)
+    B = 16384
+    X = np.zeros((B, 16), np.float32)
+    X[:, :12] = rng.normal(size=(B, 12))
+    gam = np.log1p(np.exp(X[:, 1] - X[:, 5])).astype(np.float32)
+    out = []
+    for _ in range(2):
+        ctx = capi.Context()
+        ctx.mlp_train(X, gam, epochs=3, batch=8192, lr=2e-3, seed=11)
+        out.append(ctx.mlp_get()[0])
+        ctx.close()
+    assert np.array_equal(out[0].view(np.uint32), out[1].view(np.uint32))
+
+
 def test_learned_warm_start_reduces_iterations():
     """GPU training on the solver output of one monolayer run must lower the loss; the shipped network
-    (trained once on 6M contact pairs from early and relaxed states of slabs and 3-D boxes) is then evaluated on another run (same rod states fed to cold,
-    previous-gamma and network warm starts) and must beat the cold start.  Training with split-K
-    atomics is not bit-reproducible, so the gain is asserted on the fixed shipped weights; the network
-    trained here is evaluated and its ratio printed."""
+    (trained once on 6M contact pairs from early and relaxed states of slabs and 3-D boxes) is then
+    evaluated on another run (same rod states fed to cold, previous-gamma and network warm starts).  The
+    runs are deterministic, so the shipped network's iteration ratio is asserted at its measured value;
+    the network trained here is evaluated and its ratio printed."""
     from alens_amd import warmstart as W
 
     X, g = W.collect("monolayer_slab", 20000, 14, seed=21, skip=2)
@@ -78,4 +95,4 @@ def test_learned_warm_start_reduces_iterations():
         cold, prev, mlp = ev["cold"].mean(), ev["prev"].mean(), ev["mlp"].mean()
         print(f"{name}: BBPGD iterations/step: cold {cold:.1f}, previous gamma {prev:.1f}, network {mlp:.1f} "
               f"(cold/network = {cold / mlp:.2f}), fallbacks {ev['fallback_mlp']}")
-    assert mlp < cold
+    assert cold / mlp >= 1.02  # shipped network (the last evaluated): measured 101.7 / 99.2 = 1.025

@@ -23,7 +23,7 @@ pytestmark = pytest.mark.gpu
 # plain FP64 iterations / form iterations, summed over the measured states; the runs are
 # deterministic (bit-exact solver), so these are the measured ratios (the form does not reach the 2x
 # target; at the 1M box's early states it takes more iterations than the plain solve)
-MIN_RATIO = {"monolayer_slab": 1.0, "lognormal_3d": 0.89}
+MIN_RATIO = {"monolayer_slab": 1.04, "lognormal_3d": 0.89}  # measured 1.046 and 0.892
 
 
 @pytest.mark.parametrize("kind,relax", [("monolayer_slab", 20), ("lognormal_3d", 8)])
