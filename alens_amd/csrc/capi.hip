@@ -2224,10 +2224,21 @@ alens_status alens_mlp_train(alens_ctx *ctx, int64_t m, const float *X, const fl
 
 }  // extern "C"
 
-// Streaming copy used by alens_copy_probe: each thread moves 16-byte words, grid-stride.
-__global__ void k_copy_probe(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t nvec) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvec; i += (size_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+// Streaming copy used by alens_copy_probe: 16-byte words, grid-stride, four loads in flight per
+// thread before their stores.  Launched as one 1024-thread block per CU (the fastest of the shapes
+// measured on MI355X for 0.25-4 GiB: 4.9-5.3 TB/s).
+__global__ __launch_bounds__(1024) void k_copy_probe(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                     size_t nvec) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvec; i += 4 * stride) {
+        uint4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i + u * stride < nvec) v[u] = src[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i + u * stride < nvec) dst[i + u * stride] = v[u];
+    }
 }
 
 __global__ void k_fill_probe(uint4 *__restrict__ dst, size_t nvec) {
@@ -2247,7 +2258,9 @@ alens_status alens_copy_probe(int32_t device, int64_t nbytes, int32_t reps, doub
     hipEvent_t e0 = nullptr, e1 = nullptr;
     alens_status rc = ALENS_E_HIP;
     float ms = 0.f;
-    const unsigned grid = 8192, blk = 256;
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu < 1) ncu = 256;
+    const unsigned grid = (unsigned)ncu, blk = 1024;
     if (hipMalloc((void **)&a, nvec * 16) != hipSuccess || hipMalloc((void **)&b, nvec * 16) != hipSuccess) {
         rc = ALENS_E_OOM;
         goto done;
