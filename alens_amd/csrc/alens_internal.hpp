@@ -270,6 +270,8 @@ struct SolveArgsT {
     int32_t bar_mode;            // grid barrier variant (k_solve.hip)
     int32_t xcd_map;             // 1: blocks of one XCD take consecutive work (k_solve.hip)
     int32_t split_ok;            // 1: heavy rods may take helper lanes in phase A (k_solve.hip)
+    int32_t group_width;         // phase-B reduction groups: 0 = 256 lanes unless 128 saves a round of
+                                 // chunks, 256 or 128 = always that width (k_solve.hip)
 };
 using SolveArgs = SolveArgsT<double>;
 // entries: adjacency entries in flight per phase-A step (2, or 0 for the largest instantiated)

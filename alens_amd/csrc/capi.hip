@@ -85,7 +85,7 @@ struct alens_ctx {
     bool have_xprev = false;
     DBuf<uint32_t> bar;
     // solver schedule (the autotuner's choices; environment variables override them unless tuning)
-    int32_t bar_mode = 3, xcd_map = 0, max_blocks = 0, entries = 0;
+    int32_t bar_mode = 3, xcd_map = 0, max_blocks = 0, entries = 0, group_width = 0;
     bool tuning = false;
     // device-resident step (graph mode): capacities, device counts, the captured step
     bool capture = false;                  // enqueueing into a graph: no host waits
@@ -754,6 +754,8 @@ static alens_status run_bbpgd_on(alens_ctx *ctx, SolveArgsT<T> a, const SolveMem
     if (a.bar_mode < 0 || a.bar_mode > 3) a.bar_mode = 0;
     a.xcd_map = env("ALENS_XCD_MAP", ctx->xcd_map) != 0;
     a.split_ok = env("ALENS_SOLVER_SPLIT", 1) != 0;
+    a.group_width = env("ALENS_GROUP_WIDTH", ctx->group_width);
+    if (a.group_width != 128 && a.group_width != 256) a.group_width = 0;
     const int entries = env("ALENS_SOLVER_ENTRIES", ctx->entries);
     // inside a graph the timing points are clock readings: stamp 2 before the first solver launch,
     // stamp 3 after each one (the last reading is the one read after the replay)
@@ -1678,6 +1680,7 @@ alens_status alens_autotune(alens_ctx *ctx, alens_tune_report *rep_out) {
     ctx->xcd_map = 0;
     ctx->max_blocks = 0;
     ctx->entries = 0;
+    ctx->group_width = 0;
     float best = 0.f;
     alens_status s = run(&best);
     tr.default_ms = best;
@@ -1692,6 +1695,7 @@ alens_status alens_autotune(alens_ctx *ctx, alens_tune_report *rep_out) {
         {&ctx->bar_mode, {3, 1, 0, 2}},
         {&ctx->xcd_map, {0, 1}},
         {&ctx->max_blocks, {0, (int32_t)std::max<int64_t>(8, grid * 3 / 4), (int32_t)std::max<int64_t>(8, grid / 2)}},
+        {&ctx->group_width, {0, 256, 128}},
     };
     for (auto &ax : axes) {
         if (s != ALENS_OK) break;
@@ -1718,6 +1722,7 @@ alens_status alens_autotune(alens_ctx *ctx, alens_tune_report *rep_out) {
     tr.xcd_map = ctx->xcd_map;
     tr.max_blocks = ctx->max_blocks;
     tr.entries = ctx->entries;
+    tr.group_width = ctx->group_width;
     tr.best_ms = best;
     if (rep_out) *rep_out = tr;
     return ALENS_OK;

@@ -685,8 +685,9 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     // 256-lane groups (two elements per lane) unless 128-lane groups save a round of chunks
     const int G = kGrpB / 2;
     // (the split kernel keeps to 256-lane groups: the 128-lane path would not fit its register budget)
-    const bool wide = SPLIT || (nch + gridDim.x * G - 1) / (gridDim.x * G) ==
-                                   (nch + gridDim.x * kGrpB - 1) / (gridDim.x * kGrpB);
+    const bool wide = SPLIT || a.group_width == 256 ||
+                      (a.group_width != 128 && (nch + gridDim.x * G - 1) / (gridDim.x * G) ==
+                                                   (nch + gridDim.x * kGrpB - 1) / (gridDim.x * kGrpB));
     const int gb = wide ? G : kGrpB;
     if (threadIdx.x < kChunkRounds * gb) {
         const int r = threadIdx.x / gb, g = threadIdx.x % gb;

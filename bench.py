@@ -563,14 +563,58 @@ def bench_replicas(args, capi, systems, copy_gbs=None):
         from alens_amd import warmstart as W
 
         P, norm = W.load()
-        b = _run_replicas(args, capi, parts, cfg, 1, 2, P, norm)
+        b = _run_replicas(args, capi, parts, cfg, 1, 2, P, norm, keep=True)
         out["chosen_form"] = {
             "form": "FP32 BBPGD + FP64 residual/refinement, warm start from the learned network",
             "value": n * K / b["elapsed"], "unit": "rod-steps/s", "ms_per_step": 1e3 * b["elapsed"] / K,
             "replica_iters_mean": b["iters_mean_per_replica"], "replica_iters_histogram": b["hist"],
             "fallbacks": b["fallbacks"], "solver_schedule": b["tune"], "graph": b["graph"],
+            "accuracy": replica_accuracy(capi, parts, cfg, b),
         }
+        b["ctx"].close()
     print(json.dumps(out), flush=True)
+
+
+def replica_accuracy(capi, parts, cfg, run):
+    """Error of the FP32 form per replica, stated per result: from the form run's final state, the FP32
+    form (cold, to tol) and the plain FP64 solve (to tol) against an FP64 solve to 1e-11, each as the
+    largest |gamma - gamma_ref| of the replica over the replica's largest |gamma_ref| (and the same for
+    U).  All replicas are solved together in one context per solve, as in the benchmark."""
+    ctx, offs, perm = run["ctx"], run["offs"], run["perm"]
+    c, t = ctx.get_rods()
+    st = ctx.step_count
+    cat = {k: np.concatenate([np.asarray(r[k]).reshape(len(r["length"]), -1) for r, _ in parts])[perm]
+           for k in ("gid", "length", "radius")}
+    sparts = []
+    for s_, (_, box) in enumerate(parts):
+        lo, hi = int(offs[s_]), int(offs[s_ + 1])
+        sparts.append((dict(gid=cat["gid"][lo:hi, 0], length=cat["length"][lo:hi, 0], radius=cat["radius"][lo:hi, 0],
+                            center=c[lo:hi], dir=t[lo:hi], immovable=None), box))
+    sols = {}
+    for name, kw in (("fp32_form", dict(precision=1)), ("plain_fp64", {}),
+                     ("tight_fp64", dict(tol=1e-11, max_ite=2_000_000))):
+        x = capi.Context(**dict(cfg, **kw))
+        x.set_systems(sparts)
+        x.set_step(st)
+        x.step()
+        g = x.constraint_arrays()
+        sols[name] = dict(I=g["I"], gamma=x.gamma(), u=x.velocity()["u"])
+        x.close()
+    ref = sols["tight_fp64"]
+    res = {}
+    for name in ("fp32_form", "plain_fp64"):
+        eg, eu = [], []
+        for s_ in range(len(parts)):
+            lo, hi = int(offs[s_]), int(offs[s_ + 1])
+            sel = (ref["I"] >= lo) & (ref["I"] < hi)
+            gr, ur = ref["gamma"][sel], ref["u"][6 * lo:6 * hi]
+            eg.append(float(np.max(np.abs(sols[name]["gamma"][sel] - gr)) / max(np.max(np.abs(gr)), 1e-300))
+                      if gr.size else 0.0)
+            eu.append(float(np.max(np.abs(sols[name]["u"][6 * lo:6 * hi] - ur)) / max(np.max(np.abs(ur)), 1e-300)))
+        res[name] = {"gamma_max_rel_err": {"median": float(np.median(eg)), "max": float(np.max(eg))},
+                     "u_max_rel_err": {"median": float(np.median(eu)), "max": float(np.max(eu))}}
+    res["reference"] = "per replica: FP64 BBPGD to residual 1e-11 on the same state"
+    return res
 
 
 if __name__ == "__main__":
