@@ -26,7 +26,7 @@ EXPORTS = [
     "alens_get_velocity", "alens_bcqp_solve", "alens_add_constraints", "alens_clear_constraints",
     "alens_get_con_stress", "alens_autotune", "alens_set_graph", "alens_get_graph_info",
     "alens_mlp_set", "alens_mlp_get", "alens_get_features", "alens_mlp_predict", "alens_mlp_gradient",
-    "alens_mlp_train", "alens_copy_probe", "alens_bcqp_solve_static",
+    "alens_mlp_train", "alens_copy_probe", "alens_bcqp_solve_static", "alens_mlp_energy", "alens_mlp_train_energy",
 ]
 MLP_PARAMS = 9473
 MLP_NORM = 25
@@ -159,6 +159,8 @@ def lib():
     L.alens_mlp_train.argtypes = [vp, ctypes.c_int64, _fp, _fp, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
                                   ctypes.c_uint64, _fp]
     L.alens_copy_probe.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, _dp]
+    L.alens_mlp_energy.argtypes = [vp, _dp, _fp]
+    L.alens_mlp_train_energy.argtypes = [vp, ctypes.c_int32, ctypes.c_float, _dp]
     L.alens_bcqp_solve_static.argtypes = L.alens_bcqp_solve.argtypes
     for name in EXPORTS:
         fn = getattr(L, name)
@@ -466,6 +468,21 @@ class Context:
         self._check(lib().alens_mlp_gradient(self._h, len(X), _p(X, ctypes.c_float), _p(t, ctypes.c_float),
                                              _p(g, ctypes.c_float), ctypes.byref(loss)))
         return g, loss.value
+
+    def mlp_energy(self, grad=False):
+        """BCQP energy of the network's start point on the last step's rows (alens_mlp_energy), and with
+        grad=True its gradient with respect to the parameters."""
+        e = ctypes.c_double()
+        g = np.zeros(MLP_PARAMS, np.float32) if grad else None
+        self._check(lib().alens_mlp_energy(self._h, ctypes.byref(e), _p(g, ctypes.c_float)))
+        return (e.value, g) if grad else e.value
+
+    def mlp_train_energy(self, iters=10, lr=1e-3):
+        """Adam on the start point's BCQP energy for the last step's rows (alens_mlp_train_energy);
+        returns the energy before each step and after the last."""
+        e = np.zeros(iters + 1)
+        self._check(lib().alens_mlp_train_energy(self._h, int(iters), float(lr), _p(e, ctypes.c_double)))
+        return e
 
     def mlp_train(self, X, gamma, epochs=10, batch=4096, lr=2e-3, seed=1):
         X = np.ascontiguousarray(X, np.float32).reshape(-1, 16)

@@ -81,6 +81,8 @@ struct alens_ctx {
     // learned warm start: FP32 master parameters, normalisation, features of the step
     DBuf<float> mlpP, mlpNorm, feat;
     bool mlp_ready = false;
+    DBuf<float> adam1, adam2;                // Adam moments of energy training (reset by alens_mlp_set)
+    int adam_step = 0;
     DBuf<double2> xprev;  // previous-step gamma matched to this step's rows (x component)
     bool have_xprev = false;
     DBuf<uint32_t> bar;
@@ -2022,6 +2024,7 @@ alens_status alens_mlp_set(alens_ctx *ctx, const float *params, const float *nor
     TRY_HIP(hipMemcpyAsync(ctx->mlpNorm.p, norm, sizeof(float) * kMlpNorm, hipMemcpyHostToDevice, ctx->st));
     TRY_HIP(hipStreamSynchronize(ctx->st));
     ctx->mlp_ready = true;
+    ctx->adam_step = 0;
     return ALENS_OK;
 }
 
@@ -2224,6 +2227,114 @@ alens_status alens_mlp_train(alens_ctx *ctx, int64_t m, const float *X, const fl
     tb.release();
     for (auto *b : {&dX, &dT, &dXB, &dTB, &dG, &dM1, &dM2, &dL}) b->release();
     dIdx.release();
+    return ALENS_OK;
+}
+
+}  // extern "C"
+
+// The learned start point's BCQP energy on the last step's collision rows, and optionally its gradient
+// with respect to the network parameters (into grad_dev, on the device).
+//   gamma0 = network(features of the rows); g = A gamma0 + b from one start-point pass of the solver
+//   (max_ite 0, no warm-start test); E = 1/2 gamma0' (g + b) = 1/2 gamma0' A gamma0 + b' gamma0.
+//   dE/dgamma0 = g.  With gamma0 = s y (s the network's output scale) the mean-square loss of
+//   mlp_gradient, mean((y - t)^2) over B rows, has gradient (2 / B)(y - t) dy; the targets
+//   t = y - (B / 2) s g make that s g dy = dE.  Rows are padded to a multiple of 16 with copies of row 0
+//   whose target is their own output (zero gradient).
+// The solver's iterate buffer is overwritten; the step's outputs (gamma, velocities, rows) are not.
+namespace {
+struct EnergyBufs {
+    TrainBufs tb;
+    DBuf<float> X16, tgt, grad;
+};
+}  // namespace
+
+static alens_status mlp_energy_eval(alens_ctx *ctx, double *energy, EnergyBufs *eb) {
+    if (!ctx->have_step) return fail(ctx, ALENS_E_STATE, "mlp energy: no step has been run");
+    if (!ctx->mlp_ready) return fail(ctx, ALENS_E_STATE, "mlp energy: no network parameters");
+    if (ctx->mext > 0) return fail(ctx, ALENS_E_STATE, "mlp energy: the step had caller constraint blocks");
+    const uint32_t M = ctx->mcol;
+    if (M == 0) return fail(ctx, ALENS_E_STATE, "mlp energy: the last step had no constraints");
+    hipStream_t st = ctx->st;
+    TRY_HIP(ctx->feat.ensure(16 * (size_t)M));
+    TRY_K(launch_features(st, M, ctx->gstride, ctx->cI.p, ctx->cJ.p, ctx->delta0.p, ctx->geo.p, ctx->posI.p,
+                          ctx->posJ.p, ctx->dir.p, ctx->length.p, ctx->unc.p, ctx->have_xprev ? ctx->xprev.p : nullptr,
+                          ctx->cfg.dt, ctx->feat.p));
+    TRY_K(launch_mlp_infer_xg(st, M, ctx->feat.p, ctx->mlpP.p, ctx->mlpNorm.p, ctx->xg.p));
+    const int32_t keep_ite = ctx->cfg.max_ite;
+    ctx->cfg.max_ite = 0;
+    alens_solve_report rep{};
+    float ms = 0.f;
+    alens_status s = run_solver(ctx, ctx->n, M, ctx->mob.p, false, &rep, &ms, ctx->rsys.p, ctx->smono.p, nullptr);
+    ctx->cfg.max_ite = keep_ite;
+    if (s != ALENS_OK && s != ALENS_E_NOCONV) return s;
+    std::vector<double2> xg(M);
+    std::vector<double> b(M);
+    TRY_HIP(hipMemcpyAsync(xg.data(), ctx->xg.p, sizeof(double2) * M, hipMemcpyDeviceToHost, st));
+    TRY_HIP(hipMemcpyAsync(b.data(), ctx->b.p, sizeof(double) * M, hipMemcpyDeviceToHost, st));
+    float scale = 1.f;
+    TRY_HIP(hipMemcpyAsync(&scale, ctx->mlpNorm.p + 2 * kMlpIn, sizeof(float), hipMemcpyDeviceToHost, st));
+    TRY_HIP(hipStreamSynchronize(st));
+    double e = 0.0;
+    for (uint32_t k = 0; k < M; ++k) e += xg[k].x * (xg[k].y + b[k]);
+    *energy = 0.5 * e;
+    if (!std::isfinite(*energy)) return fail(ctx, ALENS_E_NONFINITE, "mlp energy: non-finite energy");
+    if (!eb) return ALENS_OK;
+    const int B = (int)((M + 15) / 16 * 16);
+    TRY_HIP(eb->tb.ensure(B));
+    TRY_HIP(eb->X16.ensure(16 * (size_t)B));
+    TRY_HIP(eb->tgt.ensure((size_t)B));
+    TRY_HIP(eb->grad.ensure(kMlpParams));
+    std::vector<float> t((size_t)B);
+    const double half_b = 0.5 * (double)B;
+    for (uint32_t k = 0; k < M; ++k) {
+        const double y = xg[k].x / (double)scale;
+        t[k] = (float)(y - half_b * (double)scale * xg[k].y);
+    }
+    for (int k = (int)M; k < B; ++k) t[k] = (float)(xg[0].x / (double)scale);
+    TRY_HIP(hipMemcpyAsync(eb->X16.p, ctx->feat.p, sizeof(float) * 16 * M, hipMemcpyDeviceToDevice, st));
+    for (int k = (int)M; k < B; ++k)
+        TRY_HIP(hipMemcpyAsync(eb->X16.p + 16 * (size_t)k, ctx->feat.p, sizeof(float) * 16, hipMemcpyDeviceToDevice, st));
+    TRY_HIP(hipMemcpyAsync(eb->tgt.p, t.data(), sizeof(float) * B, hipMemcpyHostToDevice, st));
+    MlpScratch sv = eb->tb.view();
+    TRY_K(mlp_gradient(st, B, eb->X16.p, eb->tgt.p, ctx->mlpP.p, ctx->mlpNorm.p, sv, eb->grad.p));
+    return ALENS_OK;
+}
+
+extern "C" {
+
+alens_status alens_mlp_energy(alens_ctx *ctx, double *energy, float *grad) {
+    if (!ctx || !energy) return ALENS_E_ARG;
+    TRY_HIP(hipSetDevice(ctx->dev));
+    EnergyBufs eb;
+    alens_status s = mlp_energy_eval(ctx, energy, grad ? &eb : nullptr);
+    if (s != ALENS_OK) return s;
+    if (grad) {
+        TRY_HIP(hipMemcpyAsync(grad, eb.grad.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, ctx->st));
+        TRY_HIP(hipStreamSynchronize(ctx->st));
+    }
+    return ALENS_OK;
+}
+
+alens_status alens_mlp_train_energy(alens_ctx *ctx, int32_t iters, float lr, double *energy) {
+    if (ctx) drop_graph(ctx);  // the network is part of a captured step
+    if (!ctx || iters < 0 || !(lr > 0.f)) return ALENS_E_ARG;
+    TRY_HIP(hipSetDevice(ctx->dev));
+    EnergyBufs eb;
+    TRY_HIP(ctx->adam1.ensure(kMlpParams));
+    TRY_HIP(ctx->adam2.ensure(kMlpParams));
+    if (ctx->adam_step == 0) {
+        TRY_K(launch_fill_u32(ctx->st, reinterpret_cast<uint32_t *>(ctx->adam1.p), 0u, kMlpParams));
+        TRY_K(launch_fill_u32(ctx->st, reinterpret_cast<uint32_t *>(ctx->adam2.p), 0u, kMlpParams));
+    }
+    for (int32_t it = 0; it <= iters; ++it) {
+        double e = 0.0;
+        alens_status s = mlp_energy_eval(ctx, &e, it < iters ? &eb : nullptr);
+        if (s != ALENS_OK) return s;
+        if (energy) energy[it] = e;
+        if (it < iters)
+            TRY_K(mlp_adam_step(ctx->st, ctx->mlpP.p, eb.grad.p, ctx->adam1.p, ctx->adam2.p, lr, ++ctx->adam_step));
+    }
+    TRY_HIP(hipStreamSynchronize(ctx->st));
     return ALENS_OK;
 }
 

@@ -88,6 +88,37 @@ def evaluate(kind, n, steps, P, norm, seed=99, warmup=2, precision=0, forms=None
     return {k: (np.array(v) if isinstance(v, list) else v) for k, v in out.items()}
 
 
+def train_energy(P, norm, kind, n, steps, iters=8, lr=3e-4, seed=1234, warmup=10):
+    """Fine-tune the network on the BCQP energy of its own start point (alens_mlp_train_energy): a
+    plain FP64 driver moves the rods from the seeded start; from step `warmup` on, a context running
+    the network takes each state's step (so its features carry its own previous impulses) and then
+    `iters` Adam steps on that step's rows.  Returns the parameters and the energies per state."""
+    rods, cfg = systems.scaled(kind, n, seed=seed) if kind != "small_3d" else systems.small_3d(seed=seed)
+    driver = capi.Context(**cfg)
+    driver.set_rods(rods)
+    perm = driver.sort_rods()
+    rods = {k: (v[perm] if isinstance(v, np.ndarray) else v) for k, v in rods.items()}
+    ctx = capi.Context(**dict(cfg, warm_start=2))
+    ctx.mlp_set(P, norm)
+    ctx.set_rods(rods)
+    hist = []
+    for s in range(warmup + steps):
+        if s >= warmup - 1:
+            c, t = driver.get_rods()
+            ctx.set_rods(dict(rods, center=c, dir=t))
+            ctx.set_step(s)
+            ctx.step()
+            if s >= warmup:
+                hist.append(ctx.mlp_train_energy(iters, lr))
+        driver.set_step(s)
+        driver.step()
+        driver.euler()
+    P2, norm2 = ctx.mlp_get()
+    for c in (ctx, driver):
+        c.close()
+    return P2, norm2, hist
+
+
 def save(path, P, norm, meta):
     os.makedirs(os.path.dirname(path), exist_ok=True)
     np.savez(path, params=np.asarray(P, np.float32), norm=np.asarray(norm, np.float32),

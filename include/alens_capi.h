@@ -257,6 +257,16 @@ alens_status alens_mlp_gradient(alens_ctx *ctx, int64_t m, const float *X, const
  * the weights when no network is loaded.  batch is a multiple of 16; loss_per_epoch may be NULL. */
 alens_status alens_mlp_train(alens_ctx *ctx, int64_t m, const float *X, const float *gamma, int32_t epochs,
                              int32_t batch, float lr, uint64_t seed, float *loss_per_epoch);
+/* Training on the quantity that sets the solver's work: the BCQP energy of the network's start point
+ * on the last step's collision rows (the step must have no caller blocks), E = 1/2 g0' A g0 + b' g0 with
+ * g0 = network(features), A = dt D^T M D and b of the step.  dE/dg0 = A g0 + b is one start-point pass
+ * of the solver; it is backpropagated through the network's BF16-MFMA training kernels.
+ * alens_mlp_energy: E and (grad non-NULL, ALENS_MLP_PARAMS floats) dE/dparameters.
+ * alens_mlp_train_energy: iters Adam steps of rate lr on E (moments kept in the context across calls,
+ * reset by alens_mlp_set); energy (iters + 1 values or NULL): E before each step and after the last.
+ * Both overwrite the solver's iterate buffer; the step's outputs (gamma, velocities, rows) stay. */
+alens_status alens_mlp_energy(alens_ctx *ctx, double *energy, float *grad);
+alens_status alens_mlp_train_energy(alens_ctx *ctx, int32_t iters, float lr, double *energy);
 
 /* Device copy probe (no counterpart in the reference; a measurement aid for benchmark scripts): on
  * HIP device `device`, one kernel (k_copy_probe) copies nbytes between two fresh buffers, reps times;

@@ -96,3 +96,56 @@ def test_learned_warm_start_reduces_iterations():
         print(f"{name}: BBPGD iterations/step: cold {cold:.1f}, previous gamma {prev:.1f}, network {mlp:.1f} "
               f"(cold/network = {cold / mlp:.2f}), fallbacks {ev['fallback_mlp']}")
     assert cold / mlp >= 1.02  # shipped network (the last evaluated): measured 101.7 / 99.2 = 1.025
+
+
+def test_energy_and_gradient_match_oracle_and_training_descends():
+    """alens_mlp_energy on a small system's rows: the start point's BCQP energy 1/2 g0'A g0 + b'g0
+    against the network oracle (BF16 operand rounding emulated) and the oracle's D, M, D^T; its parameter
+    gradient against the oracle's backpropagation of dE/dg0 = A g0 + b; a short step against the
+    gradient lowers the energy, and alens_mlp_train_energy lowers it over its iterations."""
+    from alens_amd import warmstart as W
+    from oracle import oracle as O
+
+    P, norm = W.load()
+    rods, cfg = systems.small_3d(n=3000, box=4.0 * 3.0 ** (1 / 3), seed=12)
+    ctx = capi.Context(**dict(cfg, warm_start=2))
+    ctx.mlp_set(P, norm)
+    ctx.set_rods(rods)
+    ctx.step()
+    ctx.euler()
+    ctx.step()  # the second step's features carry the previous impulses
+    X = ctx.features()
+    g = ctx.constraint_arrays()
+    vel = ctx.velocity()
+    m = g["I"].size
+    assert m > 500
+    E, grad = ctx.mlp_energy(grad=True)
+
+    s = float(norm[24])
+    g0 = MO.predict(P, norm, X, bf=True)
+    oc = O.cfg_from_dict(cfg)
+    n = rods["length"].size
+    rw = dict(nrm=g["nrm"].reshape(-1), rI=g["rI"].reshape(-1), rJ=g["rJ"].reshape(-1))
+    _, U = O.apply_D_M(n, g["I"], g["J"], rw, vel["mob"], oc, g0)
+    gr = oc.dt * O.apply_DT(g["I"], g["J"], rw, U) + g["b"]
+    E_ref = 0.5 * float(np.dot(g0, gr + g["b"]))
+    assert abs(E - E_ref) <= 2e-3 * abs(E_ref), (E, E_ref)
+    B = (m + 15) // 16 * 16
+    Xp = np.concatenate([X, np.repeat(X[:1], B - m, axis=0)])
+    y = g0 / s
+    t = np.concatenate([y - 0.5 * B * s * gr, np.repeat(y[:1], B - m)])
+    _, rg = MO.loss_and_grad(P, norm, Xp, t, bf=True)
+    blocks = [(MO.PW1, MO.PB1), (MO.PB1, MO.PW2), (MO.PW2, MO.PB2), (MO.PB2, MO.PW3), (MO.PW3, MO.PB3),
+              (MO.PB3, MO.PW4), (MO.PW4, MO.PB4), (MO.PB4, MO.NPARAM)]
+    for a, b in blocks:
+        err = np.linalg.norm(grad[a:b] - rg[a:b]) / max(np.linalg.norm(rg[a:b]), 1e-30)
+        assert err <= 2e-2, (a, b, err)
+    # a step against the gradient lowers the energy
+    step = 1e-3 * np.linalg.norm(P) / np.linalg.norm(grad)
+    ctx.mlp_set((P - step * grad).astype(np.float32), norm)
+    assert ctx.mlp_energy() < E
+    ctx.mlp_set(P, norm)
+    es = ctx.mlp_train_energy(iters=12, lr=3e-4)
+    print(f"energy over 12 Adam steps: {es[0]:.6e} -> {es[-1]:.6e}", flush=True)
+    assert abs(es[0] - E) <= 1e-9 * abs(E) and es[-1] < es[0]
+    ctx.close()
