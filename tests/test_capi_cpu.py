@@ -86,3 +86,12 @@ def test_cpp_rod_file_reader():
             out[f] = dict(zip(tok[0::2], (int(v) for v in tok[1::2])))
     assert out["dense_monolayer_TubuleInitial.dat"] == dict(rods=9700, stationary=0, group0=-1)
     assert out["mix_motor_sliding_TubuleInitial.dat"] == dict(rods=2, stationary=1, group0=-1)
+
+
+def test_copy_probe_rejects_bad_arguments():
+    """Argument checks of alens_copy_probe come before any device work."""
+    g = ctypes.c_double(-1.0)
+    L = capi.lib()
+    assert L.alens_copy_probe(0, 8, 1, ctypes.byref(g)) == capi.ALENS_E_ARG
+    assert L.alens_copy_probe(0, 1 << 20, 0, ctypes.byref(g)) == capi.ALENS_E_ARG
+    assert L.alens_copy_probe(0, 1 << 20, 1, None) == capi.ALENS_E_ARG
