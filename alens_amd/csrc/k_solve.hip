@@ -154,38 +154,25 @@ __device__ inline bool grid_barrier(uint32_t *bar, uint32_t &epoch, int *s_flag,
     return *s_flag == 0;
 }
 
-// Fixed-order sum within one 256-lane reduction group: each lane's partial is already accumulated
-// left to right; each wave folds by halving (lane l += lane l + h, h = 32 .. 1); the group's four
-// wave totals are added left to right.  Called by every thread of the block (all groups together).
-template <int NQ>
-__device__ inline void group_sums(double (&v)[NQ], double *sh /* kGrp * NQ * 4 */, double (&tot)[NQ]) {
+// Fixed-order sums (NS of them) and maxima (NX) within each 256-lane reduction group: each lane's
+// partial is already accumulated left to right; each wave folds by halving (lane l += lane l + h,
+// h = 32 .. 1); the group's four wave results are added (or compared) left to right, with one LDS
+// exchange for all of them.  Called by every thread of the block (all groups together).
+template <int NS, int NX>
+__device__ inline void group_fold(const double (&vs)[NS], const double (&vx)[NX], double *sh /* kGrp * (NS + NX) * 4 */,
+                                  double (&ts)[NS], double (&tx)[NX]) {
+    constexpr int NQ = NS + NX;
     const int lane = threadIdx.x & 63, w = (threadIdx.x >> 6) & 3, g = threadIdx.x >> 8;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        double x = v[q];
-#pragma unroll
-        for (int h = 32; h >= 1; h >>= 1) x = x + __shfl_down(x, h, 64);
-        if (lane == 0) sh[(g * NQ + q) * 4 + w] = x;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        const double *s4 = sh + (g * NQ + q) * 4;
-        tot[q] = ((s4[0] + s4[1]) + s4[2]) + s4[3];
-    }
-    __syncthreads();
-}
-
-template <int NQ>
-__device__ inline void group_maxes(double (&v)[NQ], double *sh, double (&tot)[NQ]) {
-    const int lane = threadIdx.x & 63, w = (threadIdx.x >> 6) & 3, g = threadIdx.x >> 8;
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        double x = v[q];
+        double x = q < NS ? vs[q] : vx[q - NS];
 #pragma unroll
         for (int h = 32; h >= 1; h >>= 1) {
             const double o = __shfl_down(x, h, 64);
-            x = o > x ? o : x;
+            if (q >= NS)
+                x = o > x ? o : x;
+            else
+                x = x + o;
         }
         if (lane == 0) sh[(g * NQ + q) * 4 + w] = x;
     }
@@ -193,11 +180,15 @@ __device__ inline void group_maxes(double (&v)[NQ], double *sh, double (&tot)[NQ
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         const double *s4 = sh + (g * NQ + q) * 4;
-        double t = s4[0];
-        t = s4[1] > t ? s4[1] : t;
-        t = s4[2] > t ? s4[2] : t;
-        t = s4[3] > t ? s4[3] : t;
-        tot[q] = t;
+        if (q >= NS) {
+            double t = s4[0];
+            t = s4[1] > t ? s4[1] : t;
+            t = s4[2] > t ? s4[2] : t;
+            t = s4[3] > t ? s4[3] : t;
+            tx[q - NS] = t;
+        } else {
+            ts[q] = ((s4[0] + s4[1]) + s4[2]) + s4[3];
+        }
     }
     __syncthreads();
 }
@@ -245,8 +236,7 @@ __device__ inline void fold_systems(const double *part, int nchunks, const uint3
             }
         }
         double ts[NS], tx[NX];
-        group_sums<NS>(vs, sh, ts);
-        group_maxes<NX>(vx, sh, tx);
+        group_fold<NS, NX>(vs, vx, sh, ts, tx);
         if (lane == 0 && sy < nsys) {
 #pragma unroll
             for (int q = 0; q < NS; ++q) tot[q][sy] = ts[q];
@@ -394,6 +384,20 @@ struct SplitMeta {
 // solve: a pass of a stopped system costs one LDS read instead of five dependent global loads
 constexpr int kPasses = 6;
 
+// Slice of a window that wave w walks in pass q.  Slices are in descending degree within each
+// window, so with the identity every pass gives wave 0 the heaviest slice and the block waits on that
+// one wave.  Passes q = 0, 1, 2, 3 (mod 4) give wave w the slices w, W-1-w, (w + W/2) % W and
+// W-1-(w + W/2) % W, so the heaviest and lightest slices of successive windows go to the same
+// waves.  Every position of pass q lies below every position of pass q + 1, so a lane whose pass-q
+// position is past the rods has no later pass either.
+constexpr int kWaves = kBlk / 64;
+__device__ inline int pass_pos(int base, int q, int stride, int t) {
+    const int w = t >> 6;
+    const int v = (q & 2) ? (w + kWaves / 2) % kWaves : w;
+    const int s = (q & 1) ? kWaves - 1 - v : v;
+    return base + q * stride + s * 64 + (t & 63);
+}
+
 template <int MODE, typename T, int kAG, bool SPLIT>
 __device__ inline void phase_A(const SolveArgsT<T> &a, const double *s_alpha, const int *s_state,
                                const RodMeta &rm0, bool split, const SplitMeta *s_help,
@@ -447,8 +451,10 @@ __device__ inline void phase_A(const SolveArgsT<T> &a, const double *s_alpha, co
         }
         return;
     }
-    int q = 0;
-    for (int p = i0; p < a.n; p += stride, ++q) {
+    const int base = i0 - (int)threadIdx.x;
+    for (int q = 0;; ++q) {
+        const int p = pass_pos(base, q, stride, threadIdx.x);
+        if (p >= a.n) break;
         const RodMeta rm = q == 0 ? rm0 : (q < kPasses ? s_rmeta[q - 1][threadIdx.x] : rod_meta(a, p));
         if (s_state[rm.sy] != S_ACTIVE) continue;
         T f[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
@@ -493,20 +499,23 @@ struct GroupShape {
     static constexpr int per_block = kBlk / GL;   // groups per block
 };
 
-template <int NQ, bool MAX, int GL>
-__device__ inline void vgroup_fold(const double (&v)[NQ][GroupShape<GL>::vpt], double *sh /* groups * NQ * 4 */,
-                                   double (&tot)[NQ]) {
-    constexpr int VPT = GroupShape<GL>::vpt;
+// Folds NS sums (vs) and NX maxima (vx) of every group in one pass: each physical wave folds its
+// virtual waves by halving, one LDS exchange, then each group adds (or takes the maximum of) its four
+// virtual-wave results left to right.  sh holds groups * (NS + NX) * 4 doubles.
+template <int NS, int NX, int GL>
+__device__ inline void vgroup_fold(const double (*vs)[GroupShape<GL>::vpt], const double (*vx)[GroupShape<GL>::vpt],
+                                   double *sh, double *ts, double *tx) {
+    constexpr int VPT = GroupShape<GL>::vpt, NQ = NS + NX;
     const int l = threadIdx.x % GL, g = threadIdx.x / GL, lane = l & 63, pw = l >> 6;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
 #pragma unroll
         for (int j = 0; j < VPT; ++j) {
-            double x = v[q][j];
+            double x = q < NS ? vs[q][j] : vx[q - NS][j];
 #pragma unroll
             for (int h = 32; h >= 1; h >>= 1) {
                 const double o = __shfl_down(x, h, 64);
-                if (MAX)
+                if (q >= NS)
                     x = o > x ? o : x;
                 else
                     x = x + o;
@@ -518,14 +527,14 @@ __device__ inline void vgroup_fold(const double (&v)[NQ][GroupShape<GL>::vpt], d
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         const double *s4 = sh + (g * NQ + q) * 4;
-        if (MAX) {
+        if (q >= NS) {
             double t = s4[0];
             t = s4[1] > t ? s4[1] : t;
             t = s4[2] > t ? s4[2] : t;
             t = s4[3] > t ? s4[3] : t;
-            tot[q] = t;
+            tx[q - NS] = t;
         } else {
-            tot[q] = ((s4[0] + s4[1]) + s4[2]) + s4[3];
+            ts[q] = ((s4[0] + s4[1]) + s4[2]) + s4[3];
         }
     }
     __syncthreads();
@@ -566,18 +575,20 @@ __device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, co
         const int c = base + g;
         // per virtual lane: sums (s^T s, s^T y, y^T y, non-finite count), max residual, and for a warm
         // start the cold-start residual and its non-finite count
-        double vs[4][kVPT], vx[1][kVPT], vrc[1][kVPT], vnfc[1][kVPT];
+        double vs[5][kVPT], vx[2][kVPT];
 #pragma unroll
         for (int j = 0; j < kVPT; ++j) {
-            vs[0][j] = vs[1][j] = vs[2][j] = vs[3][j] = 0.0;
-            vx[0][j] = vrc[0][j] = vnfc[0][j] = 0.0;
+            vs[0][j] = vs[1][j] = vs[2][j] = vs[3][j] = vs[4][j] = 0.0;
+            vx[0][j] = vx[1][j] = 0.0;
         }
         const ChunkMeta cm = r < kChunkRounds ? s_cm[r][g] : chunk_meta(a, c, nch);
         const int sy = cm.sy < 0 ? 0 : cm.sy;
         const bool act = cm.sy >= 0 && s_state[sy] == S_ACTIVE;
         if (act) {
             const T alpha = (T)s_alpha[sy];
-            T xo[NE], go[NE], du[NE];
+            // every load of the round (including the rhs) is issued before the first store to xg,
+            // which the compiler could not move them past
+            T xo[NE], go[NE], du[NE], bb[NE];
             uint32_t kk[NE];
             bool ok[NE];
 #pragma unroll
@@ -591,6 +602,7 @@ __device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, co
                         const typename Vec2<T>::type xg = a.xg[kk[q]];
                         xo[q] = xg.x;
                         go[q] = xg.y;
+                        if (MODE != B_ALPHA0) bb[q] = a.b[kk[q]];
                         du[q] = du_row<T>(a, kk[q]);
                     }
                 }
@@ -610,7 +622,7 @@ __device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, co
                         vs[1][j] = vs[1][j] + (double)(go[q] * ag);
                     } else {
                         const T xn = proj(xo[q] - alpha * go[q], lo);
-                        const T bk = a.b[k];
+                        const T bk = bb[q];
                         T gn = dt * du[q] + bk;
                         if (a.invk) gn = gn + a.invk[k] * xn;
                         const double res = (double)fabs(xn - proj(xn - gn, lo));
@@ -622,9 +634,9 @@ __device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, co
                             if (a.warm) {
                                 const double r0 = (double)fabs(T(0) - proj(T(0) - bk, lo));
                                 if (!isfinite(r0))
-                                    vnfc[0][j] = vnfc[0][j] + 1.0;
-                                else if (r0 > vrc[0][j])
-                                    vrc[0][j] = r0;
+                                    vs[4][j] = vs[4][j] + 1.0;
+                                else if (r0 > vx[1][j])
+                                    vx[1][j] = r0;
                             }
                         } else {
                             const T sv = xn - xo[q];
@@ -638,9 +650,11 @@ __device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, co
                 }
             }
         }
-        double ts[4], tx[1];
-        vgroup_fold<4, false, GL>(vs, sh, ts);
-        vgroup_fold<1, true, GL>(vx, sh, tx);
+        double ts[5], tx[2];
+        if (MODE == B_INIT && a.warm)
+            vgroup_fold<5, 2, GL>(vs, vx, sh, ts, tx);
+        else
+            vgroup_fold<4, 1, GL>(vs, vx, sh, ts, tx);
         double *pc = a.part + c;
         const int nc = a.nchunks;
         if (act && l == 0) {
@@ -654,14 +668,9 @@ __device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, co
                 pc[(size_t)Q_NF * nc] = ts[3];
             }
         }
-        if (MODE == B_INIT && a.warm) {
-            double t2[1], t3[1];
-            vgroup_fold<1, false, GL>(vnfc, sh, t2);
-            vgroup_fold<1, true, GL>(vrc, sh, t3);
-            if (act && l == 0) {
-                pc[(size_t)Q_RC * nc] = t3[0];
-                pc[(size_t)Q_NFC * nc] = t2[0];
-            }
+        if (MODE == B_INIT && a.warm && act && l == 0) {
+            pc[(size_t)Q_RC * nc] = tx[1];
+            pc[(size_t)Q_NFC * nc] = ts[4];
         }
     }
 }
@@ -699,7 +708,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     {
         const int stride = gridDim.x * blockDim.x;
         for (int q = 1; q < kPasses; ++q) {
-            const int p = i0 + q * stride;
+            const int p = pass_pos(i0 - (int)threadIdx.x, q, stride, threadIdx.x);
             if (p >= a.n) break;
             s_rmeta[q - 1][threadIdx.x] = rod_meta(a, p);
         }
