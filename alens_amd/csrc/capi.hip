@@ -1053,7 +1053,8 @@ static alens_status step_plain(alens_ctx *ctx, const double *f_ext, alens_solve_
         if (slot_order_on()) {
             TRY_HIP(ctx->sord.ensure((size_t)std::max(n, 1)));
             TRY_HIP(ctx->spos.ensure((size_t)std::max(n, 1)));
-            TRY_K(launch_slot_order(st, n, ctx->rowStart.p, ctx->jStart.p, xs, ctx->sord.p, ctx->spos.p));
+            TRY_K(launch_slot_order(st, n, ctx->rowStart.p, ctx->jStart.p, xs, ctx->sord.p, ctx->spos.p,
+                                    slot_order_window(n, ctx->ncu)));
             ctx->sord_cur = ctx->sord.p;
             ctx->spos_cur = ctx->spos.p;
         }
@@ -1293,7 +1294,8 @@ static alens_status enqueue_resident(alens_ctx *ctx, bool fx, bool warm_prev) {
     const int64_t nsl = ((int64_t)n + kSlice - 1) / kSlice;
     ctx->sord_cur = ctx->spos_cur = nullptr;
     if (slot_order_on()) {
-        TRY_K(launch_slot_order(st, n, ctx->rowStart.p, ctx->jStart.p, nullptr, ctx->sord.p, ctx->spos.p));
+        TRY_K(launch_slot_order(st, n, ctx->rowStart.p, ctx->jStart.p, nullptr, ctx->sord.p, ctx->spos.p,
+                                slot_order_window(n, ctx->ncu)));
         ctx->sord_cur = ctx->sord.p;
         ctx->spos_cur = ctx->spos.p;
     }

@@ -919,21 +919,21 @@ __device__ inline uint32_t rod_degree(const uint32_t *rowStart, const uint32_t *
     return deg;
 }
 
-// one block per window of kOrderWin rods: a bitonic sort in LDS of the keys (descending degree,
+// one block per window of W rods: a bitonic sort in LDS of the keys (descending degree,
 // then ascending index); rods past n sort last
-__global__ __launch_bounds__(kOrderWin) void k_slot_order(int32_t n, const uint32_t *__restrict__ rowStart,
-                                                          const uint32_t *__restrict__ jStart,
-                                                          const uint32_t *__restrict__ extStart,
-                                                          uint32_t *__restrict__ sord,
-                                                          uint32_t *__restrict__ spos) {
-    __shared__ uint64_t s_key[kOrderWin];
-    const int32_t base = blockIdx.x * kOrderWin;
+template <int W>
+__global__ __launch_bounds__(W) void k_slot_order(int32_t n, const uint32_t *__restrict__ rowStart,
+                                                  const uint32_t *__restrict__ jStart,
+                                                  const uint32_t *__restrict__ extStart, uint32_t *__restrict__ sord,
+                                                  uint32_t *__restrict__ spos) {
+    __shared__ uint64_t s_key[W];
+    const int32_t base = blockIdx.x * W;
     const int t = threadIdx.x;
     const int32_t i = base + t;
     s_key[t] = i < n ? ((uint64_t)(0xffffffffu - rod_degree(rowStart, jStart, extStart, i)) << 32) | (uint32_t)t
                      : ~0ull;
     __syncthreads();
-    for (int k = 2; k <= kOrderWin; k <<= 1) {
+    for (int k = 2; k <= W; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
             const int u = t ^ j;
             if (u > t) {
@@ -954,9 +954,13 @@ __global__ __launch_bounds__(kOrderWin) void k_slot_order(int32_t n, const uint3
 }
 
 HipErr launch_slot_order(hipStream_t st, int32_t n, const uint32_t *rowStart, const uint32_t *jStart,
-                         const uint32_t *extStart, uint32_t *sord, uint32_t *spos) {
+                         const uint32_t *extStart, uint32_t *sord, uint32_t *spos, int win) {
     if (n == 0) return hip_ok();
-    k_slot_order<<<nblk(n, kOrderWin), kOrderWin, 0, st>>>(n, rowStart, jStart, extStart, sord, spos);
+    if (win == kOrderWinPasses)
+        k_slot_order<kOrderWinPasses><<<nblk(n, kOrderWinPasses), kOrderWinPasses, 0, st>>>(n, rowStart, jStart,
+                                                                                         extStart, sord, spos);
+    else
+        k_slot_order<kOrderWin><<<nblk(n, kOrderWin), kOrderWin, 0, st>>>(n, rowStart, jStart, extStart, sord, spos);
     LAUNCH_CHECK();
     return hip_ok();
 }
@@ -1068,7 +1072,7 @@ HipErr launch_chunk_table(hipStream_t st, int32_t nsys, int32_t n, const SysDev 
 HipErr launch_slice_sizes(hipStream_t st, int32_t n, const uint32_t *rowStart, const uint32_t *jStart,
                           const uint32_t *extStart, uint32_t *slice_len, const uint32_t *sord) {
     static_assert(kSlice == 64, "one wave per slice");
-    static_assert(kOrderWin % kSlice == 0, "order windows hold whole slices");
+    static_assert(kOrderWin % kSlice == 0 && kOrderWinPasses % kSlice == 0, "order windows hold whole slices");
     if (n == 0) return hip_ok();
     k_slice_sizes<<<nblk(n, 256), 256, 0, st>>>(n, rowStart, jStart, extStart, slice_len, sord);
     LAUNCH_CHECK();
