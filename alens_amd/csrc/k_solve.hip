@@ -384,12 +384,13 @@ struct SplitMeta {
 // solve: a pass of a stopped system costs one LDS read instead of five dependent global loads
 constexpr int kPasses = 6;
 
-// Slice of a window that wave w walks in pass q.  Slices are in descending degree within each
-// window, so with the identity every pass gives wave 0 the heaviest slice and the block waits on that
-// one wave.  Passes q = 0, 1, 2, 3 (mod 4) give wave w the slices w, W-1-w, (w + W/2) % W and
-// W-1-(w + W/2) % W, so the heaviest and lightest slices of successive windows go to the same
-// waves.  Every position of pass q lies below every position of pass q + 1, so a lane whose pass-q
-// position is past the rods has no later pass either.
+// Slice of its block's 1024 positions that wave w walks in pass q.  Slices are in descending degree
+// within each degree window (1024 or 128 rods), so with the identity every pass gives the same waves
+// the heavy slices (wave 0 alone, or the even waves) and the block waits on them.  Passes
+// q = 0, 1, 2, 3 (mod 4) give wave w the slices w, W-1-w, (w + W/2) % W and W-1-(w + W/2) % W, so
+// each wave alternates between the heavy and the light ends.  Every position of pass q lies below
+// every position of pass q + 1, so a lane whose pass-q position is past the rods has no later pass
+// either.
 constexpr int kWaves = kBlk / 64;
 __device__ inline int pass_pos(int base, int q, int stride, int t) {
     const int w = t >> 6;
