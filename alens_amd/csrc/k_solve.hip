@@ -728,6 +728,10 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
             s_help[threadIdx.x] = help ? SplitMeta{rm0.idx, rm0.deg, rm0.sy} : SplitMeta{0u, 0u, 0};
         }
     }
+    // each system's chunk range, fixed through the solve: the fold reads it from LDS instead of
+    // starting every fold with a dependent global load
+    __shared__ uint32_t s_choff[kMaxSys + 1];
+    for (int q = threadIdx.x; q <= ns; q += blockDim.x) s_choff[q] = a.chunk_off[q];
     for (int q = threadIdx.x; q < ns; q += blockDim.x) {
         s_alpha[q] = 0.0;
         s_state[q] = S_ACTIVE;
@@ -753,7 +757,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     if (!grid_barrier(a.bar, epoch, &s_flag, bm, bg)) return;
     {
         const int S[1] = {Q_NF}, X[1] = {Q_RMAX};
-        fold_systems<1, 1>(a.part, a.nchunks, a.chunk_off, ns, s_state, S, X, sh, tot);
+        fold_systems<1, 1>(a.part, a.nchunks, s_choff, ns, s_state, S, X, sh, tot);
     }
     if (a.warm) {
         // tot[0] = NF0, tot[1] = R0 -> keep them in tot[2], tot[3]
@@ -763,7 +767,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
         }
         __syncthreads();
         const int S[1] = {Q_NFC}, X[1] = {Q_RC};
-        fold_systems<1, 1>(a.part, a.nchunks, a.chunk_off, ns, s_state, S, X, sh, tot);
+        fold_systems<1, 1>(a.part, a.nchunks, s_choff, ns, s_state, S, X, sh, tot);
         // tot[0] = NFc, tot[1] = Rc
         if (threadIdx.x == 0) s_any = 0;
         __syncthreads();
@@ -812,7 +816,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
         if (!grid_barrier(a.bar, epoch, &s_flag, bm, bg)) return;
         {
             const int S[2] = {Q_SS, Q_SY}, X[1] = {Q_RMAX};
-            fold_systems<2, 1>(a.part, a.nchunks, a.chunk_off, ns, s_state, S, X, sh, tot);
+            fold_systems<2, 1>(a.part, a.nchunks, s_choff, ns, s_state, S, X, sh, tot);
         }
         for (int q = threadIdx.x; q < ns; q += blockDim.x) {
             if (s_state[q] != S_ACTIVE) continue;
@@ -832,7 +836,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
             if (!grid_barrier(a.bar, epoch, &s_flag, bm, bg)) return;
             {
                 const int S[4] = {Q_SS, Q_SY, Q_YY, Q_NF}, X[1] = {Q_RMAX};
-                fold_systems<4, 1>(a.part, a.nchunks, a.chunk_off, ns, s_state, S, X, sh, tot);
+                fold_systems<4, 1>(a.part, a.nchunks, s_choff, ns, s_state, S, X, sh, tot);
             }
             if (threadIdx.x == 0) s_any = 0;
             __syncthreads();
