@@ -835,8 +835,17 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
                 phase_B<B_ITER, T, 128>(a, s_alpha, s_state, sh, s_cm, nch);
             if (!grid_barrier(a.bar, epoch, &s_flag, bm, bg)) return;
             {
-                const int S[4] = {Q_SS, Q_SY, Q_YY, Q_NF}, X[1] = {Q_RMAX};
-                fold_systems<4, 1>(a.part, a.nchunks, s_choff, ns, s_state, S, X, sh, tot);
+                // the step length of this iteration needs two of the three sums: s^T s / s^T y
+                // (odd) or s^T y / y^T y (even); the fold reads only those, so tot[0] / tot[1] is
+                // the step either way, tot[2] the non-finite count and tot[3] the residual
+                const int X[1] = {Q_RMAX};
+                if (it & 1) {
+                    const int S[3] = {Q_SS, Q_SY, Q_NF};
+                    fold_systems<3, 1>(a.part, a.nchunks, s_choff, ns, s_state, S, X, sh, tot);
+                } else {
+                    const int S[3] = {Q_SY, Q_YY, Q_NF};
+                    fold_systems<3, 1>(a.part, a.nchunks, s_choff, ns, s_state, S, X, sh, tot);
+                }
             }
             if (threadIdx.x == 0) s_any = 0;
             __syncthreads();
@@ -844,13 +853,13 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
                 if (s_state[q] != S_ACTIVE) continue;
                 ++s_mv[q];
                 s_iters[q] = it;
-                s_res[q] = tot[4][q];
-                if (tot[3][q] > 0.0) {
+                s_res[q] = tot[3][q];
+                if (tot[2][q] > 0.0) {
                     s_state[q] = S_NONFINITE;
-                } else if (tot[4][q] <= a.tol) {
+                } else if (tot[3][q] <= a.tol) {
                     s_state[q] = S_CONVERGED;
                 } else {
-                    const double an = (it & 1) ? tot[0][q] / tot[1][q] : tot[1][q] / tot[2][q];
+                    const double an = tot[0][q] / tot[1][q];
                     if (an > 0.0 && an < INFINITY) s_alpha[q] = an;
                     s_any = 1;
                 }
