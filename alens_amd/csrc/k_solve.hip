@@ -199,10 +199,89 @@ __device__ inline void group_fold(const double (&vs)[NS], const double (&vx)[NX]
 // tot[q][s] (q < NS: sums, then the maxes) for every thread; systems not active are skipped.
 enum : int { S_ACTIVE = 0, S_CONVERGED = 1, S_NONFINITE = 2, S_NOCONV = 3 };
 
+// The same totals with one wave per system, for many small systems (replicas): lane l of the wave
+// holds the virtual lanes l, l + 64, l + 128 and l + 192 of the system's 256-lane group, accumulates
+// each one's chunks in order, folds each virtual wave by halving with shuffles and adds the four
+// results left to right -- the order above, with no LDS exchange and sixteen systems in flight.
+constexpr int kWaveFoldChunks = 4 * kRedThreads;  // most chunks per system the wave fold takes
+template <int NS, int NX>
+__device__ inline void fold_waves(const double *part, int nchunks, const uint32_t *chunk_off, int nsys,
+                                  const int *state, const int (&S)[NS], const int (&X)[NX], double (*tot)[kMaxSys]) {
+    constexpr int NQ = NS + NX, kW = kBlk / 64;
+    const int wv = threadIdx.x >> 6, wl = threadIdx.x & 63;
+    for (int s0 = 0; s0 < nsys; s0 += kW) {
+        const int sy = s0 + wv;
+        double vs[4][NS], vx[4][NX];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+#pragma unroll
+            for (int q = 0; q < NS; ++q) vs[v][q] = 0.0;
+#pragma unroll
+            for (int q = 0; q < NX; ++q) vx[v][q] = 0.0;
+        }
+        if (sy < nsys && state[sy] == S_ACTIVE) {
+            const int cb = (int)chunk_off[sy], c1 = (int)chunk_off[sy + 1];
+            // chunk cb + v * 64 + wl + 256 j is term j of virtual lane v * 64 + wl
+            for (int c = cb + wl; c < c1; c += kRedThreads) {
+                double ls[4][NS], lx[4][NX];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int cc = c + v * 64;
+#pragma unroll
+                    for (int q = 0; q < NS; ++q) ls[v][q] = cc < c1 ? part[(size_t)S[q] * nchunks + cc] : 0.0;
+#pragma unroll
+                    for (int q = 0; q < NX; ++q) lx[v][q] = cc < c1 ? part[(size_t)X[q] * nchunks + cc] : 0.0;
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    if (c + v * 64 >= c1) continue;
+#pragma unroll
+                    for (int q = 0; q < NS; ++q) vs[v][q] = vs[v][q] + ls[v][q];
+#pragma unroll
+                    for (int q = 0; q < NX; ++q) vx[v][q] = lx[v][q] > vx[v][q] ? lx[v][q] : vx[v][q];
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            double t[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                double x = q < NS ? vs[v][q] : vx[v][q - NS];
+#pragma unroll
+                for (int h = 32; h >= 1; h >>= 1) {
+                    const double o = __shfl_down(x, h, 64);
+                    if (q >= NS)
+                        x = o > x ? o : x;
+                    else
+                        x = x + o;
+                }
+                t[v] = x;
+            }
+            if (wl == 0 && sy < nsys) {
+                if (q >= NS) {
+                    double m = t[0];
+                    m = t[1] > m ? t[1] : m;
+                    m = t[2] > m ? t[2] : m;
+                    m = t[3] > m ? t[3] : m;
+                    tot[q][sy] = m;
+                } else {
+                    tot[q][sy] = ((t[0] + t[1]) + t[2]) + t[3];
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
 template <int NS, int NX>
 __device__ inline void fold_systems(const double *part, int nchunks, const uint32_t *chunk_off, int nsys,
                                     const int *state, const int (&S)[NS], const int (&X)[NX], double *sh,
-                                    double (*tot)[kMaxSys]) {
+                                    double (*tot)[kMaxSys], bool by_wave) {
+    if (by_wave) {
+        fold_waves<NS, NX>(part, nchunks, chunk_off, nsys, state, S, X, tot);
+        return;
+    }
     const int g = threadIdx.x >> 8, lane = threadIdx.x & 255;
     for (int s0 = 0; s0 < nsys; s0 += kGrp) {
         const int sy = s0 + g;
@@ -678,7 +757,9 @@ __device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, co
 
 // Per-system report words in sysout[8 s ..]: iters, matvecs, converged, fallback, status, res0, res,
 // alpha at exit.  All blocks hold identical per-system state, so control flow stays uniform.
-template <typename T, int AG, bool SPLIT>
+// MANY: the instance for more systems than reduction groups per block, whose fold may take one wave
+// per system (a separate instance, so the single-system kernels keep their register budget)
+template <typename T, int AG, bool SPLIT, bool MANY>
 __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     __shared__ double sh[kGrpB * 8 * 4];
     __shared__ int s_flag;
@@ -732,6 +813,14 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     // starting every fold with a dependent global load
     __shared__ uint32_t s_choff[kMaxSys + 1];
     for (int q = threadIdx.x; q <= ns; q += blockDim.x) s_choff[q] = a.chunk_off[q];
+    // many systems, none with more chunks than a wave folds quickly: one wave per system in the fold
+    __shared__ int s_big;
+    if (threadIdx.x == 0) s_big = 0;
+    __syncthreads();
+    for (int q = threadIdx.x; q < ns; q += blockDim.x)
+        if (s_choff[q + 1] - s_choff[q] > (uint32_t)kWaveFoldChunks) s_big = 1;
+    __syncthreads();
+    const bool by_wave = MANY && ns > kGrp && !s_big;
     for (int q = threadIdx.x; q < ns; q += blockDim.x) {
         s_alpha[q] = 0.0;
         s_state[q] = S_ACTIVE;
@@ -757,7 +846,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     if (!grid_barrier(a.bar, epoch, &s_flag, bm, bg)) return;
     {
         const int S[1] = {Q_NF}, X[1] = {Q_RMAX};
-        fold_systems<1, 1>(a.part, a.nchunks, s_choff, ns, s_state, S, X, sh, tot);
+        fold_systems<1, 1>(a.part, a.nchunks, s_choff, ns, s_state, S, X, sh, tot, by_wave);
     }
     if (a.warm) {
         // tot[0] = NF0, tot[1] = R0 -> keep them in tot[2], tot[3]
@@ -767,7 +856,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
         }
         __syncthreads();
         const int S[1] = {Q_NFC}, X[1] = {Q_RC};
-        fold_systems<1, 1>(a.part, a.nchunks, s_choff, ns, s_state, S, X, sh, tot);
+        fold_systems<1, 1>(a.part, a.nchunks, s_choff, ns, s_state, S, X, sh, tot, by_wave);
         // tot[0] = NFc, tot[1] = Rc
         if (threadIdx.x == 0) s_any = 0;
         __syncthreads();
@@ -816,7 +905,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
         if (!grid_barrier(a.bar, epoch, &s_flag, bm, bg)) return;
         {
             const int S[2] = {Q_SS, Q_SY}, X[1] = {Q_RMAX};
-            fold_systems<2, 1>(a.part, a.nchunks, s_choff, ns, s_state, S, X, sh, tot);
+            fold_systems<2, 1>(a.part, a.nchunks, s_choff, ns, s_state, S, X, sh, tot, by_wave);
         }
         for (int q = threadIdx.x; q < ns; q += blockDim.x) {
             if (s_state[q] != S_ACTIVE) continue;
@@ -841,10 +930,10 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
                 const int X[1] = {Q_RMAX};
                 if (it & 1) {
                     const int S[3] = {Q_SS, Q_SY, Q_NF};
-                    fold_systems<3, 1>(a.part, a.nchunks, s_choff, ns, s_state, S, X, sh, tot);
+                    fold_systems<3, 1>(a.part, a.nchunks, s_choff, ns, s_state, S, X, sh, tot, by_wave);
                 } else {
                     const int S[3] = {Q_SY, Q_YY, Q_NF};
-                    fold_systems<3, 1>(a.part, a.nchunks, s_choff, ns, s_state, S, X, sh, tot);
+                    fold_systems<3, 1>(a.part, a.nchunks, s_choff, ns, s_state, S, X, sh, tot, by_wave);
                 }
             }
             if (threadIdx.x == 0) s_any = 0;
@@ -884,12 +973,12 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     }
 }
 
-template <typename T, int AG, bool SPLIT>
+template <typename T, int AG, bool SPLIT, bool MANY>
 static HipErr launch_bbpgd_k(hipStream_t st, SolveArgsT<T> a, int max_blocks, int *grid_used) {
     int dev = 0, ncu = 0, per_cu = 0;
     ALENS_HIP_TRY(hipGetDevice(&dev));
     ALENS_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    ALENS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_bbpgd<T, AG, SPLIT>, kBlk, 0));
+    ALENS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_bbpgd<T, AG, SPLIT, MANY>, kBlk, 0));
     if (per_cu < 1) per_cu = 1;
     int64_t need = (int64_t)nblk(a.n, kBlk);
     const int64_t needB = ((int64_t)a.nchunks + kGrpB - 1) / kGrpB;
@@ -900,7 +989,7 @@ static HipErr launch_bbpgd_k(hipStream_t st, SolveArgsT<T> a, int max_blocks, in
     const int grid = (int)(need < lim ? need : lim);
     *grid_used = grid;
     void *args[] = {&a};
-    ALENS_HIP_TRY(hipLaunchCooperativeKernel((const void *)k_bbpgd<T, AG, SPLIT>, dim3(grid), dim3(kBlk), args, 0, st));
+    ALENS_HIP_TRY(hipLaunchCooperativeKernel((const void *)k_bbpgd<T, AG, SPLIT, MANY>, dim3(grid), dim3(kBlk), args, 0, st));
     return hip_ok();
 }
 
@@ -914,9 +1003,10 @@ static HipErr launch_bbpgd(hipStream_t st, SolveArgsT<T> a, int max_blocks, int 
         ALENS_HIP_TRY(hipGetDevice(&dev));
         ALENS_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
         const int64_t lanes = (int64_t)(max_blocks > 0 && max_blocks < ncu ? max_blocks : ncu) * kBlk;
-        if (a.split_ok && (int64_t)a.n <= lanes) return launch_bbpgd_k<T, AG, true>(st, a, max_blocks, grid_used);
+        if (a.split_ok && (int64_t)a.n <= lanes) return launch_bbpgd_k<T, AG, true, false>(st, a, max_blocks, grid_used);
     }
-    return launch_bbpgd_k<T, AG, false>(st, a, max_blocks, grid_used);
+    if (a.nsys > kGrp) return launch_bbpgd_k<T, AG, false, true>(st, a, max_blocks, grid_used);
+    return launch_bbpgd_k<T, AG, false, false>(st, a, max_blocks, grid_used);
 }
 
 HipErr launch_bbpgd64(hipStream_t st, SolveArgsT<double> a, int max_blocks, int entries, int *grid_used) {

@@ -369,6 +369,45 @@ def test_batched_systems_match_single_system_oracle():
                 assert np.all(cc[:, k] >= cfg["box_lo"][k]) and np.all(cc[:, k] < cfg["box_hi"][k])
 
 
+@pytest.mark.parametrize("case", ["nine_small", "one_large"])
+def test_many_systems_match_single_system_oracle(case):
+    """More systems than reduction groups per block: with nine small systems the fold takes one
+    wave per system; with a system of more than 1,024 chunks among them it keeps the 256-lane
+    groups.  Either way every system is bit-identical to its own single-system oracle run."""
+    if case == "nine_small":
+        specs = [systems.small_3d(n=300 + 60 * q, box=3.5 + 0.1 * q, seed=70 + q) for q in range(6)]
+        specs += [systems.scaled("monolayer_slab", 2500 + 500 * q, seed=80 + q) for q in range(3)]
+    else:
+        specs = [systems.scaled("lognormal_3d", 180000, seed=90)]
+        specs += [systems.small_3d(n=400 + 100 * q, box=3.2, seed=91 + q) for q in range(4)]
+    off = 0
+    parts = []
+    for rods, cfg in specs:
+        parts.append((dict(rods, gid=rods["gid"] + off), cfg))
+        off += len(rods["length"])
+    base = specs[0][1]
+    ctx = capi.Context(**base)
+    offs = ctx.set_systems(parts)
+    ctx.step()
+    g = ctx.constraint_arrays()
+    gam = ctx.gamma()
+    vel = ctx.velocity()
+    sreps = ctx.system_reports()
+    ctx.close()
+    nch = [(int(r["n_con"]) + 511) // 512 for r in sreps]
+    assert len(sreps) == len(specs) > 4
+    assert (max(nch) > 1024) == (case == "one_large"), nch
+    for s, (r, cfg) in enumerate(parts):
+        o = O.step(r, O.cfg_from_dict(dict(base, box_lo=cfg["box_lo"], box_hi=cfg["box_hi"], pbc=cfg["pbc"],
+                                           monolayer=cfg["monolayer"])))
+        lo, hi = offs[s], offs[s + 1]
+        sel = (g["I"] >= lo) & (g["I"] < hi)
+        assert np.array_equal(g["I"][sel] - lo, o["I"]) and np.array_equal(g["J"][sel] - lo, o["J"])
+        assert_bits(gam[sel], o["gamma"], f"{case}: system {s} gamma")
+        assert_bits(vel["u"][6 * lo:6 * hi], o["u"], f"{case}: system {s} U")
+        assert sreps[s]["iters"] == o["report"]["iters"], (case, s, sreps[s], o["report"])
+
+
 def test_autotune_keeps_results_bit_identical():
     """The autotuner times the last step's solve under candidate schedules and keeps the fastest; the
     step's outputs stay as they were, and later steps are bit-identical to an untuned context."""
