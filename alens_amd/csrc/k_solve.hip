@@ -158,6 +158,44 @@ __device__ inline bool grid_barrier(uint32_t *bar, uint32_t &epoch, int *s_flag,
 // partial is already accumulated left to right; each wave folds by halving (lane l += lane l + h,
 // h = 32 .. 1); the group's four wave results are added (or compared) left to right, with one LDS
 // exchange for all of them.  Called by every thread of the block (all groups together).
+// Halving fold of one wave: lane l takes lane l + h for h = 32, 16, ..., 1 (sums: x + o; maxima:
+// o > x ? o : x) and the result is read from lane 0.  The partner values come from the gfx950 lane
+// swaps (h = 32 and 16) and DPP row shifts (h <= 8), register moves without the LDS round trip of a
+// general shuffle; lanes at or above h receive values that never reach lane 0.
+__device__ inline double lanes_up32(double x) {
+    const unsigned lo = (unsigned)__double2loint(x), hi = (unsigned)__double2hiint(x);
+    const auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto b = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    return __hiloint2double((int)b[1], (int)a[1]);
+}
+__device__ inline double lanes_up16(double x) {
+    const unsigned lo = (unsigned)__double2loint(x), hi = (unsigned)__double2hiint(x);
+    const auto a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto b = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    return __hiloint2double((int)b[1], (int)a[1]);
+}
+template <int H>
+__device__ inline double lanes_up_row(double x) {  // DPP row_shl:H (H < 16)
+    const unsigned lo = (unsigned)__double2loint(x), hi = (unsigned)__double2hiint(x);
+    return __hiloint2double((int)__builtin_amdgcn_update_dpp(0u, hi, 0x100 + H, 0xF, 0xF, true),
+                            (int)__builtin_amdgcn_update_dpp(0u, lo, 0x100 + H, 0xF, 0xF, true));
+}
+template <bool MAX>
+__device__ inline double fold_step(double x, double o) {
+    if (MAX) return o > x ? o : x;
+    return x + o;
+}
+template <bool MAX>
+__device__ inline double wave_fold(double x) {
+    x = fold_step<MAX>(x, lanes_up32(x));
+    x = fold_step<MAX>(x, lanes_up16(x));
+    x = fold_step<MAX>(x, lanes_up_row<8>(x));
+    x = fold_step<MAX>(x, lanes_up_row<4>(x));
+    x = fold_step<MAX>(x, lanes_up_row<2>(x));
+    x = fold_step<MAX>(x, lanes_up_row<1>(x));
+    return x;
+}
+
 template <int NS, int NX>
 __device__ inline void group_fold(const double (&vs)[NS], const double (&vx)[NX], double *sh /* kGrp * (NS + NX) * 4 */,
                                   double (&ts)[NS], double (&tx)[NX]) {
@@ -166,14 +204,7 @@ __device__ inline void group_fold(const double (&vs)[NS], const double (&vx)[NX]
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         double x = q < NS ? vs[q] : vx[q - NS];
-#pragma unroll
-        for (int h = 32; h >= 1; h >>= 1) {
-            const double o = __shfl_down(x, h, 64);
-            if (q >= NS)
-                x = o > x ? o : x;
-            else
-                x = x + o;
-        }
+        x = q >= NS ? wave_fold<true>(x) : wave_fold<false>(x);
         if (lane == 0) sh[(g * NQ + q) * 4 + w] = x;
     }
     __syncthreads();
@@ -248,14 +279,7 @@ __device__ inline void fold_waves(const double *part, int nchunks, const uint32_
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 double x = q < NS ? vs[v][q] : vx[v][q - NS];
-#pragma unroll
-                for (int h = 32; h >= 1; h >>= 1) {
-                    const double o = __shfl_down(x, h, 64);
-                    if (q >= NS)
-                        x = o > x ? o : x;
-                    else
-                        x = x + o;
-                }
+                x = q >= NS ? wave_fold<true>(x) : wave_fold<false>(x);
                 t[v] = x;
             }
             if (wl == 0 && sy < nsys) {
@@ -592,14 +616,7 @@ __device__ inline void vgroup_fold(const double (*vs)[GroupShape<GL>::vpt], cons
 #pragma unroll
         for (int j = 0; j < VPT; ++j) {
             double x = q < NS ? vs[q][j] : vx[q - NS][j];
-#pragma unroll
-            for (int h = 32; h >= 1; h >>= 1) {
-                const double o = __shfl_down(x, h, 64);
-                if (q >= NS)
-                    x = o > x ? o : x;
-                else
-                    x = x + o;
-            }
+            x = q >= NS ? wave_fold<true>(x) : wave_fold<false>(x);
             if (lane == 0) sh[(g * NQ + q) * 4 + pw + j * (GL / 64)] = x;
         }
     }
