@@ -78,6 +78,7 @@ class TuneReport(ctypes.Structure):
         ("entries", ctypes.c_int32), ("bar_mode", ctypes.c_int32), ("xcd_map", ctypes.c_int32),
         ("max_blocks", ctypes.c_int32), ("candidates", ctypes.c_int32),
         ("default_ms", ctypes.c_float), ("best_ms", ctypes.c_float), ("group_width", ctypes.c_int32),
+        ("stream_hint", ctypes.c_int32),
     ]
 
     def as_dict(self):

@@ -279,6 +279,7 @@ struct SolveArgsT {
     int32_t split_ok;            // 1: heavy rods may take helper lanes in phase A (k_solve.hip)
     int32_t group_width;         // phase-B reduction groups: 0 = 256 lanes unless 128 saves a round of
                                  // chunks, 256 or 128 = always that width (k_solve.hip)
+    int32_t stream_nt;           // 1: phase B loads its row streams non-temporally (k_solve.hip)
 };
 using SolveArgs = SolveArgsT<double>;
 // entries: adjacency entries in flight per phase-A step (2, or 0 for the largest instantiated)
@@ -312,6 +313,8 @@ constexpr int kUStride = 6;
 constexpr int kBarWords = 32;
 constexpr int kBarGroups = 8;
 constexpr int kBarXcc = 16;  // XCC id slots (the hardware field is 4 bits wide)
+// MI355X's last-level (Infinity) cache, which serves data that HBM has already sent once
+constexpr double kLastLevelCacheBytes = 256.0 * 1024 * 1024;
 constexpr int kBarXccArr = 2 + kBarGroups, kBarXccSize = kBarXccArr + kBarXcc, kBarSetup = kBarXccSize + kBarXcc;
 constexpr int kBarTotalWords = (kBarSetup + 1) * kBarWords;
 

@@ -87,7 +87,7 @@ struct alens_ctx {
     bool have_xprev = false;
     DBuf<uint32_t> bar;
     // solver schedule (the autotuner's choices; environment variables override them unless tuning)
-    int32_t bar_mode = 3, xcd_map = 0, max_blocks = 0, entries = 0, group_width = 0;
+    int32_t bar_mode = 3, xcd_map = 0, max_blocks = 0, entries = 0, group_width = 0, stream_hint = 0;
     bool tuning = false;
     // device-resident step (graph mode): capacities, device counts, the captured step
     bool capture = false;                  // enqueueing into a graph: no host waits
@@ -746,7 +746,8 @@ static alens_status run_bbpgd_on(alens_ctx *ctx, SolveArgsT<T> a, const SolveMem
     a.nch_dev = ctx->cur_nchdev;
     a.m_dev = ctx->cur_mdev;
     // schedule: the context's (autotuned) choice, or the environment's (ALENS_SOLVER_MAX_BLOCKS,
-    // ALENS_BARRIER_MODE, ALENS_XCD_MAP, ALENS_SOLVER_ENTRIES) outside of tuning
+    // ALENS_BARRIER_MODE, ALENS_XCD_MAP, ALENS_SOLVER_ENTRIES, ALENS_GROUP_WIDTH, ALENS_STREAM_HINT)
+    // outside of tuning
     auto env = [&](const char *name, int dflt) {
         const char *v = ctx->tuning ? nullptr : getenv(name);
         return v ? atoi(v) : dflt;
@@ -758,6 +759,13 @@ static alens_status run_bbpgd_on(alens_ctx *ctx, SolveArgsT<T> a, const SolveMem
     a.split_ok = env("ALENS_SOLVER_SPLIT", 1) != 0;
     a.group_width = env("ALENS_GROUP_WIDTH", ctx->group_width);
     if (a.group_width != 128 && a.group_width != 256) a.group_width = 0;
+    // row streams of phase B: non-temporal when an iteration's bytes (per constraint 2 (9w + 8) + 5w,
+    // per rod 18w) overflow the GPU's last-level cache, which otherwise keeps them between iterations
+    // (a resident step's a.m is its row capacity; its last row count is the host's ctx->m)
+    const int hint = env("ALENS_STREAM_HINT", ctx->stream_hint);
+    const double rows = a.m_dev ? (double)ctx->m : (double)a.m;
+    const double iter_bytes = (double)sizeof(T) * (23.0 * rows + 18.0 * (double)a.n) + 16.0 * rows;
+    a.stream_nt = hint == 2 || (hint != 1 && iter_bytes > kLastLevelCacheBytes);
     const int entries = env("ALENS_SOLVER_ENTRIES", ctx->entries);
     // inside a graph the timing points are clock readings: stamp 2 before the first solver launch,
     // stamp 3 after each one (the last reading is the one read after the replay)
@@ -1677,6 +1685,7 @@ alens_status alens_autotune(alens_ctx *ctx, alens_tune_report *rep_out) {
         }
         return ALENS_OK;
     };
+    constexpr float kTuneMargin = 0.01f;
     alens_tune_report tr{};
     ctx->tuning = true;
     // coordinate descent from the defaults over the switches, one axis at a time
@@ -1685,6 +1694,7 @@ alens_status alens_autotune(alens_ctx *ctx, alens_tune_report *rep_out) {
     ctx->max_blocks = 0;
     ctx->entries = 0;
     ctx->group_width = 0;
+    ctx->stream_hint = 0;
     float best = 0.f;
     alens_status s = run(&best);
     tr.default_ms = best;
@@ -1700,6 +1710,7 @@ alens_status alens_autotune(alens_ctx *ctx, alens_tune_report *rep_out) {
         {&ctx->xcd_map, {0, 1}},
         {&ctx->max_blocks, {0, (int32_t)std::max<int64_t>(8, grid * 3 / 4), (int32_t)std::max<int64_t>(8, grid / 2)}},
         {&ctx->group_width, {0, 256, 128}},
+        {&ctx->stream_hint, {0, 1, 2}},
     };
     for (auto &ax : axes) {
         if (s != ALENS_OK) break;
@@ -1712,7 +1723,10 @@ alens_status alens_autotune(alens_ctx *ctx, alens_tune_report *rep_out) {
             s = run(&t);
             if (s != ALENS_OK) break;
             ++tr.candidates;
-            if (t < best) {
+            // a value replaces the current one only when it is clearly faster: the automatic
+            // values (entries, group width, stream hint) adapt to later steps' row counts, and a fixed
+            // value that ties with them on the tuned step can lose on later ones
+            if (t < best * (1.0f - kTuneMargin)) {
                 best = t;
                 pick = v;
             }
@@ -1727,6 +1741,7 @@ alens_status alens_autotune(alens_ctx *ctx, alens_tune_report *rep_out) {
     tr.max_blocks = ctx->max_blocks;
     tr.entries = ctx->entries;
     tr.group_width = ctx->group_width;
+    tr.stream_hint = ctx->stream_hint;
     tr.best_ms = best;
     if (rep_out) *rep_out = tr;
     return ALENS_OK;

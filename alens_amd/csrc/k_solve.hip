@@ -568,13 +568,23 @@ __device__ inline void phase_A(const SolveArgsT<T> &a, const double *s_alpha, co
 }
 
 
-template <typename T>
+// A load of a word that the phase reads once: with NT it carries the non-temporal hint, so that the
+// line goes out of the caches first and the rod velocities and the partials stay longer.
+template <bool NT, typename X>
+__device__ inline X ld_stream(const X *p) {
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
+template <typename T, bool NT>
 __device__ inline T du_row(const SolveArgsT<T> &a, uint32_t k) {
     using V2 = typename Vec2<T>::type;
     T gk[9];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) gk[q] = a.geo[(size_t)q * a.m + k];
-    const size_t I = (size_t)a.cI[k], J = (size_t)a.cJ[k];
+    for (int q = 0; q < 9; ++q) gk[q] = ld_stream<NT>(a.geo + (size_t)q * a.m + k);
+    const size_t I = (size_t)ld_stream<NT>(a.cI + k), J = (size_t)ld_stream<NT>(a.cJ + k);
     const V2 *UI = reinterpret_cast<const V2 *>(a.U + kUStride * I);
     const V2 *UJ = reinterpret_cast<const V2 *>(a.U + kUStride * J);
     T uI[6], uJ[6];
@@ -658,8 +668,8 @@ __device__ inline ChunkMeta chunk_meta(const SolveArgsT<T> &a, int c, int nch) {
 }
 
 // phase B: constraints in 512-element chunks, one chunk per GL-lane group; chunk partials go to
-// part[q * nchunks + c]
-template <int MODE, typename T, int GL>
+// part[q * nchunks + c].  NT: the row streams (geometry, rod ids, rhs) load non-temporally.
+template <int MODE, typename T, int GL, bool NT = false>
 __device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, const int *s_state, double *sh,
                                const ChunkMeta (*s_cm)[kGrpB], int nch) {
     constexpr int kVPT = GroupShape<GL>::vpt, G = GroupShape<GL>::per_block;
@@ -699,8 +709,8 @@ __device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, co
                         const typename Vec2<T>::type xg = a.xg[kk[q]];
                         xo[q] = xg.x;
                         go[q] = xg.y;
-                        if (MODE != B_ALPHA0) bb[q] = a.b[kk[q]];
-                        du[q] = du_row<T>(a, kk[q]);
+                        if (MODE != B_ALPHA0) bb[q] = ld_stream<NT>(a.b + kk[q]);
+                        du[q] = du_row<T, NT>(a, kk[q]);
                     }
                 }
             }
@@ -775,8 +785,10 @@ __device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, co
 // Per-system report words in sysout[8 s ..]: iters, matvecs, converged, fallback, status, res0, res,
 // alpha at exit.  All blocks hold identical per-system state, so control flow stays uniform.
 // MANY: the instance for more systems than reduction groups per block, whose fold may take one wave
-// per system (a separate instance, so the single-system kernels keep their register budget)
-template <typename T, int AG, bool SPLIT, bool MANY>
+// per system (a separate instance, so the single-system kernels keep their register budget).
+// NT: the constraint phase's row streams load non-temporally (an instance of its own for the same
+// reason: both load forms in one kernel spilled 16 B more per lane and slowed the 200k slab)
+template <typename T, int AG, bool SPLIT, bool MANY, bool NT>
 __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     __shared__ double sh[kGrpB * 8 * 4];
     __shared__ int s_flag;
@@ -936,9 +948,9 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
             phase_A<A_PROJ, T, AG, SPLIT>(a, s_alpha, s_state, rm0, split, s_help, s_prod, s_rmeta);
             if (!grid_barrier(a.bar, epoch, &s_flag, bm, bg)) return;
             if (wide || SPLIT)
-                phase_B<B_ITER, T, 256>(a, s_alpha, s_state, sh, s_cm, nch);
+                phase_B<B_ITER, T, 256, NT>(a, s_alpha, s_state, sh, s_cm, nch);
             else
-                phase_B<B_ITER, T, 128>(a, s_alpha, s_state, sh, s_cm, nch);
+                phase_B<B_ITER, T, 128, NT>(a, s_alpha, s_state, sh, s_cm, nch);
             if (!grid_barrier(a.bar, epoch, &s_flag, bm, bg)) return;
             {
                 // the step length of this iteration needs two of the three sums: s^T s / s^T y
@@ -990,12 +1002,12 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     }
 }
 
-template <typename T, int AG, bool SPLIT, bool MANY>
+template <typename T, int AG, bool SPLIT, bool MANY, bool NT>
 static HipErr launch_bbpgd_k(hipStream_t st, SolveArgsT<T> a, int max_blocks, int *grid_used) {
     int dev = 0, ncu = 0, per_cu = 0;
     ALENS_HIP_TRY(hipGetDevice(&dev));
     ALENS_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    ALENS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_bbpgd<T, AG, SPLIT, MANY>, kBlk, 0));
+    ALENS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_bbpgd<T, AG, SPLIT, MANY, NT>, kBlk, 0));
     if (per_cu < 1) per_cu = 1;
     int64_t need = (int64_t)nblk(a.n, kBlk);
     const int64_t needB = ((int64_t)a.nchunks + kGrpB - 1) / kGrpB;
@@ -1006,7 +1018,7 @@ static HipErr launch_bbpgd_k(hipStream_t st, SolveArgsT<T> a, int max_blocks, in
     const int grid = (int)(need < lim ? need : lim);
     *grid_used = grid;
     void *args[] = {&a};
-    ALENS_HIP_TRY(hipLaunchCooperativeKernel((const void *)k_bbpgd<T, AG, SPLIT, MANY>, dim3(grid), dim3(kBlk), args, 0, st));
+    ALENS_HIP_TRY(hipLaunchCooperativeKernel((const void *)k_bbpgd<T, AG, SPLIT, MANY, NT>, dim3(grid), dim3(kBlk), args, 0, st));
     return hip_ok();
 }
 
@@ -1020,10 +1032,14 @@ static HipErr launch_bbpgd(hipStream_t st, SolveArgsT<T> a, int max_blocks, int 
         ALENS_HIP_TRY(hipGetDevice(&dev));
         ALENS_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
         const int64_t lanes = (int64_t)(max_blocks > 0 && max_blocks < ncu ? max_blocks : ncu) * kBlk;
-        if (a.split_ok && (int64_t)a.n <= lanes) return launch_bbpgd_k<T, AG, true, false>(st, a, max_blocks, grid_used);
+        if (a.split_ok && (int64_t)a.n <= lanes)
+            return launch_bbpgd_k<T, AG, true, false, false>(st, a, max_blocks, grid_used);
     }
-    if (a.nsys > kGrp) return launch_bbpgd_k<T, AG, false, true>(st, a, max_blocks, grid_used);
-    return launch_bbpgd_k<T, AG, false, false>(st, a, max_blocks, grid_used);
+    if (a.nsys > kGrp)
+        return a.stream_nt ? launch_bbpgd_k<T, AG, false, true, true>(st, a, max_blocks, grid_used)
+                           : launch_bbpgd_k<T, AG, false, true, false>(st, a, max_blocks, grid_used);
+    return a.stream_nt ? launch_bbpgd_k<T, AG, false, false, true>(st, a, max_blocks, grid_used)
+                       : launch_bbpgd_k<T, AG, false, false, false>(st, a, max_blocks, grid_used);
 }
 
 HipErr launch_bbpgd64(hipStream_t st, SolveArgsT<double> a, int max_blocks, int entries, int *grid_used) {

@@ -192,12 +192,14 @@ alens_status alens_get_velocity(alens_ctx *ctx, double *u, double *u_nc, double 
  * times cold solves of the last step's rows under candidate schedules - adjacency entries in flight
  * per rod step (largest, 2), grid-barrier variant (grouped by XCC, grouped, flat, grouped with longer polls),
  * XCD-contiguous work mapping, persistent grid size (all co-resident, 3/4, 1/2), reduction-group width of
- * the constraint phase (automatic, 256, 128 lanes) - one axis at a time
- * from the defaults, keeps the fastest in the context for later steps, and reports the choice.
+ * the constraint phase (automatic, 256, 128 lanes), cache hint of the constraint phase's row streams
+ * (automatic, cached, non-temporal) - one axis at a time
+ * from the defaults (a value replaces the current one only when at least 1% faster), keeps the
+ * fastest in the context for later steps, and reports the choice.
  * Results are bit-identical under every schedule; only the time changes.  Needs a step with
  * constraints; leaves that step's outputs untouched.  Environment variables ALENS_SOLVER_ENTRIES,
- * ALENS_BARRIER_MODE, ALENS_XCD_MAP, ALENS_SOLVER_MAX_BLOCKS and ALENS_GROUP_WIDTH override the choice
- * outside tuning. */
+ * ALENS_BARRIER_MODE, ALENS_XCD_MAP, ALENS_SOLVER_MAX_BLOCKS, ALENS_GROUP_WIDTH and ALENS_STREAM_HINT
+ * override the choice outside tuning. */
 typedef struct {
     int32_t entries;     /* 0 = largest instantiated (3 in FP64, 4 in FP32), or 2 */
     int32_t bar_mode;    /* 0 flat, 1 grouped, 2 grouped with longer polls, 3 grouped by XCC with one
@@ -209,6 +211,8 @@ typedef struct {
     float best_ms;       /* solve time of the chosen schedule */
     int32_t group_width; /* phase-B reduction-group width: 0 = automatic (256 lanes unless 128 saves a
                             round of chunks), 256, 128 */
+    int32_t stream_hint; /* phase-B row streams: 0 = automatic (non-temporal loads when an iteration's
+                            bytes exceed the 256 MiB last-level cache), 1 = cached, 2 = non-temporal */
 } alens_tune_report;
 alens_status alens_autotune(alens_ctx *ctx, alens_tune_report *rep);
 

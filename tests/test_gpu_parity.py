@@ -425,6 +425,7 @@ def test_autotune_keeps_results_bit_identical():
         assert tr["candidates"] >= 6 and 0 < tr["best_ms"] <= tr["default_ms"], tr
         assert tr["entries"] in (0, 2) and tr["bar_mode"] in (0, 1, 2, 3) and tr["xcd_map"] in (0, 1), tr
         assert tr["group_width"] in (0, 128, 256), tr
+        assert tr["stream_hint"] in (0, 1, 2), tr
         assert_bits(a.gamma(), g_before, "gamma after tuning")
         assert_bits(a.velocity()["u"], b.velocity()["u"], "U after tuning")
         for s in range(3):
@@ -608,18 +609,15 @@ def test_barrier_variants_bit_identical_replicas(monkeypatch):
             assert_bits(ra[2], rb[2], f"U, barrier mode {mode}, step {s}")
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_group_width_variants_bit_identical(monkeypatch, precision):
-    """The constraint phase's reduction groups of 256 or 128 lanes (a 128-lane group holds two virtual
-    lanes per lane and folds them apart) give the fixed reduction order either way: iterates, gamma
-    and U agree bit for bit under ALENS_GROUP_WIDTH = 0 (automatic), 256 and 128, for one system and
-    for 12 replicas solved in one launch."""
+def _schedule_axis_bit_identical(monkeypatch, precision, var, values):
+    """Runs three steps of one 30k-rod slab and of 12 replicas in one launch under each value of the
+    schedule variable `var`; iteration counts, gamma and U must agree bit for bit with values[0]."""
     single = systems.scaled("monolayer_slab", 30000, seed=13)
     parts, rcfg = systems.replicas(nsys=12, nmin=2000, nmax=10000, seed=23)
     for kind in ("single", "replicas"):
         runs = {}
-        for width in (0, 256, 128):
-            monkeypatch.setenv("ALENS_GROUP_WIDTH", str(width))
+        for v in values:
+            monkeypatch.setenv(var, str(v))
             if kind == "single":
                 rods, cfg = single
                 c = capi.Context(**dict(cfg, precision=precision))
@@ -633,13 +631,29 @@ def test_group_width_variants_bit_identical(monkeypatch, precision):
                 r = c.step()
                 out.append((r["iters"], r["iters_fp32"], c.gamma().copy(), c.velocity()["u"].copy()))
                 c.euler()
-            runs[width] = out
+            runs[v] = out
             c.close()
-        for width in (256, 128):
-            for s, (ra, rb) in enumerate(zip(runs[0], runs[width])):
-                assert ra[0] == rb[0] and ra[1] == rb[1], (kind, width, s, ra[:2], rb[:2])
-                assert_bits(ra[2], rb[2], f"gamma, {kind}, width {width}, step {s}")
-                assert_bits(ra[3], rb[3], f"U, {kind}, width {width}, step {s}")
+        for v in values[1:]:
+            for s, (ra, rb) in enumerate(zip(runs[values[0]], runs[v])):
+                assert ra[0] == rb[0] and ra[1] == rb[1], (kind, var, v, s, ra[:2], rb[:2])
+                assert_bits(ra[2], rb[2], f"gamma, {kind}, {var}={v}, step {s}")
+                assert_bits(ra[3], rb[3], f"U, {kind}, {var}={v}, step {s}")
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_group_width_variants_bit_identical(monkeypatch, precision):
+    """The constraint phase's reduction groups of 256 or 128 lanes (a 128-lane group holds two virtual
+    lanes per lane and folds them apart) give the fixed reduction order either way: iterates, gamma
+    and U agree bit for bit under ALENS_GROUP_WIDTH = 0 (automatic), 256 and 128, for one system and
+    for 12 replicas solved in one launch."""
+    _schedule_axis_bit_identical(monkeypatch, precision, "ALENS_GROUP_WIDTH", (0, 256, 128))
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_stream_hint_bit_identical(monkeypatch, precision):
+    """The cache hint of the constraint phase's row streams changes only where lines are kept: results
+    agree bit for bit under ALENS_STREAM_HINT = 0 (automatic), 1 (cached) and 2 (non-temporal)."""
+    _schedule_axis_bit_identical(monkeypatch, precision, "ALENS_STREAM_HINT", (0, 1, 2))
 
 
 def test_copy_probe_rate():
