@@ -529,11 +529,14 @@ constexpr int kStage = 512;
 constexpr int kSweepPre = 64;                          // entries staged before the block's own
 constexpr int kSweepS = kSweepPre + kSweepT + 64;      // and after them
 
-// Box data of one sweep entry (rod id, box range start, inflated box), from the block's LDS stage
-// or, outside it, from global memory.
+// Box data of one sweep entry (rod id, unwrapped cell index, inflated box), from the block's LDS
+// stage or, outside it, from global memory.  The unwrapped index of a rod in a periodic direction is
+// the copy of the entry's cell that lies in the rod's box range: cl + ((c - cl) mod nc) for box range
+// start cl and cell c; two rods of one cell differ in it by a whole number of periods, their image
+// shift.  It depends on the entry only, so the stage computes it once instead of once per pair.
 struct SweepBox {
     uint32_t id;
-    int cl[3];
+    int u[3];
     double lo[3], hi[3];
 };
 
@@ -552,28 +555,45 @@ void k_sweep_entries(RodsDev r, CfgDev c, const uint32_t *__restrict__ keys, int
     __shared__ uint32_t s_pi[kStage], s_pj[kStage], s_pm[kStage];
     // box data of entries e0 - kSweepPre .. e0 + kSweepT + 64, structure of arrays
     __shared__ uint32_t s_id[kSweepS];
-    __shared__ int s_cl[3][kSweepS];
+    __shared__ int s_u[3][kSweepS];
     __shared__ double s_lo[3][kSweepS], s_hi[3][kSweepS];
     const int lane = threadIdx.x & 63;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const int64_t e0 = (int64_t)blockIdx.x * blockDim.x;
     const int64_t es = e0 - kSweepPre;  // first staged entry
     const int64_t e = e0 + threadIdx.x;
+    // cell coordinates of a cell id in its system's grid
+    auto cell_xyz = [](uint32_t lc, const int (&n)[3], int (&x)[3]) {
+        x[0] = (int)(lc % (uint32_t)n[0]);
+        x[1] = (int)((lc / (uint32_t)n[0]) % (uint32_t)n[1]);
+        x[2] = (int)(lc / ((uint32_t)n[0] * (uint32_t)n[1]));
+    };
     for (int q = threadIdx.x; q < kSweepS; q += blockDim.x) {
         const int64_t eq = es + q;
         if (eq < 0) continue;
         if (eq >= E) break;
         const uint32_t j = vals[eq];
+        const SysDev &g = r.sys[r.rsys[j]];
+        const int gn[3] = {g.nc[0], g.nc[1], g.nc[2]};
+        int x[3];
+        cell_xyz(keys[eq] - g.cell_off, gn, x);
         s_id[q] = j;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            s_cl[k][q] = clo[3 * j + k];
+            const int cl = clo[3 * j + k];
+            s_u[k][q] = g.pbc[k] ? cl + pmod_i(x[k] - cl, gn[k]) : x[k];
             s_lo[k][q] = blo[3 * j + k];
             s_hi[k][q] = bhi[3 * j + k];
         }
     }
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
+    // this lane's entry: cell, place r in the cell of kc entries, pair count
+    uint32_t cnt = 0, cs = 0, kc = 1, rr = 0;
+    int cc[3] = {0, 0, 0};
+    int pbc[3] = {0, 0, 0}, nc[3] = {1, 1, 1};
+    double len[3] = {0, 0, 0}, blo0[3] = {0, 0, 0}, hcell[3] = {1, 1, 1};
+    // an entry of this lane's cell (its own, or a partner outside the stage)
     auto box_of = [&](int64_t ex) {
         SweepBox w;
         const int64_t q = ex - es;
@@ -581,7 +601,7 @@ void k_sweep_entries(RodsDev r, CfgDev c, const uint32_t *__restrict__ keys, int
             w.id = s_id[q];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                w.cl[k] = s_cl[k][q];
+                w.u[k] = s_u[k][q];
                 w.lo[k] = s_lo[k][q];
                 w.hi[k] = s_hi[k][q];
             }
@@ -589,26 +609,22 @@ void k_sweep_entries(RodsDev r, CfgDev c, const uint32_t *__restrict__ keys, int
             w.id = vals[ex];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                w.cl[k] = clo[3 * w.id + k];
+                const int cl = clo[3 * w.id + k];
+                w.u[k] = pbc[k] ? cl + pmod_i(cc[k] - cl, nc[k]) : cc[k];
                 w.lo[k] = blo[3 * w.id + k];
                 w.hi[k] = bhi[3 * w.id + k];
             }
         }
         return w;
     };
-    // this lane's entry: cell, place r in the cell of kc entries, pair count
-    uint32_t cnt = 0, cs = 0, kc = 1, rr = 0;
-    int cc[3] = {0, 0, 0};
     SweepBox own{0u, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    int pbc[3] = {0, 0, 0}, nc[3] = {1, 1, 1};
-    double len[3] = {0, 0, 0}, blo0[3] = {0, 0, 0}, hcell[3] = {1, 1, 1};
     if (e < E) {
         const uint32_t cell = keys[e];
         cs = cstart[cell];
         kc = cend[cell] - cs;
         rr = (uint32_t)e - cs;
         cnt = (kc - 1u) / 2u + ((kc % 2u == 0u && rr < kc / 2u) ? 1u : 0u);
-        own = box_of(e);
+        own = box_of(e);  // always staged
         const SysDev &g = r.sys[r.rsys[own.id]];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -618,10 +634,7 @@ void k_sweep_entries(RodsDev r, CfgDev c, const uint32_t *__restrict__ keys, int
             blo0[k] = g.box_lo[k];
             hcell[k] = g.h[k];
         }
-        const uint32_t lc = cell - g.cell_off;
-        cc[0] = (int)(lc % (uint32_t)nc[0]);
-        cc[1] = (int)((lc / (uint32_t)nc[0]) % (uint32_t)nc[1]);
-        cc[2] = (int)(lc / ((uint32_t)nc[0] * (uint32_t)nc[1]));
+        cell_xyz(cell - g.cell_off, nc, cc);
     }
     uint32_t maxc = cnt;
 #pragma unroll
@@ -648,9 +661,9 @@ void k_sweep_entries(RodsDev r, CfgDev c, const uint32_t *__restrict__ keys, int
             bool ok = true;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                ui[k] = pbc[k] ? wi.cl[k] + pmod_i(cc[k] - wi.cl[k], nc[k]) : cc[k];
-                const int uj = pbc[k] ? wj.cl[k] + pmod_i(cc[k] - wj.cl[k], nc[k]) : cc[k];
-                m[k] = pbc[k] ? (ui[k] - uj) / nc[k] : 0;
+                ui[k] = wi.u[k];
+                const int du = ui[k] - wj.u[k];  // a whole number of periods (0 if not periodic)
+                m[k] = du == 0 ? 0 : du / nc[k];
                 const double sh = pbc[k] ? (double)m[k] * len[k] : 0.0;
                 ok = ok && (wj.lo[k] + sh <= wi.hi[k]) && (wi.lo[k] <= wj.hi[k] + sh);
             }
