@@ -1709,7 +1709,6 @@ alens_status alens_autotune(alens_ctx *ctx, alens_tune_report *rep_out) {
         {&ctx->bar_mode, {3, 1, 0, 2}},
         {&ctx->xcd_map, {0, 1}},
         {&ctx->max_blocks, {0, (int32_t)std::max<int64_t>(8, grid * 3 / 4), (int32_t)std::max<int64_t>(8, grid / 2)}},
-        {&ctx->group_width, {0, 256, 128}},
         {&ctx->stream_hint, {0, 1, 2}},
     };
     for (auto &ax : axes) {

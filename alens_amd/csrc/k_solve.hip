@@ -601,8 +601,8 @@ __device__ inline T du_row(const SolveArgsT<T> &a, uint32_t k) {
 
 enum : int { B_INIT = 0, B_ALPHA0 = 1, B_ITER = 2 };
 
-// Phase-B reduction groups.  A 512-element chunk is reduced by GL lanes (GL = 256, or 128 when that
-// saves a round of chunks); in the fixed order element e belongs to virtual lane e % 256 (pass
+// Phase-B reduction groups.  A 512-element chunk is reduced by GL lanes (GL = 256, or 128 when the
+// schedule asks for it); in the fixed order element e belongs to virtual lane e % 256 (pass
 // e / 256), and lane l of a group holds the virtual lanes l + j * GL (j < 256 / GL).  Each physical
 // wave folds its virtual waves separately by halving (lane l += lane l + h, h = 32 .. 1) and the four
 // virtual-wave totals are added left to right, so the result is the one a 256-lane group gives.
@@ -802,12 +802,13 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     // row and chunk counts: on the device for a step replayed from a captured graph
     const int nch = a.nch_dev ? (*a.nch_dev < a.nchunks ? *a.nch_dev : a.nchunks) : a.nchunks;
     const uint32_t mrows = dcount(a.m, a.m_dev);
-    // 256-lane groups (two elements per lane) unless 128-lane groups save a round of chunks
+    // 256-lane groups (two elements per lane) unless the schedule asks for 128 (four per lane, half the
+    // rounds of chunks).  With the wave folds in register moves, 128-lane groups were faster only
+    // around 1,000-1,100 chunks on 256 blocks (3.8-6%); at 645 chunks they were 12% slower, and at
+    // 1,500-6,300 chunks 1.4-4.8% slower.  The split kernel keeps to 256-lane groups (the 128-lane
+    // path would not fit its register budget).
     const int G = kGrpB / 2;
-    // (the split kernel keeps to 256-lane groups: the 128-lane path would not fit its register budget)
-    const bool wide = SPLIT || a.group_width == 256 ||
-                      (a.group_width != 128 && (nch + gridDim.x * G - 1) / (gridDim.x * G) ==
-                                                   (nch + gridDim.x * kGrpB - 1) / (gridDim.x * kGrpB));
+    const bool wide = SPLIT || a.group_width != 128;
     const int gb = wide ? G : kGrpB;
     if (threadIdx.x < kChunkRounds * gb) {
         const int r = threadIdx.x / gb, g = threadIdx.x % gb;
