@@ -278,6 +278,7 @@ struct SolveArgsT {
     int32_t xcd_map;             // 1: blocks of one XCD take consecutive work (k_solve.hip)
     int32_t split_ok;            // 1: heavy rods may take helper lanes in phase A (k_solve.hip)
     int32_t group_width;         // phase-B reduction groups: 128 = 128 lanes, else 256 (k_solve.hip)
+    uint32_t gw_rows_lo, gw_rows_hi;  // group_width applies to row counts in [lo, hi] (hi = 0: any)
     int32_t stream_nt;           // 1: phase B loads its row streams non-temporally (k_solve.hip)
 };
 using SolveArgs = SolveArgsT<double>;

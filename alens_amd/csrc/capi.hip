@@ -88,6 +88,7 @@ struct alens_ctx {
     DBuf<uint32_t> bar;
     // solver schedule (the autotuner's choices; environment variables override them unless tuning)
     int32_t bar_mode = 3, xcd_map = 0, max_blocks = 0, entries = 0, group_width = 0, stream_hint = 0;
+    uint32_t gw_rows = 0;  // row count of the step the group width was tuned on (0: not tuned)
     bool tuning = false;
     // device-resident step (graph mode): capacities, device counts, the captured step
     bool capture = false;                  // enqueueing into a graph: no host waits
@@ -759,6 +760,11 @@ static alens_status run_bbpgd_on(alens_ctx *ctx, SolveArgsT<T> a, const SolveMem
     a.split_ok = env("ALENS_SOLVER_SPLIT", 1) != 0;
     a.group_width = env("ALENS_GROUP_WIDTH", ctx->group_width);
     if (a.group_width != 128 && a.group_width != 256) a.group_width = 0;
+    // a tuned width holds within 10% of the tuned step's row count (one set by the environment, or by
+    // the tuner while it times candidates, holds for every row count)
+    const bool gw_band = !ctx->tuning && !getenv("ALENS_GROUP_WIDTH") && ctx->gw_rows > 0;
+    a.gw_rows_lo = gw_band ? (uint32_t)(0.9 * ctx->gw_rows) : 0u;
+    a.gw_rows_hi = gw_band ? (uint32_t)std::min(4294967295.0, 1.1 * ctx->gw_rows) : 0u;
     // row streams of phase B: non-temporal when an iteration's bytes (per constraint 2 (9w + 8) + 5w,
     // per rod 18w) overflow the GPU's last-level cache, which otherwise keeps them between iterations
     // (a resident step's a.m is its row capacity; its last row count is the host's ctx->m)
@@ -1694,6 +1700,7 @@ alens_status alens_autotune(alens_ctx *ctx, alens_tune_report *rep_out) {
     ctx->max_blocks = 0;
     ctx->entries = 0;
     ctx->group_width = 0;
+    ctx->gw_rows = 0;
     ctx->stream_hint = 0;
     float best = 0.f;
     alens_status s = run(&best);
@@ -1709,6 +1716,7 @@ alens_status alens_autotune(alens_ctx *ctx, alens_tune_report *rep_out) {
         {&ctx->bar_mode, {3, 1, 0, 2}},
         {&ctx->xcd_map, {0, 1}},
         {&ctx->max_blocks, {0, (int32_t)std::max<int64_t>(8, grid * 3 / 4), (int32_t)std::max<int64_t>(8, grid / 2)}},
+        {&ctx->group_width, {0, 128}},
         {&ctx->stream_hint, {0, 1, 2}},
     };
     for (auto &ax : axes) {
@@ -1734,6 +1742,9 @@ alens_status alens_autotune(alens_ctx *ctx, alens_tune_report *rep_out) {
     }
     ctx->tuning = false;
     ctx->sysrep = saved_rep;
+    // 128-lane groups won on this step's rows; later steps with other row counts may not gain (on the
+    // 200k slab they won at about 1,070 chunks and lost 12% at 645), so the choice holds near M only
+    ctx->gw_rows = ctx->group_width == 128 ? M : 0u;
     if (s != ALENS_OK) return s;
     tr.bar_mode = ctx->bar_mode;
     tr.xcd_map = ctx->xcd_map;

@@ -807,8 +807,11 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     // around 1,000-1,100 chunks on 256 blocks (3.8-6%); at 645 chunks they were 12% slower, and at
     // 1,500-6,300 chunks 1.4-4.8% slower.  The split kernel keeps to 256-lane groups (the 128-lane
     // path would not fit its register budget).
+    // A width the autotuner picked holds for row counts near the tuned step's (gw_rows_lo .. hi); the
+    // device's row count decides, so a replayed graph step follows it too.
     const int G = kGrpB / 2;
-    const bool wide = SPLIT || a.group_width != 128;
+    const bool gw_on = a.gw_rows_hi == 0 || (mrows >= a.gw_rows_lo && mrows <= a.gw_rows_hi);
+    const bool wide = SPLIT || !gw_on || a.group_width != 128;
     const int gb = wide ? G : kGrpB;
     if (threadIdx.x < kChunkRounds * gb) {
         const int r = threadIdx.x / gb, g = threadIdx.x % gb;

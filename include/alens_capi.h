@@ -191,11 +191,13 @@ alens_status alens_get_velocity(alens_ctx *ctx, double *u, double *u_nc, double 
 /* Solver autotuner (no counterpart in the reference, whose solver runs on Trilinos over MPI):
  * times cold solves of the last step's rows under candidate schedules - adjacency entries in flight
  * per rod step (largest, 2), grid-barrier variant (grouped by XCC, grouped, flat, grouped with longer polls),
- * XCD-contiguous work mapping, persistent grid size (all co-resident, 3/4, 1/2), cache hint of the
- * constraint phase's row streams (automatic, cached, non-temporal) - one axis at a time from the
- * defaults (a value replaces the current one only when at least 1% faster), keeps the fastest in the
- * context for later steps, and reports the choice.  The reduction-group width is not tuned: a width
- * fixed on one step's row count can lose on later steps with fewer rows (ALENS_GROUP_WIDTH sets it).
+ * XCD-contiguous work mapping, persistent grid size (all co-resident, 3/4, 1/2), reduction-group
+ * width of the constraint phase (automatic = 256, 128 lanes), cache hint of the constraint phase's
+ * row streams (automatic, cached, non-temporal) - one axis at a time from the defaults (a value
+ * replaces the current one only when at least 1% faster), keeps the fastest in the context for later
+ * steps, and reports the choice.  A tuned width of 128 lanes holds while a step's row count stays
+ * within 10% of the tuned step's; other steps use 256 (the solver checks the device's row count, so
+ * replayed graph steps follow it).
  * Results are bit-identical under every schedule; only the time changes.  Needs a step with
  * constraints; leaves that step's outputs untouched.  Environment variables ALENS_SOLVER_ENTRIES,
  * ALENS_BARRIER_MODE, ALENS_XCD_MAP, ALENS_SOLVER_MAX_BLOCKS, ALENS_GROUP_WIDTH and ALENS_STREAM_HINT
@@ -209,7 +211,8 @@ typedef struct {
     int32_t candidates;  /* schedules timed */
     float default_ms;    /* solve time of the default schedule */
     float best_ms;       /* solve time of the chosen schedule */
-    int32_t group_width; /* phase-B reduction-group width: 0 = automatic (256 lanes); not tuned */
+    int32_t group_width; /* phase-B reduction-group width: 0 = automatic (256 lanes), 128 (held for row
+                            counts within 10% of the tuned step's) */
     int32_t stream_hint; /* phase-B row streams: 0 = automatic (non-temporal loads when an iteration's
                             bytes exceed the 256 MiB last-level cache), 1 = cached, 2 = non-temporal */
 } alens_tune_report;
