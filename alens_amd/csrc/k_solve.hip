@@ -273,13 +273,19 @@ __device__ inline void fold_waves(const double *part, int nchunks, const uint32_
                 }
             }
         }
+        // a virtual wave that holds no chunk of the system (all of its lanes +0.0) folds to +0.0: its
+        // fold is skipped (the condition is the same for the whole wave)
+        const int nchs = sy < nsys && state[sy] == S_ACTIVE ? (int)(chunk_off[sy + 1] - chunk_off[sy]) : 0;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             double t[4];
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 double x = q < NS ? vs[v][q] : vx[v][q - NS];
-                x = q >= NS ? wave_fold<true>(x) : wave_fold<false>(x);
+                if (v * 64 < nchs)
+                    x = q >= NS ? wave_fold<true>(x) : wave_fold<false>(x);
+                else
+                    x = 0.0;
                 t[v] = x;
             }
             if (wl == 0 && sy < nsys) {
