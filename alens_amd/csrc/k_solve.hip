@@ -673,19 +673,73 @@ __device__ inline ChunkMeta chunk_meta(const SolveArgsT<T> &a, int c, int nch) {
     return m;
 }
 
+// Active chunks of many systems, compacted: s_cact[s] = chunks of the active systems before s, so the
+// groups of phase B walk only the chunks of systems still iterating (with fixed chunk ownership, a
+// round of chunks costs the same while any of its systems is active).  Wave 0 computes it from the
+// per-system states; the grid barrier before phase B publishes it to the block.
+__device__ inline void compact_chunks(const int *s_state, const uint32_t *s_choff, int ns, uint32_t *s_cact) {
+    static_assert(kMaxSys <= 4 * 64, "four systems per lane");
+    if (threadIdx.x >= 64) return;
+    const int l = threadIdx.x;
+    uint32_t cnt[4], sum = 0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int q = 4 * l + v;
+        cnt[v] = (q < ns && s_state[q] == S_ACTIVE) ? s_choff[q + 1] - s_choff[q] : 0u;
+        sum += cnt[v];
+    }
+    uint32_t inc = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(inc, o, 64);
+        if (l >= o) inc += t;
+    }
+    uint32_t run = inc - sum;
+    if (l == 0) s_cact[0] = 0u;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int q = 4 * l + v;
+        run += cnt[v];
+        if (q < ns) s_cact[q + 1] = run;
+    }
+}
+
 // phase B: constraints in 512-element chunks, one chunk per GL-lane group; chunk partials go to
 // part[q * nchunks + c].  NT: the row streams (geometry, rod ids, rhs) load non-temporally.
-template <int MODE, typename T, int GL, bool NT = false>
+// COMPACT: the groups walk the compacted chunks of the active systems (s_cact), each mapped back to its
+// system and chunk; a chunk's elements, order and partial slot are the same either way.
+template <int MODE, typename T, int GL, bool NT = false, bool COMPACT = false>
 __device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, const int *s_state, double *sh,
-                               const ChunkMeta (*s_cm)[kGrpB], int nch) {
+                               const ChunkMeta (*s_cm)[kGrpB], int nch, const uint32_t *s_cact = nullptr,
+                               const uint32_t *s_choff = nullptr, int ns = 0) {
     constexpr int kVPT = GroupShape<GL>::vpt, G = GroupShape<GL>::per_block;
     constexpr int NE = kVPT * kRedPasses;  // elements per lane
     const int g = threadIdx.x / GL, l = threadIdx.x % GL;
     const int ngroups = gridDim.x * G;
     const T dt = (T)a.dt;
     int r = 0;
-    for (int base = work_block(a.xcd_map) * G; base < nch; base += ngroups, ++r) {
-        const int c = base + g;
+    const int nloop = COMPACT ? (int)s_cact[ns] : nch;
+    for (int base = work_block(a.xcd_map) * G; base < nloop; base += ngroups, ++r) {
+        int c = base + g;
+        ChunkMeta cm{0u, 0u, -1};
+        if (COMPACT) {
+            if (c < nloop) {
+                // the active system holding compacted chunk c: s_cact[lo] <= c < s_cact[hi]
+                int lo = 0, hi = ns;
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if ((int)s_cact[mid] <= c) lo = mid; else hi = mid;
+                }
+                const uint32_t j = (uint32_t)c - s_cact[lo];
+                cm.sy = lo;
+                cm.k0 = a.con_off[lo] + j * kRedChunk;
+                const uint32_t k1 = cm.k0 + kRedChunk, kend = a.con_off[lo + 1];
+                cm.k1 = k1 < kend ? k1 : kend;
+                c = (int)(s_choff[lo] + j);
+            }
+        } else {
+            cm = r < kChunkRounds ? s_cm[r][g] : chunk_meta(a, c, nch);
+        }
         // per virtual lane: sums (s^T s, s^T y, y^T y, non-finite count), max residual, and for a warm
         // start the cold-start residual and its non-finite count
         double vs[5][kVPT], vx[2][kVPT];
@@ -694,7 +748,6 @@ __device__ inline void phase_B(const SolveArgsT<T> &a, const double *s_alpha, co
             vs[0][j] = vs[1][j] = vs[2][j] = vs[3][j] = vs[4][j] = 0.0;
             vx[0][j] = vx[1][j] = 0.0;
         }
-        const ChunkMeta cm = r < kChunkRounds ? s_cm[r][g] : chunk_meta(a, c, nch);
         const int sy = cm.sy < 0 ? 0 : cm.sy;
         const bool act = cm.sy >= 0 && s_state[sy] == S_ACTIVE;
         if (act) {
@@ -851,6 +904,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
     // each system's chunk range, fixed through the solve: the fold reads it from LDS instead of
     // starting every fold with a dependent global load
     __shared__ uint32_t s_choff[kMaxSys + 1];
+    __shared__ uint32_t s_cact[MANY ? kMaxSys + 1 : 1];
     for (int q = threadIdx.x; q <= ns; q += blockDim.x) s_choff[q] = a.chunk_off[q];
     // many systems, none with more chunks than a wave folds quickly: one wave per system in the fold
     __shared__ int s_big;
@@ -954,13 +1008,14 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
             s_alpha[q] = al;
         }
         __syncthreads();
+        if (MANY) compact_chunks(s_state, s_choff, ns, s_cact);  // read after the next grid barrier
         for (int it = 1; it <= a.max_ite; ++it) {
             phase_A<A_PROJ, T, AG, SPLIT>(a, s_alpha, s_state, rm0, split, s_help, s_prod, s_rmeta);
             if (!grid_barrier(a.bar, epoch, &s_flag, bm, bg)) return;
             if (wide || SPLIT)
-                phase_B<B_ITER, T, 256, NT>(a, s_alpha, s_state, sh, s_cm, nch);
+                phase_B<B_ITER, T, 256, NT, MANY>(a, s_alpha, s_state, sh, s_cm, nch, s_cact, s_choff, ns);
             else
-                phase_B<B_ITER, T, 128, NT>(a, s_alpha, s_state, sh, s_cm, nch);
+                phase_B<B_ITER, T, 128, NT, MANY>(a, s_alpha, s_state, sh, s_cm, nch, s_cact, s_choff, ns);
             if (!grid_barrier(a.bar, epoch, &s_flag, bm, bg)) return;
             {
                 // the step length of this iteration needs two of the three sums: s^T s / s^T y
@@ -994,6 +1049,7 @@ __global__ __launch_bounds__(kBlk) void k_bbpgd(SolveArgsT<T> a) {
             }
             __syncthreads();
             if (!s_any) break;
+            if (MANY) compact_chunks(s_state, s_choff, ns, s_cact);  // read after the next grid barrier
         }
     }
     if (blockIdx.x == 0) {
