@@ -1731,8 +1731,8 @@ alens_status alens_autotune(alens_ctx *ctx, alens_tune_report *rep_out) {
             if (s != ALENS_OK) break;
             ++tr.candidates;
             // a value replaces the current one only when it is clearly faster: the automatic
-            // values (entries, group width, stream hint) adapt to later steps' row counts, and a fixed
-            // value that ties with them on the tuned step can lose on later ones
+            // values (group width, stream hint) adapt to later steps' row counts, and a fixed value
+            // that ties with them on the tuned step can lose on later ones
             if (t < best * (1.0f - kTuneMargin)) {
                 best = t;
                 pick = v;
